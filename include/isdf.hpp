@@ -1,0 +1,103 @@
+// isdf.hpp — source-compatible drop-in for LibISDF's GPU entry point.
+//
+// Replaces include/isdf.hpp of LibISDF for the GPU build (-DGPU):
+//   include/isdf.hpp:37  void isdf::isdffunc::ISDF(double *d_psirow, Domain domain,
+//                          int nocc, int nstate, int nv1, int nc1, int nv2, int nc2,
+//                          int mu_points, double *d_thetaCol, int *piv,
+//                          int KmeansMaxIter_ISDF, bool check=false);
+// together with the Domain type (include/common/domain.hpp:8-36), the isdfOFS
+// log stream and ErrorHandling (src/common/global.cpp:5-11).
+//
+// Behaviour follows the reference at one rank: d_psirow is the caller's device
+// Ng x nstate column-major matrix, d_thetaCol the caller's device Ng x rk
+// matrix (column mu belongs to grid point piv[mu]), piv the caller's host
+// int[Ng].  The call blocks until the result is in place.  Errors are logged to
+// isdfOFS and thrown as std::runtime_error, as ErrorHandling does.
+//
+// Communicators: build with ISDF_WITH_MPI (and mpi.h on the include path) to
+// keep MPI_Comm members; the library runs one rank and throws if the
+// communicator holds more than one process.  Without it the members are
+// opaque integers.
+#pragma once
+
+#include <fstream>
+#include <stdexcept>
+#include <string>
+
+#ifdef ISDF_WITH_MPI
+#include <mpi.h>
+#endif
+
+namespace isdf {
+
+typedef int Int;
+typedef double Real;
+
+#ifdef ISDF_WITH_MPI
+typedef MPI_Comm Comm;
+#else
+typedef int Comm;  // opaque; 0 stands for the single-process world
+#endif
+
+// Minimal Point3 / Index3 with the accessors Domain users need
+// (LibISDF's tinyvec: operator[] / operator()).
+template <class F>
+struct Vec3 {
+  F v[3];
+  Vec3() : v{F(0), F(0), F(0)} {}
+  Vec3(F a, F b, F c) : v{a, b, c} {}
+  F& operator[](int i) { return v[i]; }
+  const F& operator[](int i) const { return v[i]; }
+  F& operator()(int i) { return v[i]; }
+  const F& operator()(int i) const { return v[i]; }
+  F* Data() { return v; }
+  const F* Data() const { return v; }
+};
+typedef Vec3<Real> Point3;
+typedef Vec3<Int> Index3;
+
+struct Domain {
+  Point3 length;
+  Point3 posStart;
+  Index3 numGrid;
+  Index3 numGridFine;
+  Comm comm;
+  Comm rowComm;
+  Comm colComm;
+
+  Domain()
+      : length(0.0, 0.0, 0.0),
+        posStart(0.0, 0.0, 0.0),
+        numGrid(0, 0, 0),
+        numGridFine(0, 0, 0),
+#ifdef ISDF_WITH_MPI
+        comm(MPI_COMM_WORLD),
+        rowComm(MPI_COMM_WORLD),
+        colComm(MPI_COMM_WORLD)
+#else
+        comm(0),
+        rowComm(0),
+        colComm(0)
+#endif
+  {
+  }
+
+  Real Volume() const { return length[0] * length[1] * length[2]; }
+  Int NumGridTotal() const { return numGrid[0] * numGrid[1] * numGrid[2]; }
+  Int NumGridTotalFine() const { return numGridFine[0] * numGridFine[1] * numGridFine[2]; }
+};
+
+// Log stream opened by the caller (the reference opens "isdf.<rank>").
+extern std::ofstream isdfOFS;
+
+// Logs msg to isdfOFS and throws std::runtime_error(msg).
+void ErrorHandling(const char* msg);
+
+namespace isdffunc {
+
+void ISDF(double* d_psirow, Domain domain, int nocc, int nstate, int nv1, int nc1, int nv2,
+          int nc2, int mu_points, double* d_thetaCol, int* piv, int KmeansMaxIter_ISDF,
+          bool check = false);
+
+}  // namespace isdffunc
+}  // namespace isdf

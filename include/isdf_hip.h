@@ -1,0 +1,126 @@
+/*
+ * isdf_hip.h — C ABI of the MI355X (gfx950) ISDF hot path.
+ *
+ * This is the boundary a drop-in binding calls.  It replaces the GPU ISDF of
+ * LibISDF:
+ *
+ *   include/isdf.hpp:37   void isdf::isdffunc::ISDF(double *d_psirow, Domain domain,
+ *                             int nocc, int nstate, int nv1, int nc1, int nv2, int nc2,
+ *                             int mu_points, double *d_thetaCol, int *piv,
+ *                             int KmeansMaxIter_ISDF, bool check=false);
+ *   src/common/isdf.cpp:132-209 (driver), src/cuda/isdf-gpu.cpp:9-403 (ISDF_GPU)
+ *
+ * with results that follow the reference CPU ISDF (src/common/isdf.cpp:8-127,
+ * "scalapack" basis + "Kmeans" points) at one rank.  Plain C types only.
+ *
+ * Buffers (all column-major, owned by the caller):
+ *   d_psi   device, Ng x nstate doubles, leading dimension ld_psi (0 -> Ng); read only
+ *   d_theta device, Ng x rk doubles, leading dimension Ng; written
+ *   h_piv   host, Ng ints; written (zeroed first, as src/common/isdf.cpp:174-175)
+ * Ng = ngrid[0]*ngrid[1]*ngrid[2], grid index g = ix + ngrid[0]*(iy + ngrid[1]*iz).
+ * rk = mu_points, or IRound(sqrt((nc1+nv1)(nc2+nv2)) * 8) when mu_points == 0
+ * (src/common/isdf.cpp:160-164); isdf_hip_rank() returns it.
+ *
+ * The call is synchronous: it returns after all device work on `stream`
+ * (a hipStream_t, NULL for the default stream) has finished.  One call at a
+ * time per process (calls are serialised internally).
+ */
+#ifndef ISDF_HIP_H
+#define ISDF_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  ISDF_ENOTSPD corresponds to the reference's
+ * ErrorHandling("Matrix is not positive definite.")
+ * (src/cpu/getbasis_scalapack.cpp:458-461, src/cuda/cusolver.cpp:136-140). */
+enum {
+  ISDF_OK = 0,
+  ISDF_EINVAL = 1,
+  ISDF_ENOTSPD = 2,
+  ISDF_EHIP = 3,
+  ISDF_ENOMEM = 4
+};
+
+/* Arithmetic of the Z C^T / C C^T contractions and of the Theta solve.
+ * FP64 is the drop-in path. */
+enum {
+  ISDF_PREC_FP64 = 0
+};
+
+typedef struct isdf_problem {
+  int ngrid[3];          /* Domain::numGrid (include/common/domain.hpp:12) */
+  double length[3];      /* Domain::length  (include/common/domain.hpp:10) */
+  int nocc, nstate;      /* pair set S1 = [nocc-nv1, nocc+nc1), S2 = [nocc-nv2, nocc+nc2) */
+  int nv1, nc1, nv2, nc2;
+  int mu_points;         /* 0 -> IRound rule */
+  int kmeans_max_iter;   /* KmeansMaxIter_ISDF */
+  int precision;         /* ISDF_PREC_* */
+  int reserved;
+  long long ld_psi;      /* 0 -> Ng */
+} isdf_problem;
+
+typedef struct isdf_report {
+  int rk;            /* columns of theta */
+  int rk_kmeans;     /* distinct K-means centers (src/common/utility.cpp:970-972) */
+  int kmeans_iters;  /* Lloyd iterations run */
+  int npt;           /* grid points with weight > 1e-8 * max weight */
+  int potrf_info;    /* 0, or the LAPACK-style index of the failing pivot */
+  int reserved;
+  double eps;        /* 1e-10 * trace(C C^T) / rk added to the diagonal */
+  /* Stage times in milliseconds (HIP events on the call's stream).  ms_potrf
+   * includes the preparation of the blocked solve operands; ms_potrs is the
+   * in-place triangular-solve kernel alone. */
+  double ms_weights, ms_kmeans, ms_select, ms_gather, ms_zct, ms_cct, ms_potrf, ms_potrs,
+      ms_total;
+} isdf_report;
+
+/* Optional intermediate outputs for validation; any pointer may be NULL. */
+typedef struct isdf_debug {
+  int* h_trace;        /* host int[kmeans_max_iter]: changed labels per iteration */
+  double* d_weights;   /* device double[Ng]: K-means weights */
+  int* d_labels;       /* device int[Ng]: final labels */
+  double* d_cct;       /* device double[rk*rk]: C C^T + eps I before the Cholesky */
+} isdf_debug;
+
+/* Full ISDF: weights -> K-means -> piv -> Psi(mu) -> Z C^T -> C C^T + eps ->
+ * potrf -> Theta = Z C^T (C C^T)^{-1} written into d_theta.
+ * rep and dbg may be NULL.  Returns an ISDF_* status; on failure
+ * isdf_hip_last_error() describes it. */
+int isdf_hip_solve(const isdf_problem* prob, const double* d_psi, double* d_theta, int* h_piv,
+                   void* stream, isdf_report* rep, const isdf_debug* dbg);
+
+/* rk the solve will use for this problem (the reference's rank rule). */
+int isdf_hip_rank(const isdf_problem* prob);
+
+/* Message of the last failing call on any thread ("" if none). */
+const char* isdf_hip_last_error(void);
+
+/* Result of the check=true validator. */
+typedef struct isdf_check {
+  double fit_error;      /* ||Z - Theta C||_F / ||Z||_F   (src/common/checkcode.cpp:62-166) */
+  double z_norm;         /* ||Z||_F */
+  double coulomb_error;  /* ||ijVkl - C^T M C||_F / ||ijVkl||_F (checkcode.cpp:168-312); -1 if skipped */
+  double coulomb_norm;   /* ||ijVkl||_F; -1 if skipped */
+} isdf_check;
+
+/* check=true validator on host copies of psi (Ng x nstate, ld_psi), theta
+ * (Ng x rk, ld Ng) and piv.  Replaces isdffunc::checkcode
+ * (src/common/checkcode.cpp:54-313).  The fit error streams over pair columns;
+ * the Coulomb part (do_coulomb != 0) holds (K1*K2 + rk) spectra of Ng complex
+ * values and is meant for small grids. */
+int isdf_hip_check(const isdf_problem* prob, const double* h_psi, const double* h_theta,
+                   const int* h_piv, int do_coulomb, isdf_check* out);
+
+/* Library version string. */
+const char* isdf_hip_version(void);
+
+/* Releases the cached device workspace of the calling device. */
+int isdf_hip_release_workspace(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISDF_HIP_H */

@@ -1,0 +1,173 @@
+"""libisdf_amd — MI355X-native ISDF hot path of LibISDF.
+
+Python mirror of the reference's GPU interface (include/isdf.hpp:37):
+
+    ISDF(d_psirow, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points,
+         d_thetaCol, piv, KmeansMaxIter_ISDF, check=False)
+
+The work happens in libisdf_hip.so (hand-written HIP kernels for gfx950 behind
+the C ABI of include/isdf_hip.h); this module only passes device pointers and
+sizes.  PyTorch supplies device memory and the current stream.  There is no
+CPU fallback: without the shared library every call raises.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (ISDF_EHIP, ISDF_EINVAL, ISDF_ENOMEM, ISDF_ENOTSPD, ISDF_OK,
+                   ISDF_PREC_FP64)
+
+__all__ = ["Domain", "ISDF", "isdf_rank", "solve", "check", "ISDF_PREC_FP64"]
+
+
+class Domain:
+    """Simulation cell, as include/common/domain.hpp:8-36 (one rank: no communicators)."""
+
+    def __init__(self, length=(0.0, 0.0, 0.0), numGrid=(0, 0, 0), posStart=(0.0, 0.0, 0.0),
+                 numGridFine=(0, 0, 0)):
+        self.length = tuple(float(x) for x in length)
+        self.posStart = tuple(float(x) for x in posStart)
+        self.numGrid = tuple(int(x) for x in numGrid)
+        self.numGridFine = tuple(int(x) for x in numGridFine)
+
+    def Volume(self):
+        return self.length[0] * self.length[1] * self.length[2]
+
+    def NumGridTotal(self):
+        return self.numGrid[0] * self.numGrid[1] * self.numGrid[2]
+
+
+def _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, max_iter,
+             precision=ISDF_PREC_FP64, ld_psi=0):
+    p = _lib.IsdfProblem()
+    for d in range(3):
+        p.ngrid[d] = int(domain.numGrid[d])
+        p.length[d] = float(domain.length[d])
+    p.nocc, p.nstate = int(nocc), int(nstate)
+    p.nv1, p.nc1, p.nv2, p.nc2 = int(nv1), int(nc1), int(nv2), int(nc2)
+    p.mu_points = int(mu_points)
+    p.kmeans_max_iter = int(max_iter)
+    p.precision = int(precision)
+    p.ld_psi = int(ld_psi)
+    return p
+
+
+def isdf_rank(nv1, nc1, nv2, nc2, mu_points):
+    """rk of the reference's rank rule (src/common/isdf.cpp:160-164)."""
+    p = _lib.IsdfProblem()
+    p.nv1, p.nc1, p.nv2, p.nc2, p.mu_points = nv1, nc1, nv2, nc2, mu_points
+    return _lib.lib().isdf_hip_rank(ctypes.byref(p))
+
+
+def _dev_ptr(x):
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        import torch
+        if not x.is_cuda:
+            raise ValueError("device buffer expected (a CUDA/HIP tensor)")
+        if x.dtype != torch.float64:
+            raise ValueError("float64 buffer expected")
+        if not x.is_contiguous():
+            raise ValueError("contiguous buffer expected")
+        return x.data_ptr()
+    raise TypeError("device buffer must be a torch tensor on the GPU or an int pointer")
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def _raise(rc):
+    msg = _lib.last_error()
+    if rc == ISDF_EINVAL:
+        raise ValueError(msg)
+    if rc == ISDF_ENOMEM:
+        raise MemoryError(msg)
+    # ISDF_ENOTSPD mirrors ErrorHandling("Matrix is not positive definite."),
+    # which throws std::runtime_error (src/common/global.cpp:7-11).
+    raise RuntimeError(msg)
+
+
+def solve(d_psi, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_theta, piv,
+          max_iter, stream=None, precision=ISDF_PREC_FP64, debug=None, ld_psi=0):
+    """Run the C-ABI solve.  piv: numpy int32 array of Ng entries (written).
+    debug: optional dict with any of 'trace' (numpy int32 array), 'weights',
+    'labels', 'cct' (device tensors).  Returns the report as a dict."""
+    L = _lib.lib()
+    p = _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, max_iter, precision, ld_psi)
+    ng = domain.NumGridTotal()
+    if not (isinstance(piv, np.ndarray) and piv.dtype == np.int32 and piv.flags.c_contiguous
+            and piv.size >= ng):
+        raise ValueError("piv must be a contiguous numpy int32 array of Ng entries")
+    rep = _lib.IsdfReport()
+    dbg = None
+    if debug:
+        dbg = _lib.IsdfDebug()
+        if "trace" in debug:
+            t = debug["trace"]
+            assert t.dtype == np.int32 and t.size >= max(max_iter, 1)
+            dbg.h_trace = t.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        for key in ("weights", "labels", "cct"):
+            if key in debug:
+                setattr(dbg, "d_" + key, debug[key].data_ptr())
+    rc = L.isdf_hip_solve(ctypes.byref(p), ctypes.c_void_p(_dev_ptr(d_psi)),
+                          ctypes.c_void_p(_dev_ptr(d_theta)),
+                          piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                          ctypes.c_void_p(_stream_ptr(stream)), ctypes.byref(rep),
+                          ctypes.byref(dbg) if dbg is not None else None)
+    if rc != ISDF_OK:
+        _raise(rc)
+    return rep.as_dict()
+
+
+def check(psi, theta, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, coulomb=True):
+    """check=true validator (src/common/checkcode.cpp:54-313) on host arrays:
+    psi (nstate, Ng), theta (rk, Ng) float64, piv int32.  Returns dict."""
+    L = _lib.lib()
+    p = _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, 0)
+    psi = np.ascontiguousarray(psi, np.float64)
+    theta = np.ascontiguousarray(theta, np.float64)
+    piv = np.ascontiguousarray(piv, np.int32)
+    out = _lib.IsdfCheck()
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = L.isdf_hip_check(ctypes.byref(p), psi.ctypes.data_as(dp), theta.ctypes.data_as(dp),
+                          piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(bool(coulomb)),
+                          ctypes.byref(out))
+    if rc != ISDF_OK:
+        _raise(rc)
+    return dict(fit_error=out.fit_error, z_norm=out.z_norm, coulomb_error=out.coulomb_error,
+                coulomb_norm=out.coulomb_norm)
+
+
+_validate = check
+
+
+def ISDF(d_psirow, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_thetaCol, piv,
+         KmeansMaxIter_ISDF, check=False, stream=None):
+    """Drop-in mirror of isdf::isdffunc::ISDF (GPU, include/isdf.hpp:37).
+
+    d_psirow: device float64 tensor holding the column-major Ng x nstate matrix
+    (a (nstate, Ng) C-contiguous tensor).  d_thetaCol: device float64 tensor of
+    Ng x rk column-major ((rk, Ng) C-contiguous), written.  piv: host int32
+    numpy array of Ng entries, written.  Raises RuntimeError with the
+    reference's message when the C C^T Cholesky fails.  With check=True the
+    fit and Coulomb errors are returned in the report (keys 'fit_error',
+    'coulomb_error'), as checkcode prints them.
+    """
+    rep = solve(d_psirow, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_thetaCol, piv,
+                KmeansMaxIter_ISDF, stream=stream)
+    if check:
+        import torch
+        torch.cuda.synchronize()
+        psi_h = d_psirow.detach().cpu().numpy()
+        th_h = d_thetaCol.detach().cpu().numpy()
+        rep.update(_validate(psi_h, th_h, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2,
+                             mu_points))
+    return rep

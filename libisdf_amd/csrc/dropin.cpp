@@ -1,0 +1,96 @@
+// C++ drop-in isdf::isdffunc::ISDF over the C ABI (include/isdf_hip.h).
+//
+// Mirrors the GPU driver of LibISDF, src/common/isdf.cpp:132-209: rank rule,
+// piv zeroing, the solve, the isdfOFS log lines, and the check=true validator
+// (src/common/checkcode.cpp, reached from isdf.cpp:193-206).  Unlike the
+// reference there is no row->column relayout: at one rank that relayout is the
+// identity, so Theta is solved directly in d_thetaCol.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/isdf.hpp"
+#include "../../include/isdf_hip.h"
+
+namespace isdf {
+
+std::ofstream isdfOFS;
+
+void ErrorHandling(const char* msg) {
+  isdfOFS << std::endl << "ERROR!" << std::endl << msg << std::endl << std::endl;
+  throw std::runtime_error(msg);
+}
+
+namespace isdffunc {
+
+void ISDF(double* d_psirow, Domain domain, int nocc, int nstate, int nv1, int nc1, int nv2,
+          int nc2, int mu_points, double* d_thetaCol, int* piv, int KmeansMaxIter_ISDF,
+          bool check) {
+#ifdef ISDF_WITH_MPI
+  int mpisize = 1;
+  MPI_Comm_size(domain.comm, &mpisize);
+  if (mpisize != 1) ErrorHandling("libisdf_hip runs one rank; the communicator has more.");
+#endif
+  isdf_problem p{};
+  for (int d = 0; d < 3; d++) {
+    p.ngrid[d] = domain.numGrid[d];
+    p.length[d] = domain.length[d];
+  }
+  p.nocc = nocc;
+  p.nstate = nstate;
+  p.nv1 = nv1;
+  p.nc1 = nc1;
+  p.nv2 = nv2;
+  p.nc2 = nc2;
+  p.mu_points = mu_points;
+  p.kmeans_max_iter = KmeansMaxIter_ISDF;
+  p.precision = ISDF_PREC_FP64;
+  p.ld_psi = 0;
+  const int rk = isdf_hip_rank(&p);
+  isdfOFS << "rk=" << rk << std::endl;
+
+  isdf_report rep{};
+  std::vector<int> trace((size_t)(KmeansMaxIter_ISDF > 0 ? KmeansMaxIter_ISDF : 1));
+  isdf_debug dbg{};
+  dbg.h_trace = trace.data();
+  int rc = isdf_hip_solve(&p, d_psirow, d_thetaCol, piv, nullptr, &rep, &dbg);
+  if (rc != ISDF_OK) ErrorHandling(isdf_hip_last_error());
+
+  // The K-means trace as utility.cpp:920-923 prints it.
+  for (int s = 0; s < rep.kmeans_iters; s++) isdfOFS << trace[(size_t)s] << " ";
+  isdfOFS << std::endl << "Converged in " << rep.kmeans_iters << " iterations." << std::endl;
+  isdfOFS << "Time for pivQR_ with Kmeans      = " << (rep.ms_weights + rep.ms_kmeans + rep.ms_select) * 1e-3
+          << " [s]" << std::endl;
+  isdfOFS << "Time for Z*C^T                   = " << rep.ms_zct * 1e-3 << " [s]" << std::endl;
+  isdfOFS << "Time for C*C^T                   = " << rep.ms_cct * 1e-3 << " [s]" << std::endl;
+  isdfOFS << "Time for Potrf                   = " << rep.ms_potrf * 1e-3 << " [s]" << std::endl;
+  isdfOFS << "Time for Trsm                    = " << rep.ms_potrs * 1e-3 << " [s]" << std::endl;
+  isdfOFS << "Time for ISDF time =" << rep.ms_total * 1e-3 << " [s]" << std::endl;
+
+  if (check) {
+    const long long ng = (long long)p.ngrid[0] * p.ngrid[1] * p.ngrid[2];
+    std::vector<double> psi((size_t)(ng * nstate)), theta((size_t)(ng * rk));
+    if (hipMemcpy(psi.data(), d_psirow, sizeof(double) * psi.size(), hipMemcpyDeviceToHost) !=
+            hipSuccess ||
+        hipMemcpy(theta.data(), d_thetaCol, sizeof(double) * theta.size(),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+      ErrorHandling("check: device to host copy failed");
+    isdf_check chk{};
+    rc = isdf_hip_check(&p, psi.data(), theta.data(), piv, 1, &chk);
+    if (rc != ISDF_OK) ErrorHandling(isdf_hip_last_error());
+    // checkcode.cpp:164-166 and :310-312.
+    isdfOFS << "sqrt((rhoij-rho^hat)^2) =" << chk.fit_error * chk.z_norm << std::endl;
+    isdfOFS << "sqrt(rhoij^2)= " << chk.z_norm << std::endl;
+    isdfOFS << "sqrt((rhoij-rho^hat)^2)/sqrt(rhoij^2)= " << chk.fit_error << std::endl;
+    isdfOFS << "sqrt((ijkl-ijkl^hat)^2)) =" << chk.coulomb_error * chk.coulomb_norm << std::endl;
+    isdfOFS << "sqrt(ijkl^2) =" << chk.coulomb_norm << std::endl;
+    isdfOFS << "sqrt((ijkl-ijkl^hat)^2)/sqrt(ijkl^2) =" << chk.coulomb_error << std::endl;
+  }
+}
+
+}  // namespace isdffunc
+}  // namespace isdf

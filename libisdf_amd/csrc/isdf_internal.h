@@ -1,0 +1,109 @@
+// Internal declarations shared by the HIP translation units of libisdf_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+namespace isdf_hip {
+
+// Records a message for isdf_hip_last_error(); returns `code`.
+int set_error(int code, const std::string& msg);
+
+#define ISDF_HIP_TRY(expr)                                                        \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::isdf_hip::set_error(3, std::string(#expr " failed: ") +            \
+                                          hipGetErrorString(e_));                 \
+  } while (0)
+
+#define ISDF_HIP_LAUNCHED(name)                                                   \
+  do {                                                                            \
+    hipError_t e_ = hipGetLastError();                                            \
+    if (e_ != hipSuccess)                                                         \
+      return ::isdf_hip::set_error(3, std::string("launch of " name " failed: ") + \
+                                          hipGetErrorString(e_));                 \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// K-means point selection (kmeans.hip).  All FP64 arithmetic there is compiled
+// with contraction off so that it reproduces the CPU reference bit for bit.
+// ---------------------------------------------------------------------------
+
+// Device buffers of one K-means run.  n = Ng points, rkmax = initial rk.
+struct KmeansBufs {
+  int n, rkmax, nchunk_sort, nchunk_final, chunk_sort, chunk_final, nblk_flag;
+  double* w;         // [n]   density weights
+  uint8_t* flags;    // [n]   threshold / center flags
+  int* index;        // [n]   points above the threshold (ascending)
+  int* gidx;         // [n]   grid index of local point i
+  double *X, *Y, *Z; // [n]   point coordinates
+  double* WL;        // [n]   point weights
+  int* label;        // [n]
+  int* sorted;       // [n]   point indices sorted stably by label
+  int* bcount;       // [nblk_flag+1]
+  int* hist;         // [nchunk_sort * rkmax]  chunk-major label histogram
+  int* cnt_pts;      // [rkmax]
+  int* base;         // [rkmax+1]
+  int* map;          // [rkmax]
+  double *cx, *cy, *cz;        // [rkmax] centroids
+  double *sw, *sx, *sy, *sz;   // [rkmax] centroid sums
+  double* pbest;     // [nchunk_final * rkmax]
+  int* pidx;         // [nchunk_final * rkmax]
+  int* imin;         // [rkmax]
+  int* centers;      // [rkmax]
+  int* trace;        // [max_iter]
+  int* scal;         // small device scalars, see KS_* below
+};
+
+enum {
+  KS_RK = 0,       // current number of centroids
+  KS_NPT = 1,      // points above the threshold
+  KS_CHANGED = 2,  // labels changed in the current iteration
+  KS_ACTIVE = 3,   // 1 while the K-means loop should continue
+  KS_ITERS = 4,    // iterations done
+  KS_RKU = 5,      // distinct centers after unique()
+  KS_MAXW = 6,     // (2 ints) bit pattern of max weight
+  KS_COUNT = 8
+};
+
+size_t kmeans_bytes(int n, int rkmax, int max_iter);
+void kmeans_carve(KmeansBufs* b, void* base, int n, int rkmax, int max_iter);
+
+// Weights w(r) = (sum_{S2} psi^2)(sum_{S1} psi^2) for all grid points.
+int launch_weights(const double* psi, long long ld, int ng, int s1lo, int s1hi, int s2lo,
+                   int s2hi, double* w, hipStream_t st);
+
+// Threshold, point setup, initial centroids.  Leaves scal[KS_NPT] on device and
+// returns it through *npt_host (one synchronisation).
+int kmeans_prepare(KmeansBufs* b, const int ngrid[3], const double length[3], double df_tol,
+                   int* npt_host, hipStream_t st);
+int kmeans_setup_points(KmeansBufs* b, const int ngrid[3], const double length[3], int npt,
+                        int rk, hipStream_t st);
+// The Lloyd loop; returns the number of iterations through *iters_host.
+int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* iters_host,
+                   hipStream_t st);
+// Final nearest points, unique(), and the full pivot list piv_dev[n].
+int kmeans_finish(KmeansBufs* b, int* piv_dev, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// Basis construction (basis.hip)
+// ---------------------------------------------------------------------------
+int launch_gather_psimu(const double* psi, long long ld, int nstate, const int* piv, int rk,
+                        double* psimuT, hipStream_t st);
+int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, int rk, int s1,
+               int K1, int s2, int K2, bool dual, double* C, long long ldc, hipStream_t st);
+int launch_cct(const double* C, long long ldc, const int* piv, int rk, double* P,
+               double* eps_dev, hipStream_t st);
+int launch_potrf(double* P, int rk, double* Dinv, int* info_dev, hipStream_t st);
+int launch_make_fg(const double* L, int rk, const double* Dinv, double* Ft, double* Gt,
+                   hipStream_t st);
+int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* Ft,
+                 const double* Gt, hipStream_t st);
+
+constexpr int kNB = 64;  // block size of the Cholesky and of the triangular solves
+
+}  // namespace isdf_hip

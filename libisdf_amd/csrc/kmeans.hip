@@ -1,0 +1,801 @@
+// Density-weighted K-means interpolation-point selection for gfx950.
+//
+// Reproduces KMEAN of LibISDF (src/common/utility.cpp:631-988) at one rank, bit
+// for bit: every FP64 expression keeps the reference's evaluation order and
+// rounding (no FMA contraction: this file is built with -ffp-contract=off and
+// the pragma below), and every order-dependent sum runs in the reference's
+// order.  The parallel structure is the GPU's own:
+//
+//  * weights: one lane per grid point, states streamed in ascending order
+//    (coalesced along the column-major grid; HBM-bound);
+//  * assignment: distance + first-minimum argmin fused, the distance matrix is
+//    never stored; centroids are wave-uniform so they come through the scalar
+//    cache;
+//  * centroid update: a stable counting sort of point indices by label (chunk
+//    histograms, scans, wave-ballot ranking), then one wave per cluster sums
+//    its members in ascending point order: the same sequence of roundings as
+//    the reference's single loop over points (utility.cpp:816-822);
+//  * the loop runs on the device behind an "active" flag, so the host only
+//    looks at convergence every few iterations.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "isdf_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace isdf_hip {
+
+namespace {
+
+constexpr int kSortChunk = 2048;   // points per histogram/scatter chunk
+constexpr int kFlagTile = 2048;    // elements per compaction block (256 x 8)
+constexpr int kMaxRk = 8192;       // LDS-resident label histograms and sort
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+  unsigned lane = __lane_id();
+  return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+// Reference coordinates, utility.cpp:711-713 (points) and :758-760 (centroids).
+__device__ __forceinline__ void grid_coords(int tmp, int n0, int n1, double dx0, double dx1,
+                                            double dx2, double* x, double* y, double* z) {
+  double gx = (double)(tmp % n0) * dx0;
+  double gy = ((double)(tmp % (n0 * n1)) - gx / dx0) / (double)n0 * dx1;
+  double gz = (((double)tmp - gx / dx0) - gy / dx1 * (double)n0) / (double)(n0 * n1) * dx2;
+  *x = gx;
+  *y = gy;
+  *z = gz;
+}
+
+// Distance of pdist2, utility.cpp:609-612: ((d1*d1 + d2*d2) + d3*d3).
+__device__ __forceinline__ double dist2(double px, double py, double pz, double qx, double qy,
+                                        double qz) {
+  double d1 = px - qx;
+  double d2 = py - qy;
+  double d3 = pz - qz;
+  double s = d1 * d1;
+  s = s + d2 * d2;
+  s = s + d3 * d3;
+  return s;
+}
+
+// Block-wide exclusive scan of one int per thread (blockDim.x multiple of 64,
+// at most 1024).  Returns the exclusive prefix; *total gets the block sum.
+__device__ int block_excl_scan(int v, int* total) {
+  __shared__ int wsum[17];
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int x = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    int y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  int nw = blockDim.x >> 6;
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int i = 0; i < nw; i++) {
+      int t = wsum[i];
+      wsum[i] = s;
+      s += t;
+    }
+    wsum[16] = s;
+  }
+  __syncthreads();
+  int res = x - v + wsum[wave];
+  *total = wsum[16];
+  __syncthreads();
+  return res;
+}
+
+// ---------------------------------------------------------------------------
+// Weights, src/cpu/getpoints_Kmeans_MPI.cpp:37-45.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_weights(const double* __restrict__ psi, long long ld,
+                                                 int ng, int lo, int hi, int s1lo, int s1hi,
+                                                 int s2lo, int s2hi, double* __restrict__ w) {
+  long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ng) return;
+  double sI = 0.0, sJ = 0.0;  // sI over S2, sJ over S1 (reference names)
+  const double* p = psi + r;
+#pragma unroll 4
+  for (int i = lo; i < hi; i++) {
+    double v = p[(long long)i * ld];
+    double q = v * v;
+    if (i >= s2lo && i < s2hi) sI = sI + q;
+    if (i >= s1lo && i < s1hi) sJ = sJ + q;
+  }
+  double acc = 0.0;
+  acc = acc + sI * sJ;
+  w[r] = acc;
+}
+
+// findMax (numvec_impl.hpp:259-266) as an atomic max over IEEE bit patterns:
+// all weights are >= +0, where the bit order is the numeric order.
+__global__ __launch_bounds__(256) void k_maxw(const double* __restrict__ w, int n,
+                                              unsigned long long* maxbits) {
+  unsigned long long m = 0ull;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double v = w[i];
+    unsigned long long b = (v > 0.0) ? (unsigned long long)__double_as_longlong(v) : 0ull;
+    m = b > m ? b : m;
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    unsigned long long o = __shfl_xor(m, d);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(maxbits, m);
+}
+
+// flags[i] = w[i] > df_tol * maxW (utility.cpp:650-659).
+__global__ __launch_bounds__(256) void k_thresh(const double* __restrict__ w, int n,
+                                                const unsigned long long* maxbits, double df_tol,
+                                                uint8_t* __restrict__ flags) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double maxW = __longlong_as_double((long long)*maxbits);
+  double thr = df_tol * maxW;
+  flags[i] = (w[i] > thr) ? 1 : 0;
+}
+
+// Ordered compaction: count, scan, scatter of indices whose flag == want.
+__global__ __launch_bounds__(256) void k_flag_count(const uint8_t* __restrict__ flags, int n,
+                                                    int want, int* __restrict__ bcount) {
+  long long i0 = (long long)blockIdx.x * kFlagTile + threadIdx.x * 8;
+  int c = 0;
+  for (int k = 0; k < 8; k++) {
+    long long i = i0 + k;
+    if (i < n && flags[i] == want) c++;
+  }
+  int tot;
+  block_excl_scan(c, &tot);
+  if (threadIdx.x == 0) bcount[blockIdx.x] = tot;
+}
+
+// In-place exclusive scan of a[0..m) by one block; a[m] receives the total.
+__global__ __launch_bounds__(1024) void k_scan_single(int* a, int m) {
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int off = 0; off < m; off += blockDim.x) {
+    int i = off + threadIdx.x;
+    int v = (i < m) ? a[i] : 0;
+    int tot;
+    int ex = block_excl_scan(v, &tot);
+    if (i < m) a[i] = ex + carry;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a[m] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_flag_scatter(const uint8_t* __restrict__ flags, int n,
+                                                      int want, const int* __restrict__ bexcl,
+                                                      int base, int* __restrict__ out) {
+  long long i0 = (long long)blockIdx.x * kFlagTile + threadIdx.x * 8;
+  int c = 0;
+  for (int k = 0; k < 8; k++) {
+    long long i = i0 + k;
+    if (i < n && flags[i] == want) c++;
+  }
+  int tot;
+  int pos = base + bexcl[blockIdx.x] + block_excl_scan(c, &tot);
+  for (int k = 0; k < 8; k++) {
+    long long i = i0 + k;
+    if (i < n && flags[i] == want) out[pos++] = (int)i;
+  }
+}
+
+// Local points, utility.cpp:707-715.  With one rank nptLocal = n; entries of
+// `index` past npt read the zero-initialised tail of the reference's vector,
+// i.e. grid point 0 (utility.cpp:648, :659, :708).
+__global__ __launch_bounds__(256) void k_setup_points(int n, int npt, const int* __restrict__ index,
+                                                      const double* __restrict__ w, int n0, int n1,
+                                                      double dx0, double dx1, double dx2,
+                                                      int* __restrict__ gidx, double* __restrict__ X,
+                                                      double* __restrict__ Y, double* __restrict__ Z,
+                                                      double* __restrict__ WL) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int tmp = (npt == n) ? i : ((i < npt) ? index[i] : 0);
+  gidx[i] = tmp;
+  WL[i] = w[tmp];
+  double x, y, z;
+  grid_coords(tmp, n0, n1, dx0, dx1, dx2, &x, &y, &z);
+  X[i] = x;
+  Y[i] = y;
+  Z[i] = z;
+}
+
+// Initial centroids = the first rk points above the threshold (utility.cpp:750-762).
+__global__ __launch_bounds__(256) void k_init_centroids(int rk, int n, int npt,
+                                                        const int* __restrict__ index, int n0,
+                                                        int n1, double dx0, double dx1, double dx2,
+                                                        double* cx, double* cy, double* cz,
+                                                        int* scal) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c == 0) scal[KS_RK] = rk;
+  if (c >= rk) return;
+  int tmp = (npt == n) ? c : index[c];
+  double x, y, z;
+  grid_coords(tmp, n0, n1, dx0, dx1, dx2, &x, &y, &z);
+  cx[c] = x;
+  cy[c] = y;
+  cz[c] = z;
+}
+
+// Fused pdist2 + findMin(Dim=1) (utility.cpp:598-615, :535-562): first index
+// of the minimum over the current centroids.  Optionally counts changed labels.
+__global__ __launch_bounds__(256) void k_assign(int n, const double* __restrict__ X,
+                                                const double* __restrict__ Y,
+                                                const double* __restrict__ Z,
+                                                const double* __restrict__ cx,
+                                                const double* __restrict__ cy,
+                                                const double* __restrict__ cz,
+                                                int* __restrict__ label, int* scal,
+                                                int count_changes) {
+  if (count_changes && scal[KS_ACTIVE] == 0) return;
+  const int rk = scal[KS_RK];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool valid = i < n;
+  int ii = valid ? i : 0;
+  double px = X[ii], py = Y[ii], pz = Z[ii];
+  double best = dist2(px, py, pz, cx[0], cy[0], cz[0]);
+  int bi = 0;
+  int c = 1;
+  for (; c + 3 < rk; c += 4) {
+    double d0 = dist2(px, py, pz, cx[c], cy[c], cz[c]);
+    double d1 = dist2(px, py, pz, cx[c + 1], cy[c + 1], cz[c + 1]);
+    double d2 = dist2(px, py, pz, cx[c + 2], cy[c + 2], cz[c + 2]);
+    double d3 = dist2(px, py, pz, cx[c + 3], cy[c + 3], cz[c + 3]);
+    if (d0 < best) { best = d0; bi = c; }
+    if (d1 < best) { best = d1; bi = c + 1; }
+    if (d2 < best) { best = d2; bi = c + 2; }
+    if (d3 < best) { best = d3; bi = c + 3; }
+  }
+  for (; c < rk; c++) {
+    double d = dist2(px, py, pz, cx[c], cy[c], cz[c]);
+    if (d < best) { best = d; bi = c; }
+  }
+  bool changed = false;
+  if (valid) {
+    if (count_changes) changed = (label[i] != bi);
+    label[i] = bi;
+  }
+  if (count_changes) {
+    unsigned long long m = __ballot(changed);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&scal[KS_CHANGED], (int)__popcll(m));
+  }
+}
+
+// Stable counting sort of point indices by label: chunk histograms.
+__global__ __launch_bounds__(256) void k_sort_hist(int n, const int* __restrict__ label,
+                                                   int* __restrict__ hist, const int* scal) {
+  extern __shared__ int h[];
+  if (scal[KS_ACTIVE] == 0) return;
+  const int rk = scal[KS_RK];
+  for (int c = threadIdx.x; c < rk; c += blockDim.x) h[c] = 0;
+  __syncthreads();
+  int i0 = blockIdx.x * kSortChunk, i1 = min(n, i0 + kSortChunk);
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) atomicAdd(&h[label[i]], 1);
+  __syncthreads();
+  int* out = hist + (long long)blockIdx.x * rk;
+  for (int c = threadIdx.x; c < rk; c += blockDim.x) out[c] = h[c];
+}
+
+// Per-cluster exclusive scan over chunks (chunk-major layout); totals -> cnt_pts.
+__global__ __launch_bounds__(256) void k_sort_scan_chunks(int nchunk, int* __restrict__ hist,
+                                                          int* __restrict__ cnt_pts,
+                                                          const int* scal) {
+  if (scal[KS_ACTIVE] == 0) return;
+  const int rk = scal[KS_RK];
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= rk) return;
+  int s = 0;
+  for (int b = 0; b < nchunk; b++) {
+    int* p = hist + (long long)b * rk + c;
+    int t = *p;
+    *p = s;
+    s += t;
+  }
+  cnt_pts[c] = s;
+}
+
+// Exclusive scan of the cluster sizes -> cluster start offsets (one block).
+__global__ __launch_bounds__(1024) void k_sort_scan_clusters(const int* __restrict__ cnt_pts,
+                                                             int* __restrict__ base,
+                                                             const int* scal) {
+  __shared__ int carry;
+  if (scal[KS_ACTIVE] == 0) return;
+  const int rk = scal[KS_RK];
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int off = 0; off < rk; off += blockDim.x) {
+    int c = off + threadIdx.x;
+    int v = (c < rk) ? cnt_pts[c] : 0;
+    int tot;
+    int ex = block_excl_scan(v, &tot);
+    if (c < rk) base[c] = ex + carry;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+}
+
+// Stable scatter: one wave per chunk walks its points in order; lanes with the
+// same label are ranked with ballots, so equal labels keep ascending order.
+__global__ __launch_bounds__(64) void k_sort_scatter(int n, const int* __restrict__ label,
+                                                     const int* __restrict__ hist,
+                                                     const int* __restrict__ base,
+                                                     int* __restrict__ sorted, const int* scal) {
+  extern __shared__ int run[];
+  if (scal[KS_ACTIVE] == 0) return;
+  const int rk = scal[KS_RK];
+  const int b = blockIdx.x;
+  const int* hx = hist + (long long)b * rk;
+  for (int c = threadIdx.x; c < rk; c += 64) run[c] = base[c] + hx[c];
+  __syncthreads();
+  const unsigned long long below = lanemask_lt();
+  int i0 = b * kSortChunk, i1 = min(n, i0 + kSortChunk);
+  for (int s = i0; s < i1; s += 64) {
+    int i = s + (int)threadIdx.x;
+    bool valid = i < i1;
+    int L = valid ? label[i] : -1;
+    unsigned long long todo = __ballot(valid);
+    int pos = 0;
+    while (todo) {
+      int leader = __ffsll((long long)todo) - 1;
+      int Lr = __shfl(L, leader);
+      bool mine = valid && (L == Lr);
+      unsigned long long m = __ballot(mine);
+      int r0 = run[Lr];
+      if (mine) pos = r0 + __popcll(m & below);
+      __builtin_amdgcn_wave_barrier();
+      if ((int)threadIdx.x == leader) run[Lr] = r0 + __popcll(m);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      todo &= ~m;
+    }
+    if (valid) sorted[pos] = i;
+  }
+}
+
+// Centroid sums, utility.cpp:816-822: one wave per cluster; members in
+// ascending point order; lanes 0..3 carry the four sequential sums.
+__global__ __launch_bounds__(256) void k_centroid_sums(const int* __restrict__ sorted,
+                                                       const int* __restrict__ base,
+                                                       const int* __restrict__ cnt_pts,
+                                                       const double* __restrict__ WL,
+                                                       const double* __restrict__ X,
+                                                       const double* __restrict__ Y,
+                                                       const double* __restrict__ Z,
+                                                       double* __restrict__ sw,
+                                                       double* __restrict__ sx,
+                                                       double* __restrict__ sy,
+                                                       double* __restrict__ sz, const int* scal) {
+  __shared__ double buf[4][4][64];
+  if (scal[KS_ACTIVE] == 0) return;
+  const int rk = scal[KS_RK];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + wave;
+  if (c >= rk) return;
+  const int s0 = base[c], m = cnt_pts[c];
+  double acc = 0.0;
+  for (int off = 0; off < m; off += 64) {
+    int k = off + lane;
+    if (k < m) {
+      int i = sorted[s0 + k];
+      double wv = WL[i];
+      buf[wave][0][lane] = wv;
+      buf[wave][1][lane] = wv * X[i];
+      buf[wave][2][lane] = wv * Y[i];
+      buf[wave][3][lane] = wv * Z[i];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int cnt = min(64, m - off);
+    if (lane < 4) {
+      const double* q = buf[wave][lane];
+      for (int j = 0; j < cnt; j++) acc = acc + q[j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (lane == 0) sw[c] = acc;
+  if (lane == 1) sx[c] = acc;
+  if (lane == 2) sy[c] = acc;
+  if (lane == 3) sz[c] = acc;
+}
+
+// Division and empty-cluster compaction, utility.cpp:831-854 (swap-with-last,
+// re-testing the slot that received the last cluster).  One block.
+__global__ __launch_bounds__(1024) void k_update(const double* __restrict__ sw,
+                                                 const double* __restrict__ sx,
+                                                 const double* __restrict__ sy,
+                                                 const double* __restrict__ sz,
+                                                 double* __restrict__ cx, double* __restrict__ cy,
+                                                 double* __restrict__ cz, int* __restrict__ map,
+                                                 int* scal) {
+  __shared__ int nempty, rknew;
+  if (scal[KS_ACTIVE] == 0) return;
+  const int rk = scal[KS_RK];
+  if (threadIdx.x == 0) nempty = 0;
+  __syncthreads();
+  int e = 0;
+  for (int c = threadIdx.x; c < rk; c += blockDim.x) e += (sw[c] != 0.0) ? 0 : 1;
+  if (e) atomicAdd(&nempty, e);
+  __syncthreads();
+  if (nempty == 0) {
+    for (int c = threadIdx.x; c < rk; c += blockDim.x) {
+      double d = sw[c];
+      cx[c] = sx[c] / d;
+      cy[c] = sy[c] / d;
+      cz[c] = sz[c] / d;
+    }
+    if (threadIdx.x == 0) scal[KS_CHANGED] = 0;
+    return;
+  }
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < rk; c++) map[c] = c;
+    int r = rk;
+    int i = 0;
+    while (i < r) {
+      if (sw[map[i]] != 0.0) {
+        i++;
+      } else {
+        r--;
+        map[i] = map[r];
+      }
+    }
+    rknew = r;
+  }
+  __syncthreads();
+  const int r = rknew;
+  for (int c = threadIdx.x; c < r; c += blockDim.x) {
+    int src = map[c];
+    double d = sw[src];
+    cx[c] = sx[src] / d;
+    cy[c] = sy[src] / d;
+    cz[c] = sz[src] / d;
+  }
+  if (threadIdx.x == 0) {
+    scal[KS_RK] = r;
+    scal[KS_CHANGED] = 0;
+  }
+}
+
+// Loop control, utility.cpp:813 and :906-921.
+__global__ void k_control(int s, int max_iter, double maxF, int* trace, int* scal) {
+  if (scal[KS_ACTIVE] == 0) return;
+  int f = scal[KS_CHANGED];
+  trace[s] = f;
+  scal[KS_ITERS] = s + 1;
+  if (!((double)f > maxF && s + 1 < max_iter)) scal[KS_ACTIVE] = 0;
+}
+
+// findMin(Dim=0), utility.cpp:926-945: per centroid, the first point at the
+// minimum distance.  Lanes own centroids; a block walks one chunk of points.
+__global__ __launch_bounds__(256) void k_final_partial(int n, int chunk,
+                                                       const double* __restrict__ X,
+                                                       const double* __restrict__ Y,
+                                                       const double* __restrict__ Z,
+                                                       const double* __restrict__ cx,
+                                                       const double* __restrict__ cy,
+                                                       const double* __restrict__ cz,
+                                                       double* __restrict__ pbest,
+                                                       int* __restrict__ pidx, const int* scal) {
+  const int rk = scal[KS_RK];
+  if ((int)(blockIdx.y * 256) >= rk) return;
+  const int c = blockIdx.y * 256 + threadIdx.x;
+  const bool ok = c < rk;
+  const int cc = ok ? c : 0;
+  const double qx = cx[cc], qy = cy[cc], qz = cz[cc];
+  const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  double best = dist2(X[i0], Y[i0], Z[i0], qx, qy, qz);
+  int bi = i0;
+  for (int i = i0 + 1; i < i1; i++) {
+    double d = dist2(X[i], Y[i], Z[i], qx, qy, qz);
+    if (d < best) { best = d; bi = i; }
+  }
+  if (ok) {
+    pbest[(long long)blockIdx.x * rk + c] = best;
+    pidx[(long long)blockIdx.x * rk + c] = bi;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_final_reduce(int nchunk, const double* __restrict__ pbest,
+                                                      const int* __restrict__ pidx,
+                                                      const int* __restrict__ gidx,
+                                                      int* __restrict__ imin, const int* scal) {
+  const int rk = scal[KS_RK];
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= rk) return;
+  double best = pbest[c];
+  int bi = pidx[c];
+  for (int b = 1; b < nchunk; b++) {
+    double v = pbest[(long long)b * rk + c];
+    if (v < best) {
+      best = v;
+      bi = pidx[(long long)b * rk + c];
+    }
+  }
+  imin[c] = gidx[bi];
+}
+
+// unique(), utility.cpp:617-629: bitonic sort in LDS, then drop repeats.
+__global__ __launch_bounds__(1024) void k_unique(const int* __restrict__ imin,
+                                                 int* __restrict__ centers, int* scal) {
+  extern __shared__ int s[];
+  const int rk = scal[KS_RK];
+  int P = 1;
+  while (P < rk) P <<= 1;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) s[i] = (i < rk) ? imin[i] : 0x7fffffff;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        int l = i ^ j;
+        if (l > i) {
+          int a = s[i], b = s[l];
+          bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            s[i] = b;
+            s[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int off = 0; off < rk; off += blockDim.x) {
+    int i = off + threadIdx.x;
+    int keep = (i < rk && (i == 0 || s[i] != s[i - 1])) ? 1 : 0;
+    int tot;
+    int ex = block_excl_scan(keep, &tot);
+    if (keep) centers[carry + ex] = s[i];
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) scal[KS_RKU] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_mark_centers(const int* __restrict__ centers,
+                                                      uint8_t* __restrict__ flags,
+                                                      int* __restrict__ piv, const int* scal) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= scal[KS_RKU]) return;
+  int g = centers[k];
+  flags[g] = 1;
+  piv[k] = g;
+}
+
+__global__ void k_fill_u8(uint8_t* p, long long n, uint8_t v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_i32(int* p, long long n, int v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+// Ordered compaction of the indices whose flag equals `want` into out[base...].
+// Returns the count through the device slot `count_dev` (bcount[nblk]).
+int compact_flags(KmeansBufs* b, int want, int base, int* out, hipStream_t st) {
+  const int nblk = b->nblk_flag;
+  hipLaunchKernelGGL(k_flag_count, dim3(nblk), dim3(256), 0, st, b->flags, b->n, want, b->bcount);
+  ISDF_HIP_LAUNCHED("k_flag_count");
+  hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, b->bcount, nblk);
+  ISDF_HIP_LAUNCHED("k_scan_single");
+  if (out) {
+    hipLaunchKernelGGL(k_flag_scatter, dim3(nblk), dim3(256), 0, st, b->flags, b->n, want,
+                       b->bcount, base, out);
+    ISDF_HIP_LAUNCHED("k_flag_scatter");
+  }
+  return 0;
+}
+
+}  // namespace
+
+size_t kmeans_bytes(int n, int rkmax, int max_iter) {
+  KmeansBufs b;
+  kmeans_carve(&b, nullptr, n, rkmax, max_iter);
+  return (size_t)(uintptr_t)b.scal + 64;
+}
+
+void kmeans_carve(KmeansBufs* b, void* basep, int n, int rkmax, int max_iter) {
+  uintptr_t p = (uintptr_t)basep;
+  auto take = [&](size_t bytes) {
+    uintptr_t r = p;
+    p += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  b->n = n;
+  b->rkmax = rkmax;
+  b->chunk_sort = kSortChunk;
+  b->nchunk_sort = (int)cdiv(n, kSortChunk);
+  int tiles = (int)cdiv(rkmax, 256);
+  int want = std::max(1, 2048 / tiles);
+  int chunk = std::max(256, (int)cdiv(n, want));
+  b->chunk_final = chunk;
+  b->nchunk_final = (int)cdiv(n, chunk);
+  b->nblk_flag = (int)cdiv(n, kFlagTile);
+  b->w = (double*)take(sizeof(double) * n);
+  b->flags = (uint8_t*)take((size_t)n);
+  b->index = (int*)take(sizeof(int) * n);
+  b->gidx = (int*)take(sizeof(int) * n);
+  b->X = (double*)take(sizeof(double) * n);
+  b->Y = (double*)take(sizeof(double) * n);
+  b->Z = (double*)take(sizeof(double) * n);
+  b->WL = (double*)take(sizeof(double) * n);
+  b->label = (int*)take(sizeof(int) * n);
+  b->sorted = (int*)take(sizeof(int) * n);
+  b->bcount = (int*)take(sizeof(int) * (b->nblk_flag + 1));
+  b->hist = (int*)take(sizeof(int) * (size_t)b->nchunk_sort * rkmax);
+  b->cnt_pts = (int*)take(sizeof(int) * rkmax);
+  b->base = (int*)take(sizeof(int) * (rkmax + 1));
+  b->map = (int*)take(sizeof(int) * rkmax);
+  b->cx = (double*)take(sizeof(double) * rkmax);
+  b->cy = (double*)take(sizeof(double) * rkmax);
+  b->cz = (double*)take(sizeof(double) * rkmax);
+  b->sw = (double*)take(sizeof(double) * rkmax);
+  b->sx = (double*)take(sizeof(double) * rkmax);
+  b->sy = (double*)take(sizeof(double) * rkmax);
+  b->sz = (double*)take(sizeof(double) * rkmax);
+  b->pbest = (double*)take(sizeof(double) * (size_t)b->nchunk_final * rkmax);
+  b->pidx = (int*)take(sizeof(int) * (size_t)b->nchunk_final * rkmax);
+  b->imin = (int*)take(sizeof(int) * rkmax);
+  b->centers = (int*)take(sizeof(int) * rkmax);
+  b->trace = (int*)take(sizeof(int) * std::max(1, max_iter));
+  b->scal = (int*)take(sizeof(int) * KS_COUNT);
+}
+
+int launch_weights(const double* psi, long long ld, int ng, int s1lo, int s1hi, int s2lo,
+                   int s2hi, double* w, hipStream_t st) {
+  int lo = std::min(s1lo, s2lo), hi = std::max(s1hi, s2hi);
+  hipLaunchKernelGGL(k_weights, dim3(cdiv(ng, 256)), dim3(256), 0, st, psi, ld, ng, lo, hi, s1lo,
+                     s1hi, s2lo, s2hi, w);
+  ISDF_HIP_LAUNCHED("k_weights");
+  return 0;
+}
+
+int kmeans_prepare(KmeansBufs* b, const int ngrid[3], const double length[3], double df_tol,
+                   int* npt_host, hipStream_t st) {
+  (void)ngrid;
+  (void)length;
+  const int n = b->n;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, b->scal, (long long)KS_COUNT, 0);
+  ISDF_HIP_LAUNCHED("k_fill_i32");
+  unsigned long long* maxbits = (unsigned long long*)(b->scal + KS_MAXW);
+  hipLaunchKernelGGL(k_maxw, dim3(std::min(cdiv(n, 256), 2048u)), dim3(256), 0, st, b->w, n,
+                     maxbits);
+  ISDF_HIP_LAUNCHED("k_maxw");
+  if (df_tol > 1e-16) {
+    hipLaunchKernelGGL(k_thresh, dim3(cdiv(n, 256)), dim3(256), 0, st, b->w, n, maxbits, df_tol,
+                       b->flags);
+  } else {
+    hipLaunchKernelGGL(k_fill_u8, dim3(std::min(cdiv(n, 256), 4096u)), dim3(256), 0, st,
+                       b->flags, (long long)n, (uint8_t)1);
+  }
+  ISDF_HIP_LAUNCHED("k_thresh");
+  if (int rc = compact_flags(b, 1, 0, nullptr, st)) return rc;
+  int npt = 0;
+  ISDF_HIP_TRY(hipMemcpyAsync(&npt, b->bcount + b->nblk_flag, sizeof(int), hipMemcpyDeviceToHost,
+                              st));
+  ISDF_HIP_TRY(hipStreamSynchronize(st));
+  *npt_host = npt;
+  if (npt < n) {
+    hipLaunchKernelGGL(k_flag_scatter, dim3(b->nblk_flag), dim3(256), 0, st, b->flags, n, 1,
+                       b->bcount, 0, b->index);
+    ISDF_HIP_LAUNCHED("k_flag_scatter");
+  }
+  return 0;
+}
+
+int kmeans_setup_points(KmeansBufs* b, const int ngrid[3], const double length[3], int npt,
+                        int rk, hipStream_t st) {
+  const int n = b->n;
+  double dx0 = length[0] / ngrid[0], dx1 = length[1] / ngrid[1], dx2 = length[2] / ngrid[2];
+  hipLaunchKernelGGL(k_setup_points, dim3(cdiv(n, 256)), dim3(256), 0, st, n, npt, b->index, b->w,
+                     ngrid[0], ngrid[1], dx0, dx1, dx2, b->gidx, b->X, b->Y, b->Z, b->WL);
+  ISDF_HIP_LAUNCHED("k_setup_points");
+  hipLaunchKernelGGL(k_init_centroids, dim3(cdiv(rk, 256)), dim3(256), 0, st, rk, n, npt, b->index,
+                     ngrid[0], ngrid[1], dx0, dx1, dx2, b->cx, b->cy, b->cz, b->scal);
+  ISDF_HIP_LAUNCHED("k_init_centroids");
+  hipLaunchKernelGGL(k_assign, dim3(cdiv(n, 256)), dim3(256), 0, st, n, b->X, b->Y, b->Z, b->cx,
+                     b->cy, b->cz, b->label, b->scal, 0);
+  ISDF_HIP_LAUNCHED("k_assign");
+  return 0;
+}
+
+int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* iters_host,
+                   hipStream_t st) {
+  const int n = b->n;
+  const double maxF = kmeans_tol * n;
+  // flag = n before the first test (utility.cpp:765, :813).
+  const int active0 = ((double)n > maxF && 0 < max_iter) ? 1 : 0;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, b->scal + KS_ACTIVE, 1LL, active0);
+  ISDF_HIP_LAUNCHED("k_fill_i32");
+  *iters_host = 0;
+  if (!active0) return 0;
+  const size_t lds_rk = sizeof(int) * (size_t)rk;
+  const int check_every = 4;
+  for (int s = 0; s < max_iter; s++) {
+    hipLaunchKernelGGL(k_sort_hist, dim3(b->nchunk_sort), dim3(256), lds_rk, st, n, b->label,
+                       b->hist, b->scal);
+    hipLaunchKernelGGL(k_sort_scan_chunks, dim3(cdiv(rk, 256)), dim3(256), 0, st, b->nchunk_sort,
+                       b->hist, b->cnt_pts, b->scal);
+    hipLaunchKernelGGL(k_sort_scan_clusters, dim3(1), dim3(1024), 0, st, b->cnt_pts, b->base,
+                       b->scal);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(b->nchunk_sort), dim3(64), lds_rk, st, n, b->label,
+                       b->hist, b->base, b->sorted, b->scal);
+    hipLaunchKernelGGL(k_centroid_sums, dim3(cdiv(rk, 4)), dim3(256), 0, st, b->sorted, b->base,
+                       b->cnt_pts, b->WL, b->X, b->Y, b->Z, b->sw, b->sx, b->sy, b->sz, b->scal);
+    hipLaunchKernelGGL(k_update, dim3(1), dim3(1024), 0, st, b->sw, b->sx, b->sy, b->sz, b->cx,
+                       b->cy, b->cz, b->map, b->scal);
+    hipLaunchKernelGGL(k_assign, dim3(cdiv(n, 256)), dim3(256), 0, st, n, b->X, b->Y, b->Z, b->cx,
+                       b->cy, b->cz, b->label, b->scal, 1);
+    hipLaunchKernelGGL(k_control, dim3(1), dim3(1), 0, st, s, max_iter, maxF, b->trace, b->scal);
+    ISDF_HIP_LAUNCHED("k-means iteration");
+    if ((s + 1) % check_every == 0 && s + 1 < max_iter) {
+      int act = 1;
+      ISDF_HIP_TRY(hipMemcpyAsync(&act, b->scal + KS_ACTIVE, sizeof(int), hipMemcpyDeviceToHost,
+                                  st));
+      ISDF_HIP_TRY(hipStreamSynchronize(st));
+      if (!act) break;
+    }
+  }
+  int it = 0;
+  ISDF_HIP_TRY(hipMemcpyAsync(&it, b->scal + KS_ITERS, sizeof(int), hipMemcpyDeviceToHost, st));
+  ISDF_HIP_TRY(hipStreamSynchronize(st));
+  *iters_host = it;
+  return 0;
+}
+
+int kmeans_finish(KmeansBufs* b, int* piv_dev, hipStream_t st) {
+  const int n = b->n;
+  const int rkmax = b->rkmax;
+  dim3 g(b->nchunk_final, cdiv(rkmax, 256));
+  hipLaunchKernelGGL(k_final_partial, g, dim3(256), 0, st, n, b->chunk_final, b->X, b->Y, b->Z,
+                     b->cx, b->cy, b->cz, b->pbest, b->pidx, b->scal);
+  ISDF_HIP_LAUNCHED("k_final_partial");
+  hipLaunchKernelGGL(k_final_reduce, dim3(cdiv(rkmax, 256)), dim3(256), 0, st, b->nchunk_final,
+                     b->pbest, b->pidx, b->gidx, b->imin, b->scal);
+  ISDF_HIP_LAUNCHED("k_final_reduce");
+  int P = 1;
+  while (P < rkmax) P <<= 1;
+  hipLaunchKernelGGL(k_unique, dim3(1), dim3(1024), sizeof(int) * (size_t)P, st, b->imin,
+                     b->centers, b->scal);
+  ISDF_HIP_LAUNCHED("k_unique");
+  // piv = [sorted unique centers] ++ [all other grid indices ascending] (utility.cpp:973-983).
+  hipLaunchKernelGGL(k_fill_u8, dim3(std::min(cdiv(n, 256), 4096u)), dim3(256), 0, st, b->flags,
+                     (long long)n, (uint8_t)0);
+  hipLaunchKernelGGL(k_mark_centers, dim3(cdiv(rkmax, 256)), dim3(256), 0, st, b->centers,
+                     b->flags, piv_dev, b->scal);
+  ISDF_HIP_LAUNCHED("k_mark_centers");
+  int rku = 0;
+  ISDF_HIP_TRY(hipMemcpyAsync(&rku, b->scal + KS_RKU, sizeof(int), hipMemcpyDeviceToHost, st));
+  ISDF_HIP_TRY(hipStreamSynchronize(st));
+  return compact_flags(b, 0, rku, piv_dev, st);
+}
+
+int kmeans_max_rk() { return kMaxRk; }
+
+}  // namespace isdf_hip

@@ -1,0 +1,345 @@
+// C ABI entry points of libisdf_hip: the single-GPU ISDF solve.
+//
+// Stage order follows the reference CPU ISDF (src/common/isdf.cpp:8-127):
+// rank rule, piv zeroing, K-means weights and KMEAN (getpoints_Kmeans_MPI.cpp),
+// then the basis (getbasis_scalapack.cpp).  Everything stays on the device;
+// only piv travels to the host, because the caller's piv is host memory.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/isdf_hip.h"
+#include "isdf_internal.h"
+
+namespace isdf_hip {
+
+namespace {
+std::mutex g_mutex;
+std::string g_last_error;
+std::mutex g_err_mutex;
+
+// Grow-only device workspace, one per device.
+struct Workspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+std::vector<Workspace> g_ws;
+
+int workspace(size_t bytes, void** out) {
+  int dev = 0;
+  ISDF_HIP_TRY(hipGetDevice(&dev));
+  if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
+  Workspace& w = g_ws[dev];
+  if (w.bytes < bytes) {
+    if (w.ptr) ISDF_HIP_TRY(hipFree(w.ptr));
+    w.ptr = nullptr;
+    w.bytes = 0;
+    hipError_t e = hipMalloc(&w.ptr, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      w.ptr = nullptr;
+      return set_error(ISDF_ENOMEM, "device workspace allocation of " + std::to_string(bytes) +
+                                        " bytes failed: " + hipGetErrorString(e));
+    }
+    w.bytes = bytes;
+  }
+  *out = w.ptr;
+  return 0;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// glibc rand() with its default seed and libstdc++'s std::random_shuffle, for
+// the npt < rk branch of KMEAN (src/common/utility.cpp:667-681).
+struct GlibcRand {
+  std::vector<int32_t> r;
+  explicit GlibcRand(uint32_t seed) {
+    r.resize(34);
+    r[0] = (int32_t)seed;
+    for (int i = 1; i < 31; i++) {
+      int64_t v = (16807LL * r[i - 1]) % 2147483647LL;
+      if (v < 0) v += 2147483647LL;
+      r[i] = (int32_t)v;
+    }
+    for (int i = 31; i < 34; i++) r[i] = r[i - 31];
+    for (int i = 34; i < 344; i++) r.push_back((int32_t)((uint32_t)r[i - 31] + (uint32_t)r[i - 3]));
+  }
+  int next() {
+    size_t i = r.size();
+    int32_t v = (int32_t)((uint32_t)r[i - 31] + (uint32_t)r[i - 3]);
+    r.push_back(v);
+    return (int)(((uint32_t)v) >> 1);
+  }
+};
+
+struct Events {
+  bool on = false;
+  hipEvent_t e[10];
+  int create(bool want) {
+    on = want;
+    if (!on) return 0;
+    for (int i = 0; i < 10; i++) ISDF_HIP_TRY(hipEventCreate(&e[i]));
+    return 0;
+  }
+  int rec(int i, hipStream_t st) {
+    if (on) ISDF_HIP_TRY(hipEventRecord(e[i], st));
+    return 0;
+  }
+  double ms(int a, int b) {
+    float t = 0.f;
+    if (on && hipEventElapsedTime(&t, e[a], e[b]) == hipSuccess) return (double)t;
+    return 0.0;
+  }
+  ~Events() {
+    if (on)
+      for (int i = 0; i < 10; i++) (void)hipEventDestroy(e[i]);
+  }
+};
+
+int irank(const isdf_problem* p) {
+  if (p->mu_points != 0) return p->mu_points;
+  // IRound, include/common/utility.hpp:18-23.
+  double a = std::sqrt((double)((p->nc1 + p->nv1) * (p->nc2 + p->nv2))) * 8.0;
+  if (a > 0) return (a - (int)a < 0.5) ? (int)a : ((int)a + 1);
+  return ((int)a - a < 0.5) ? (int)a : ((int)a - 1);
+}
+
+int validate(const isdf_problem* p, long long* ng_out) {
+  if (!p) return set_error(ISDF_EINVAL, "problem is NULL");
+  for (int d = 0; d < 3; d++) {
+    if (p->ngrid[d] <= 0) return set_error(ISDF_EINVAL, "ngrid must be positive");
+    if (!(p->length[d] > 0.0)) return set_error(ISDF_EINVAL, "length must be positive");
+  }
+  long long ng = (long long)p->ngrid[0] * p->ngrid[1] * p->ngrid[2];
+  if (ng > 0x7fffffffLL) return set_error(ISDF_EINVAL, "Ng exceeds the int range of piv");
+  if (p->nstate <= 0) return set_error(ISDF_EINVAL, "nstate must be positive");
+  int s1 = p->nocc - p->nv1, e1 = p->nocc + p->nc1, s2 = p->nocc - p->nv2, e2 = p->nocc + p->nc2;
+  if (s1 < 0 || s2 < 0 || e1 > p->nstate || e2 > p->nstate || e1 <= s1 || e2 <= s2)
+    return set_error(ISDF_EINVAL,
+                     "pair ranges [nocc-nv, nocc+nc) must be non-empty and inside [0, nstate)");
+  if (p->kmeans_max_iter < 0) return set_error(ISDF_EINVAL, "kmeans_max_iter must be >= 0");
+  if (p->ld_psi != 0 && p->ld_psi < ng) return set_error(ISDF_EINVAL, "ld_psi < Ng");
+  if (p->precision != ISDF_PREC_FP64) return set_error(ISDF_EINVAL, "unknown precision mode");
+  int rk = irank(p);
+  if (rk <= 0) return set_error(ISDF_EINVAL, "rk must be positive");
+  if (rk > ng) return set_error(ISDF_EINVAL, "mu_points exceeds the number of grid points");
+  if (rk > 8192) return set_error(ISDF_EINVAL, "rk above 8192 is not supported");
+  *ng_out = ng;
+  return 0;
+}
+
+int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, int* h_piv,
+                 hipStream_t st, isdf_report* rep, const isdf_debug* dbg) {
+  long long ng64 = 0;
+  if (int rc = validate(p, &ng64)) return rc;
+  if (!d_psi || !d_theta || !h_piv) return set_error(ISDF_EINVAL, "NULL buffer argument");
+  const int ng = (int)ng64;
+  const int rk = irank(p);
+  const int nstate = p->nstate;
+  const long long ld = p->ld_psi ? p->ld_psi : ng64;
+  const int s1 = p->nocc - p->nv1, K1 = p->nv1 + p->nc1;
+  const int s2 = p->nocc - p->nv2, K2 = p->nv2 + p->nc2;
+  const bool dual = (p->nv1 != p->nv2 || p->nc1 != p->nc2);
+  const int max_iter = p->kmeans_max_iter;
+  const int nJ = (rk + kNB - 1) / kNB;
+
+  // Workspace layout: K-means buffers | piv | psimuT | P | Dinv | Ft | Gt | scalars.
+  size_t off = 0;
+  const size_t off_km = off;
+  off += align256(kmeans_bytes(ng, rk, max_iter));
+  const size_t off_piv = off;
+  off += align256(sizeof(int) * (size_t)ng);
+  const size_t off_psimu = off;
+  off += align256(sizeof(double) * (size_t)nstate * rk);
+  const size_t off_P = off;
+  off += align256(sizeof(double) * (size_t)rk * rk);
+  const size_t off_Dinv = off;
+  off += align256(sizeof(double) * (size_t)nJ * kNB * kNB);
+  const size_t off_Ft = off;
+  off += align256(sizeof(double) * (size_t)rk * rk);
+  const size_t off_Gt = off;
+  off += align256(sizeof(double) * (size_t)rk * rk);
+  const size_t off_sc = off;
+  off += 256;
+  void* wsp = nullptr;
+  if (int rc = workspace(off, &wsp)) return rc;
+  char* base = (char*)wsp;
+  KmeansBufs kb;
+  kmeans_carve(&kb, base + off_km, ng, rk, max_iter);
+  int* piv_dev = (int*)(base + off_piv);
+  double* psimuT = (double*)(base + off_psimu);
+  double* P = (double*)(base + off_P);
+  double* Dinv = (double*)(base + off_Dinv);
+  double* Ft = (double*)(base + off_Ft);
+  double* Gt = (double*)(base + off_Gt);
+  int* info_dev = (int*)(base + off_sc);
+  double* eps_dev = (double*)(base + off_sc + 64);
+
+  Events ev;
+  if (int rc = ev.create(rep != nullptr)) return rc;
+  if (int rc = ev.rec(0, st)) return rc;
+
+  // piv zeroed first (src/common/isdf.cpp:174-175).
+  std::memset(h_piv, 0, sizeof(int) * (size_t)ng);
+
+  // 1. weights (getpoints_Kmeans_MPI.cpp:33-45).
+  if (int rc = launch_weights(d_psi, ld, ng, s1, s1 + K1, s2, s2 + K2, kb.w, st)) return rc;
+  if (dbg && dbg->d_weights)
+    ISDF_HIP_TRY(hipMemcpyAsync(dbg->d_weights, kb.w, sizeof(double) * ng,
+                                hipMemcpyDeviceToDevice, st));
+  if (int rc = ev.rec(1, st)) return rc;
+
+  // 2. K-means (utility.cpp:631-988).
+  int npt = 0, iters = 0, rku = rk;
+  if (int rc = kmeans_prepare(&kb, p->ngrid, p->length, 1e-8, &npt, st)) return rc;
+  bool degenerate = npt < rk;
+  if (!degenerate) {
+    if (int rc = kmeans_setup_points(&kb, p->ngrid, p->length, npt, rk, st)) return rc;
+    if (int rc = kmeans_iterate(&kb, rk, max_iter, 1e-5, &iters, st)) return rc;
+    if (int rc = ev.rec(2, st)) return rc;
+    if (int rc = kmeans_finish(&kb, piv_dev, st)) return rc;
+    ISDF_HIP_TRY(hipMemcpyAsync(h_piv, piv_dev, sizeof(int) * (size_t)ng, hipMemcpyDeviceToHost,
+                                st));
+    ISDF_HIP_TRY(hipMemcpyAsync(&rku, kb.scal + KS_RKU, sizeof(int), hipMemcpyDeviceToHost, st));
+  } else {
+    // Fewer points than rk above the density threshold: the reference fills piv
+    // from the threshold list and shuffles the tail (utility.cpp:667-681).
+    if (int rc = ev.rec(2, st)) return rc;
+    std::vector<uint8_t> fl((size_t)ng);
+    ISDF_HIP_TRY(hipMemcpyAsync(fl.data(), kb.flags, (size_t)ng, hipMemcpyDeviceToHost, st));
+    ISDF_HIP_TRY(hipStreamSynchronize(st));
+    std::vector<int> index((size_t)ng, 0);
+    int c = 0;
+    for (int i = 0; i < ng; i++)
+      if (fl[i]) index[c++] = i;
+    int k0 = 0, k1 = 0;
+    for (int i = 0; i < npt; i++) {
+      if (i == index[k0]) {
+        h_piv[k0] = i;
+        k0 = std::min(k0 + 1, rk - 1);
+      } else {
+        h_piv[npt + k1] = i;
+        k1++;
+      }
+    }
+    GlibcRand rng(1u);
+    int* first = h_piv + npt;
+    long long len = (long long)ng - npt;
+    for (long long i = 1; i < len; i++) {
+      long long j = rng.next() % (i + 1);
+      if (i != j) std::swap(first[i], first[j]);
+    }
+    ISDF_HIP_TRY(hipMemcpyAsync(piv_dev, h_piv, sizeof(int) * (size_t)ng, hipMemcpyHostToDevice,
+                                st));
+  }
+  if (int rc = ev.rec(3, st)) return rc;
+  if (dbg && dbg->d_labels && !degenerate)
+    ISDF_HIP_TRY(hipMemcpyAsync(dbg->d_labels, kb.label, sizeof(int) * ng,
+                                hipMemcpyDeviceToDevice, st));
+  if (dbg && dbg->h_trace && max_iter > 0 && !degenerate)
+    ISDF_HIP_TRY(hipMemcpyAsync(dbg->h_trace, kb.trace, sizeof(int) * std::max(iters, 0),
+                                hipMemcpyDeviceToHost, st));
+
+  // 3. Psi(mu) gather over the caller's rk (getbasis_scalapack.cpp:249-269).
+  if (int rc = launch_gather_psimu(d_psi, ld, nstate, piv_dev, rk, psimuT, st)) return rc;
+  if (int rc = ev.rec(4, st)) return rc;
+  // 4. Z C^T into d_theta (getbasis_scalapack.cpp:297-369).
+  if (int rc = launch_zct(d_psi, ld, ng, psimuT, rk, s1, K1, s2, K2, dual, d_theta, ng, st))
+    return rc;
+  if (int rc = ev.rec(5, st)) return rc;
+  // 5. C C^T + eps (getbasis_scalapack.cpp:390-428).
+  if (int rc = launch_cct(d_theta, ng, piv_dev, rk, P, eps_dev, st)) return rc;
+  if (dbg && dbg->d_cct)
+    ISDF_HIP_TRY(hipMemcpyAsync(dbg->d_cct, P, sizeof(double) * (size_t)rk * rk,
+                                hipMemcpyDeviceToDevice, st));
+  if (int rc = ev.rec(6, st)) return rc;
+  // 6. Cholesky (getbasis_scalapack.cpp:455).
+  ISDF_HIP_TRY(hipMemsetAsync(info_dev, 0, sizeof(int), st));
+  if (int rc = launch_potrf(P, rk, Dinv, info_dev, st)) return rc;
+  if (int rc = ev.rec(7, st)) return rc;
+  // 7. Two right-side triangular solves in place (getbasis_scalapack.cpp:469-478).
+  if (int rc = launch_make_fg(P, rk, Dinv, Ft, Gt, st)) return rc;
+  if (int rc = ev.rec(8, st)) return rc;
+  if (int rc = launch_potrs(d_theta, ng64, ng64, rk, Ft, Gt, st)) return rc;
+  if (int rc = ev.rec(9, st)) return rc;
+
+  int info = 0;
+  double eps = 0.0;
+  ISDF_HIP_TRY(hipMemcpyAsync(&info, info_dev, sizeof(int), hipMemcpyDeviceToHost, st));
+  ISDF_HIP_TRY(hipMemcpyAsync(&eps, eps_dev, sizeof(double), hipMemcpyDeviceToHost, st));
+  ISDF_HIP_TRY(hipStreamSynchronize(st));
+
+  if (rep) {
+    rep->rk = rk;
+    rep->rk_kmeans = degenerate ? rk : rku;
+    rep->kmeans_iters = iters;
+    rep->npt = npt;
+    rep->potrf_info = info;
+    rep->eps = eps;
+    rep->ms_weights = ev.ms(0, 1);
+    rep->ms_kmeans = ev.ms(1, 2);
+    rep->ms_select = ev.ms(2, 3);
+    rep->ms_gather = ev.ms(3, 4);
+    rep->ms_zct = ev.ms(4, 5);
+    rep->ms_cct = ev.ms(5, 6);
+    rep->ms_potrf = ev.ms(6, 8);
+    rep->ms_potrs = ev.ms(8, 9);
+    rep->ms_total = ev.ms(0, 9);
+  }
+  if (info != 0) return set_error(ISDF_ENOTSPD, "Matrix is not positive definite.");
+  return ISDF_OK;
+}
+
+}  // namespace
+
+int set_error(int code, const std::string& msg) {
+  std::lock_guard<std::mutex> lk(g_err_mutex);
+  g_last_error = msg;
+  return code;
+}
+
+}  // namespace isdf_hip
+
+extern "C" {
+
+int isdf_hip_solve(const isdf_problem* prob, const double* d_psi, double* d_theta, int* h_piv,
+                   void* stream, isdf_report* rep, const isdf_debug* dbg) {
+  std::lock_guard<std::mutex> lk(isdf_hip::g_mutex);
+  isdf_hip::set_error(0, "");
+  return isdf_hip::solve_locked(prob, d_psi, d_theta, h_piv, (hipStream_t)stream, rep, dbg);
+}
+
+int isdf_hip_rank(const isdf_problem* prob) {
+  if (!prob) return -1;
+  return isdf_hip::irank(prob);
+}
+
+const char* isdf_hip_last_error(void) {
+  static thread_local std::string copy;
+  std::lock_guard<std::mutex> lk(isdf_hip::g_err_mutex);
+  copy = isdf_hip::g_last_error;
+  return copy.c_str();
+}
+
+const char* isdf_hip_version(void) { return "libisdf_hip 0.1 (gfx950)"; }
+
+int isdf_hip_release_workspace(void) {
+  std::lock_guard<std::mutex> lk(isdf_hip::g_mutex);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return ISDF_EHIP;
+  if ((int)isdf_hip::g_ws.size() > dev && isdf_hip::g_ws[dev].ptr) {
+    if (hipFree(isdf_hip::g_ws[dev].ptr) != hipSuccess) return ISDF_EHIP;
+    isdf_hip::g_ws[dev].ptr = nullptr;
+    isdf_hip::g_ws[dev].bytes = 0;
+  }
+  return ISDF_OK;
+}
+
+}  // extern "C"
