@@ -10,7 +10,8 @@
 // FP64 matrix work runs on v_mfma_f64_16x16x4f64.  Operand fragments are read
 // straight from global memory (they are re-used from the vector L1/L2): each
 // lane of a 16x16x4 step supplies A[m = lane%16][k = lane/16] and
-// B[k = lane/16][n = lane%16] and owns D[4*(lane/16) + r][lane%16], r = 0..3.
+// B[k = lane/16][n = lane%16] and owns D[lane/16 + 4r][lane%16], r = 0..3
+// (the gfx950 output layout, measured).
 
 #include <hip/hip_runtime.h>
 
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(256) void k_zct(int ng, int rk, const double* __res
       if (!nv[j]) continue;
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const long long m = m0 + 16 * i + 4 * lk + r;
+        const long long m = m0 + 16 * i + lk + 4 * r;
         if (m < ng) {
           double v = acc1[i][j][r];
           double o;
@@ -372,7 +373,7 @@ __global__ __launch_bounds__(64) void k_potrs(double* __restrict__ T, long long 
           if (!nv[j]) continue;
 #pragma unroll
           for (int r = 0; r < 4; r++) {
-            const long long m = m0 + 16 * i + 4 * lk + r;
+            const long long m = m0 + 16 * i + lk + 4 * r;
             if (m < ng) T[m + (long long)(J0 + 16 * j + lr) * ldt] = acc[i][j][r];
           }
         }
