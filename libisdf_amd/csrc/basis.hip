@@ -7,27 +7,19 @@
 // :249-269 (gather), :297-369 (pdgemm + Hadamard), :390-428 (C C^T + eps),
 // :455 (pdpotrf 'L'), :469-478 (pdtrsm R,L,T,N then R,L,N,N).
 //
-// FP64 matrix work runs on v_mfma_f64_16x16x4f64.  Operand fragments are read
-// straight from global memory (they are re-used from the vector L1/L2): each
-// lane of a 16x16x4 step supplies A[m = lane%16][k = lane/16] and
-// B[k = lane/16][n = lane%16] and owns D[lane/16 + 4r][lane%16], r = 0..3
-// (the gfx950 output layout, measured).
+// FP64 matrix work runs on v_mfma_f64_16x16x4f64 through the LDS-staged tile
+// of mfma_gemm.h.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
 #include "isdf_internal.h"
+#include "mfma_gemm.h"
 
 namespace isdf_hip {
 
 namespace {
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ d4 mfma_f64(double a, double b, d4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
@@ -42,97 +34,75 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ psi, 
   psimuT[t] = psi[(long long)k * ld + piv[mu]];
 }
 
-// acc[i][j] += A[m0+16i.., 0:K] * B[0:K, n0+16j..] with A column-major (ld lda)
-// and B k-major (ld ldb); fragments double-buffered in registers.
-template <int TM, int TN>
-__device__ __forceinline__ void mfma_panel(const double* __restrict__ A, long long lda,
-                                           const double* __restrict__ B, long long ldb, int K,
-                                           long long m0, int n0, int lr, int lk, const bool (&mv)[TM],
-                                           const bool (&nv)[TN], d4 (&acc)[TM][TN]) {
-  double a[TM], b[TN], an[TM], bn[TN];
-  {
-    const bool kv = lk < K;
-#pragma unroll
-    for (int i = 0; i < TM; i++)
-      a[i] = (kv && mv[i]) ? A[m0 + 16 * i + lr + (long long)lk * lda] : 0.0;
-#pragma unroll
-    for (int j = 0; j < TN; j++) b[j] = (kv && nv[j]) ? B[(long long)lk * ldb + n0 + 16 * j + lr] : 0.0;
-  }
-  for (int k0 = 0; k0 < K; k0 += 4) {
-    const int k = k0 + 4 + lk;
-    const bool kv = k < K;
-#pragma unroll
-    for (int i = 0; i < TM; i++)
-      an[i] = (kv && mv[i]) ? A[m0 + 16 * i + lr + (long long)k * lda] : 0.0;
-#pragma unroll
-    for (int j = 0; j < TN; j++)
-      bn[j] = (kv && nv[j]) ? B[(long long)k * ldb + n0 + 16 * j + lr] : 0.0;
-#pragma unroll
-    for (int i = 0; i < TM; i++)
-#pragma unroll
-      for (int j = 0; j < TN; j++) acc[i][j] = mfma_f64(a[i], b[j], acc[i][j]);
-#pragma unroll
-    for (int i = 0; i < TM; i++) a[i] = an[i];
-#pragma unroll
-    for (int j = 0; j < TN; j++) b[j] = bn[j];
+// Tile shapes: 256 x 128 per workgroup (8 waves of 64 x 64) for the single
+// product; 256 x 64 with two accumulators when the pair sets differ.
+typedef MfmaTile<256, 128, 16, 4, 2> ZctTile0;      // 8 waves of 64 x 64
+typedef MfmaTile<256, 128, 16, 4, 1> ZctTile1;      // 4 waves of 64 x 128
+typedef MfmaTile<128, 128, 16, 2, 2> ZctTile2;      // 4 waves of 64 x 64
+typedef MfmaTile<256, 64, 16, 4, 2> ZctTileDual0;   // two accumulators
+typedef MfmaTile<128, 64, 16, 2, 2> ZctTileDual1;
+
+// Maps a linear workgroup id to (row strip, column tile) so that the ncolt
+// column tiles of one row strip land on the same XCD (workgroups are dealt
+// round-robin to the 8 XCDs) and run back to back, sharing the strip in L2.
+// Row strips beyond the last full group of 8 are mapped in plain order.
+__device__ __forceinline__ void xcd_tile(int b, int ncolt, int nrows, int* rs, int* ct) {
+  const int full = (nrows / 8) * 8;
+  if (b < full * ncolt) {
+    const int x = b & 7, l = b >> 3;
+    *rs = x + 8 * (l / ncolt);
+    *ct = l % ncolt;
+  } else {
+    const int b2 = b - full * ncolt;
+    *rs = full + b2 / ncolt;
+    *ct = b2 % ncolt;
   }
 }
 
-// C (ng x rk, column-major, ld ldc) = P1 o P1 or P1 o P2, with
-// P1 = A1 (ng x K1, ld lda) * B1 (K1 x rk, k-major, ld ldb) and likewise P2.
-// Workgroup: 4 waves in 2 x 2; a wave owns TM x TN 16x16 tiles.  The factors
-// live only in accumulators: the epilogue writes their product once.
-template <int TM, int TN, bool DUAL>
-__global__ __launch_bounds__(256) void k_zct(int ng, int rk, const double* __restrict__ A1,
-                                             const double* __restrict__ A2, long long lda, int K1,
-                                             int K2, const double* __restrict__ B1,
-                                             const double* __restrict__ B2, int ldb,
-                                             double* __restrict__ C, long long ldc) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int lr = lane & 15, lk = lane >> 4;
-  const long long m0 = (long long)blockIdx.x * (32 * TM) + (wave & 1) * (16 * TM);
-  const int n0 = blockIdx.y * (32 * TN) + (wave >> 1) * (16 * TN);
-
-  d4 acc1[TM][TN];
-  d4 acc2[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; i++)
-#pragma unroll
-    for (int j = 0; j < TN; j++) acc1[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+// C (ng x rk, column-major, ld ldc) = P1 o P1 or P1 o P2 with
+// P1 = A1 (ng x K1, ld lda) * B1 (K1 x rk, k-major, ld ldb), P2 likewise.  The
+// factors live only in accumulators; the epilogue writes their product once.
+template <class Tile, bool DUAL>
+__global__ __launch_bounds__(Tile::NT, 1) void k_zct(int ng, int rk, const double* __restrict__ A1,
+                                                const double* __restrict__ A2, long long lda,
+                                                int K1, int K2, const double* __restrict__ B1,
+                                                const double* __restrict__ B2, int ldb,
+                                                double* __restrict__ C, long long ldc) {
+  __shared__ double lds[Tile::LDS_DOUBLES];
+  const int nrows = (ng + Tile::BM - 1) / Tile::BM;
+  const int ncolt = (rk + Tile::BN - 1) / Tile::BN;
+  int rs, ct;
+  xcd_tile(blockIdx.x, ncolt, nrows, &rs, &ct);
+  if (rs >= nrows) return;
+  const long long m0 = (long long)rs * Tile::BM;
+  const int n0 = ct * Tile::BN;
+  const int mlim = (int)std::min<long long>(Tile::BM, ng - m0);
+  const int nlim = std::min(Tile::BN, rk - n0);
+  typename Tile::Acc acc1, acc2;
+  Tile::zero(acc1);
+  Tile::run(A1 + m0, lda, B1 + n0, ldb, K1, mlim, nlim, lds, acc1);
   if constexpr (DUAL) {
-#pragma unroll
-    for (int i = 0; i < TM; i++)
-#pragma unroll
-      for (int j = 0; j < TN; j++) acc2[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    Tile::zero(acc2);
+    Tile::run(A2 + m0, lda, B2 + n0, ldb, K2, mlim, nlim, lds, acc2);
   }
-
-  bool mv[TM], nv[TN];
 #pragma unroll
-  for (int i = 0; i < TM; i++) mv[i] = (m0 + 16 * i + lr) < ng;
+  for (int j = 0; j < Tile::TN; j++) {
+    const int n = Tile::col(j);
+    if (n >= nlim) continue;
+    double* Cc = C + (long long)(n0 + n) * ldc + m0;
 #pragma unroll
-  for (int j = 0; j < TN; j++) nv[j] = (n0 + 16 * j + lr) < rk;
-
-  mfma_panel<TM, TN>(A1, lda, B1, ldb, K1, m0, n0, lr, lk, mv, nv, acc1);
-  if constexpr (DUAL) mfma_panel<TM, TN>(A2, lda, B2, ldb, K2, m0, n0, lr, lk, mv, nv, acc2);
-
-#pragma unroll
-  for (int i = 0; i < TM; i++) {
-#pragma unroll
-    for (int j = 0; j < TN; j++) {
-      const int nn = n0 + 16 * j + lr;
-      if (!nv[j]) continue;
+    for (int i = 0; i < Tile::TM; i++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const long long m = m0 + 16 * i + lk + 4 * r;
-        if (m < ng) {
-          double v = acc1[i][j][r];
+        const int m = Tile::row(i, r);
+        if (m < mlim) {
+          const double v = acc1[i][j][r];
           double o;
           if constexpr (DUAL) o = v * acc2[i][j][r];
           else o = v * v;
-          C[m + (long long)nn * ldc] = o;
+          Cc[m] = o;
         }
       }
-    }
   }
 }
 
@@ -151,9 +121,12 @@ __global__ __launch_bounds__(256) void k_cct_gather(const double* __restrict__ C
 __global__ __launch_bounds__(256) void k_trace_eps(double* __restrict__ P, int rk,
                                                    double* __restrict__ eps_out) {
   __shared__ double eps_s;
+  __shared__ double diag[8192];
+  for (int nu = threadIdx.x; nu < rk; nu += blockDim.x) diag[nu] = P[(long long)nu * rk + nu];
+  __syncthreads();
   if (threadIdx.x == 0) {
     double tr = 0.0;
-    for (int nu = 0; nu < rk; nu++) tr += P[(long long)nu * rk + nu];
+    for (int nu = 0; nu < rk; nu++) tr += diag[nu];
     double eps = 1e-10 * tr / (double)rk;
     eps_s = eps;
     *eps_out = eps;
@@ -167,221 +140,316 @@ __global__ __launch_bounds__(256) void k_trace_eps(double* __restrict__ P, int r
 // Blocked right-looking Cholesky, lower (the reference's pdpotrf 'L').
 // ---------------------------------------------------------------------------
 
-// Factor the jb x jb diagonal block at J0 (one wave, column-major in LDS), write
-// L_JJ back, and its inverse into Dinv[J] (64 x 64, column-major, zero padded).
+// Factor the jb x jb diagonal block at J0 with one wave, write L_JJ back and
+// its inverse into Dinv[J] (64 x 64, column-major, zero padded).  Lane r keeps
+// row r of the block in registers (fully unrolled, static indices); the column
+// of each step and the finished factor pass through LDS as broadcasts.
 __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int n, int J0,
                                                    double* __restrict__ Dinv,
                                                    int* __restrict__ info) {
-  __shared__ double at[kNB][kNB];   // at[col][row]
-  __shared__ double inv[kNB][kNB];  // inv[row][col]
+  __shared__ double colj[kNB];
+  __shared__ double Lrow[kNB][kNB + 1];  // Lrow[i][t] = L[i][t]
+  __shared__ int fail_s;
   if (*info != 0) return;
   const int lane = threadIdx.x;
   const int jb = min(kNB, n - J0);
-  for (int c = 0; c < kNB; c++) {
-    double v = 0.0;
-    if (c < jb && lane < jb && lane >= c) v = A[(long long)(J0 + c) * n + J0 + lane];
-    at[c][lane] = v;
-  }
+  const double* Ab = A + (long long)J0 * n + J0;
+  double v[kNB];  // v[k] = A[lane][k], k <= lane
+#pragma unroll
+  for (int k = 0; k < kNB; k++) v[k] = (k < jb && lane < jb && lane >= k) ? Ab[(long long)k * n + lane] : 0.0;
+  if (lane == 0) fail_s = 0;
   __syncthreads();
-  int fail = 0;
-  for (int j = 0; j < jb; j++) {
-    double d = at[j][j];
-    if (!(d > 0.0)) {
-      fail = j + 1;
-      break;
+#pragma unroll
+  for (int j = 0; j < kNB; j++) {
+    if (j < jb) {
+      // Column j: lane j publishes its pivot, lanes below publish A[lane][j].
+      colj[lane] = v[j];
+      __syncthreads();
+      const double piv = colj[j];
+      if (!(piv > 0.0)) {
+        if (lane == 0 && fail_s == 0) fail_s = j + 1;
+      }
+      const double d = sqrt(piv);
+      double lij = 0.0;
+      if (lane == j) v[j] = d;
+      if (lane > j && lane < jb) {
+        lij = v[j] / d;
+        v[j] = lij;
+      }
+      __syncthreads();
+      colj[lane] = lij;
+      __syncthreads();
+#pragma unroll
+      for (int k = j + 1; k < kNB; k++)
+        if (k <= lane) v[k] -= lij * colj[k];
+      __syncthreads();
     }
-    d = sqrt(d);
-    __syncthreads();
-    if (lane == j) at[j][j] = d;
-    if (lane > j && lane < jb) at[j][lane] = at[j][lane] / d;
-    __syncthreads();
-    if (lane > j && lane < jb) {
-      const double lij = at[j][lane];
-      for (int k = j + 1; k <= lane; k++) at[k][lane] -= lij * at[j][k];
-    }
-    __syncthreads();
   }
-  if (fail) {
-    if (lane == 0) atomicCAS(info, 0, J0 + fail);
+  if (fail_s) {
+    if (lane == 0) atomicCAS(info, 0, J0 + fail_s);
     return;
   }
-  for (int c = 0; c < jb; c++)
-    if (lane >= c && lane < jb) A[(long long)(J0 + c) * n + J0 + lane] = at[c][lane];
-  // Column `lane` of inv(L_JJ) by forward substitution.
-  const int c = lane;
-  for (int i = 0; i < kNB; i++) {
-    double val = 0.0;
-    if (c < jb && i < jb) {
-      if (i == c) {
-        val = 1.0 / at[c][c];
-      } else if (i > c) {
-        double s = 0.0;
-        for (int t = c; t < i; t++) s += at[t][i] * inv[t][c];
-        val = -s / at[i][i];
-      }
-    }
-    inv[i][c] = val;
-    __syncthreads();
-  }
-  double* D = Dinv + (long long)(J0 / kNB) * kNB * kNB;
-  for (int col = 0; col < kNB; col++) D[(long long)col * kNB + lane] = inv[lane][col];
-}
-
-// L_IJ = A_IJ inv(L_JJ)^T for the rows below the diagonal block; one wave per
-// 64 rows.
-__global__ __launch_bounds__(64) void k_potrf_panel(double* __restrict__ A, int n, int J0,
-                                                    const double* __restrict__ Dinv,
-                                                    const int* __restrict__ info) {
-  __shared__ double at[kNB][kNB];  // at[k][row]
-  __shared__ double di[kNB][kNB];  // di[col][k] = inv(L_JJ)[col][k]
-  if (*info != 0) return;
-  const int lane = threadIdx.x;
-  const int jb = min(kNB, n - J0);
-  const int r0 = J0 + jb + blockIdx.x * kNB;
-  const int r = r0 + lane;
-  const double* D = Dinv + (long long)(J0 / kNB) * kNB * kNB;
+  double* Aw = A + (long long)J0 * n + J0;
+#pragma unroll
   for (int k = 0; k < kNB; k++) {
-    at[k][lane] = (k < jb && r < n) ? A[(long long)(J0 + k) * n + r] : 0.0;
-    di[lane][k] = D[(long long)k * kNB + lane];
+    if (k < jb && lane < jb && lane >= k) Aw[(long long)k * n + lane] = v[k];
+    Lrow[lane][k] = (k <= lane && lane < jb) ? v[k] : 0.0;
   }
   __syncthreads();
-  if (r >= n) return;
-  for (int c = 0; c < jb; c++) {
+  // Column `lane` of inv(L_JJ) by forward substitution, in registers.
+  const int c = lane;
+  double x[kNB];
+#pragma unroll
+  for (int i = 0; i < kNB; i++) {
     double s = 0.0;
-    for (int k = 0; k <= c; k++) s += at[k][lane] * di[c][k];
-    A[(long long)(J0 + c) * n + r] = s;
+#pragma unroll
+    for (int t = 0; t < i; t++) s += Lrow[i][t] * x[t];  // x[t] = 0 for t < c
+    double val = 0.0;
+    if (c < jb && i < jb) {
+      if (i == c) val = 1.0 / Lrow[i][i];
+      else if (i > c) val = -s / Lrow[i][i];
+    }
+    x[i] = val;
+  }
+  double* D = Dinv + (long long)(J0 / kNB) * kNB * kNB;
+#pragma unroll
+  for (int i = 0; i < kNB; i++) D[(long long)c * kNB + i] = x[i];
+}
+
+typedef MfmaTile<128, 64, 16, 2, 2> PanelTile;
+typedef MfmaTile<128, 128, 16, 2, 2> SyrkTile;
+
+// L_IJ = A_IJ inv(L_JJ)^T for the rows below the diagonal block (MFMA).
+// Each workgroup owns 128 rows, so the in-place overwrite is safe once its
+// GEMM has read them.
+__global__ __launch_bounds__(256) void k_potrf_panel(double* __restrict__ A, int n, int J0,
+                                                     const double* __restrict__ Dinv,
+                                                     const int* __restrict__ info) {
+  __shared__ double lds[PanelTile::LDS_DOUBLES];
+  if (*info != 0) return;
+  const int jb = min(kNB, n - J0);
+  const int I0 = J0 + jb + blockIdx.x * PanelTile::BM;
+  const int mlim = min(PanelTile::BM, n - I0);
+  double* Ai = A + (long long)J0 * n + I0;
+  const double* D = Dinv + (long long)(J0 / kNB) * kNB * kNB;
+  typename PanelTile::Acc acc;
+  PanelTile::zero(acc);
+  // B(k, c) = inv(L_JJ)(c, k) = D[k*64 + c]: k-major with ld 64.
+  PanelTile::run(Ai, n, D, kNB, jb, mlim, jb, lds, acc);
+#pragma unroll
+  for (int j = 0; j < PanelTile::TN; j++) {
+    const int c = PanelTile::col(j);
+    if (c >= jb) continue;
+#pragma unroll
+    for (int i = 0; i < PanelTile::TM; i++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = PanelTile::row(i, r);
+        if (m < mlim) Ai[(long long)c * n + m] = acc[i][j][r];
+      }
   }
 }
 
-// Trailing update A_IK -= L_IJ L_KJ^T on the lower 64 x 64 tiles (I >= K).
+// Trailing update A_IK -= L_IJ L_KJ^T on the lower 128 x 128 tiles (MFMA).
 __global__ __launch_bounds__(256) void k_potrf_syrk(double* __restrict__ A, int n, int J0,
                                                     const int* __restrict__ info) {
-  __shared__ double li[kNB][kNB + 1];  // li[k][row]
-  __shared__ double lk[kNB][kNB + 1];
+  __shared__ double lds[SyrkTile::LDS_DOUBLES];
   if (*info != 0) return;
   const int ti = blockIdx.x, tk = blockIdx.y;
   if (tk > ti) return;
   const int jb = min(kNB, n - J0);
   const int T0 = J0 + jb;
-  const int I0 = T0 + ti * kNB, K0 = T0 + tk * kNB;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < kNB * kNB; e += 256) {
-    int row = e % kNB, k = e / kNB;
-    li[k][row] = (k < jb && I0 + row < n) ? A[(long long)(J0 + k) * n + I0 + row] : 0.0;
-    lk[k][row] = (k < jb && K0 + row < n) ? A[(long long)(J0 + k) * n + K0 + row] : 0.0;
+  const int I0 = T0 + ti * SyrkTile::BM, K0 = T0 + tk * SyrkTile::BN;
+  const int mlim = min(SyrkTile::BM, n - I0), nlim = min(SyrkTile::BN, n - K0);
+  typename SyrkTile::Acc acc;
+  SyrkTile::zero(acc);
+  // A(m, k) = L[I0+m, J0+k]; B(k, c) = L[K0+c, J0+k] (k-major, ld n).
+  SyrkTile::run(A + (long long)J0 * n + I0, n, A + (long long)J0 * n + K0, n, jb, mlim, nlim, lds,
+                acc);
+#pragma unroll
+  for (int j = 0; j < SyrkTile::TN; j++) {
+    const int c = SyrkTile::col(j);
+    if (c >= nlim) continue;
+    double* Ac = A + (long long)(K0 + c) * n + I0;
+#pragma unroll
+    for (int i = 0; i < SyrkTile::TM; i++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = SyrkTile::row(i, r);
+        if (m < mlim && I0 + m >= K0 + c) Ac[m] -= acc[i][j][r];
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Theta = Z (L L^T)^{-1} in place, in block columns of kSB = 128.
+//
+// Forward (X L^T = B):  X_J = [X_0 .. X_{J-1}, B_J] F_J^T with
+//   F_JQ = -inv(L_JJ) L_JQ (Q < J), F_JJ = inv(L_JJ);  Ft holds F column-major.
+// Backward (Theta L = X): Theta_J = [X_J, Theta_{J+1} ..] G_J with
+//   G_PJ = -L_PJ inv(L_JJ) (P > J), G_JJ = inv(L_JJ);  Gt holds G row-major.
+// Each block column is then one GEMM over already-solved columns; a workgroup
+// owns a strip of rows and walks the block columns in order.  Block inverses of
+// 128 x 128 come from the Cholesky's 64 x 64 ones by the 2 x 2 block formula.
+// ---------------------------------------------------------------------------
+constexpr int kSB = 128;
+
+// Dinv128[J] = inv(L_JJ) for the 128-blocks: [[Da, 0], [-Db L_ba Da, Db]].
+__global__ __launch_bounds__(256) void k_dinv128(const double* __restrict__ L, int n,
+                                                 const double* __restrict__ Dinv64,
+                                                 double* __restrict__ Dinv128) {
+  __shared__ double da[kNB][kNB];   // da[col][row]
+  __shared__ double db[kNB][kNB];
+  __shared__ double lt[kNB][kNB];   // L_ba Da, [col][row]
+  const int J = blockIdx.x, J0 = J * kSB;
+  const int t = threadIdx.x;
+  const double* Da = Dinv64 + (long long)(2 * J) * kNB * kNB;
+  const double* Db = Dinv64 + (long long)(2 * J + 1) * kNB * kNB;
+  const bool hasb = J0 + kNB < n;
+  double* D = Dinv128 + (long long)J * kSB * kSB;
+  for (int e = t; e < kNB * kNB; e += 256) {
+    const int r = e % kNB, c = e / kNB;
+    da[c][r] = Da[e];
+    db[c][r] = hasb ? Db[e] : 0.0;
+    // L_ba(r, c) = L[J0+64+r, J0+c]
+    lt[c][r] = (hasb && J0 + kNB + r < n) ? L[(long long)(J0 + c) * n + J0 + kNB + r] : 0.0;
   }
   __syncthreads();
-  const int tx = tid & 15, ty = tid >> 4;
-  double s[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) s[a][b] = 0.0;
-  for (int k = 0; k < jb; k++) {
-    double x[4], y[4];
-#pragma unroll
-    for (int a = 0; a < 4; a++) x[a] = li[k][tx + 16 * a];
-#pragma unroll
-    for (int b = 0; b < 4; b++) y[b] = lk[k][ty + 16 * b];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-      for (int b = 0; b < 4; b++) s[a][b] += x[a] * y[b];
+  // T = L_ba Da (64 x 64): thread owns rows r = t % 64, columns c = t/64 + 4q.
+  double tv[16];
+  const int r = t % kNB;
+  for (int q = 0; q < 16; q++) {
+    const int c = t / kNB + 4 * q;
+    double s = 0.0;
+    for (int k = c; k < kNB; k++) s += lt[k][r] * da[c][k];
+    tv[q] = s;
   }
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
-    const int i = I0 + tx + 16 * a;
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const int k = K0 + ty + 16 * b;
-      if (i < n && k < n && k <= i) A[(long long)k * n + i] -= s[a][b];
-    }
+  __syncthreads();
+  for (int q = 0; q < 16; q++) lt[t / kNB + 4 * q][r] = tv[q];
+  __syncthreads();
+  // X = -Db T
+  for (int q = 0; q < 16; q++) {
+    const int c = t / kNB + 4 * q;
+    double s = 0.0;
+    for (int k = 0; k <= r; k++) s += db[k][r] * lt[c][k];
+    tv[q] = -s;
   }
+  for (int e = t; e < kSB * kSB; e += 256) {
+    const int rr = e % kSB, cc = e / kSB;
+    double v = 0.0;
+    if (rr < kNB && cc < kNB) v = da[cc][rr];
+    else if (rr >= kNB && cc >= kNB) v = db[cc - kNB][rr - kNB];
+    D[e] = v;
+  }
+  __syncthreads();
+  for (int q = 0; q < 16; q++) D[(long long)(t / kNB + 4 * q) * kSB + kNB + r] = tv[q];
 }
 
-// Solve operands.  Forward (X L^T = B): X_J = [X_0..X_{J-1}, B_J] F_J^T with
-// F_JQ = -inv(L_JJ) L_JQ (Q < J) and F_JJ = inv(L_JJ); stored Ft[k*n + j] = F[j][k].
-// Backward (Theta L = X): Theta_J = [X_J, Theta_{J+1}..] G_J with
-// G_PJ = -L_PJ inv(L_JJ) (P > J) and G_JJ = inv(L_JJ); stored Gt[k*n + j] = G[k][j].
-__global__ __launch_bounds__(256) void k_make_fg(const double* __restrict__ L, int n,
-                                                 const double* __restrict__ Dinv,
-                                                 double* __restrict__ Ft, double* __restrict__ Gt) {
+typedef MfmaTile<128, 128, 16, 2, 2, false> FgTile;
+
+// One workgroup per block pair (P, Q), Q <= P: F_PQ and G_PQ.
+__global__ __launch_bounds__(256, 1) void k_make_fg(const double* __restrict__ L, int n,
+                                                    const double* __restrict__ Dinv128,
+                                                    double* __restrict__ Ft,
+                                                    double* __restrict__ Gt) {
+  __shared__ double lds[FgTile::LDS_DOUBLES];
   const int P = blockIdx.x, Q = blockIdx.y;
   if (Q > P) return;
-  const int P0 = P * kNB, Q0 = Q * kNB;
-  const int pb = min(kNB, n - P0), qb = min(kNB, n - Q0);
-  const double* DP = Dinv + (long long)P * kNB * kNB;
-  const double* DQ = Dinv + (long long)Q * kNB * kNB;
-  for (int e = threadIdx.x; e < kNB * kNB; e += 256) {
-    const int i = e % kNB, jj = e / kNB;  // row in block P, column in block Q
-    if (i >= pb || jj >= qb) continue;
-    double f, g;
-    if (P == Q) {
-      f = DP[(long long)jj * kNB + i];
-      g = f;
-    } else {
-      double sf = 0.0, sg = 0.0;
-      for (int t = 0; t <= i; t++) sf += DP[(long long)t * kNB + i] * L[(long long)(Q0 + jj) * n + P0 + t];
-      for (int t = jj; t < qb; t++) sg += L[(long long)(Q0 + t) * n + P0 + i] * DQ[(long long)jj * kNB + t];
-      f = -sf;
-      g = -sg;
+  const int P0 = P * kSB, Q0 = Q * kSB;
+  const int pb = min(kSB, n - P0), qb = min(kSB, n - Q0);
+  const double* DP = Dinv128 + (long long)P * kSB * kSB;
+  const double* DQ = Dinv128 + (long long)Q * kSB * kSB;
+  if (P == Q) {
+    for (int e = threadIdx.x; e < kSB * kSB; e += 256) {
+      const int i = e % kSB, j = e / kSB;
+      if (i < pb && j < qb) {
+        const double v = DP[e];
+        Ft[(long long)(Q0 + j) * n + P0 + i] = v;
+        Gt[(long long)(P0 + i) * n + Q0 + j] = v;
+      }
     }
-    Ft[(long long)(Q0 + jj) * n + P0 + i] = f;
-    Gt[(long long)(P0 + i) * n + Q0 + jj] = g;
+    return;
+  }
+  typename FgTile::Acc acc;
+  // F_PQ = -Dinv_P L_PQ: A = Dinv_P (ld 128), B(k, j) = L[P0+k, Q0+j] (k contiguous, ld n).
+  FgTile::zero(acc);
+  FgTile::run(DP, kSB, L + (long long)Q0 * n + P0, n, pb, pb, qb, lds, acc);
+#pragma unroll
+  for (int j = 0; j < FgTile::TN; j++) {
+    const int jj = FgTile::col(j);
+#pragma unroll
+    for (int i = 0; i < FgTile::TM; i++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int ii = FgTile::row(i, r);
+        if (ii < pb && jj < qb) Ft[(long long)(Q0 + jj) * n + P0 + ii] = -acc[i][j][r];
+      }
+  }
+  // G_PQ = -L_PQ Dinv_Q: A(i, k) = L[P0+i, Q0+k] (ld n), B(k, j) = Dinv_Q[k][j] (k contiguous, ld 128).
+  FgTile::zero(acc);
+  FgTile::run(L + (long long)Q0 * n + P0, n, DQ, kSB, qb, pb, qb, lds, acc);
+#pragma unroll
+  for (int j = 0; j < FgTile::TN; j++) {
+    const int jj = FgTile::col(j);
+#pragma unroll
+    for (int i = 0; i < FgTile::TM; i++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int ii = FgTile::row(i, r);
+        if (ii < pb && jj < qb) Gt[(long long)(P0 + ii) * n + Q0 + jj] = -acc[i][j][r];
+      }
   }
 }
 
-// Theta = Z (L L^T)^{-1} in place, one wave per 32-row strip: the forward
-// sweep writes X_J over Z_J, the backward sweep writes Theta_J over X_J.
-// Each block column is one MFMA GEMM over the already-solved columns.
-__global__ __launch_bounds__(64) void k_potrs(double* __restrict__ T, long long ng, long long ldt,
-                                              int n, const double* __restrict__ Ft,
-                                              const double* __restrict__ Gt) {
-  constexpr int TM = 2, TN = kNB / 16;
-  const int lane = threadIdx.x;
-  const int lr = lane & 15, lk = lane >> 4;
-  const long long m0 = (long long)blockIdx.x * (16 * TM);
-  bool mv[TM];
-#pragma unroll
-  for (int i = 0; i < TM; i++) mv[i] = (m0 + 16 * i + lr) < ng;
-  const int nJ = (n + kNB - 1) / kNB;
-
+// The in-place solve.  A workgroup owns Tile::BM rows; Tile::BN == kSB.
+template <class Tile>
+__global__ __launch_bounds__(Tile::NT, 1) void k_potrs(double* __restrict__ T, long long ng,
+                                                       long long ldt, int n,
+                                                       const double* __restrict__ Ft,
+                                                       const double* __restrict__ Gt) {
+  static_assert(Tile::BN == kSB, "block column width");
+  __shared__ double lds[Tile::LDS_DOUBLES];
+  const long long m0 = (long long)blockIdx.x * Tile::BM;
+  const int mlim = (int)std::min<long long>(Tile::BM, ng - m0);
+  double* Tm = T + m0;
+  const int nJ = (n + kSB - 1) / kSB;
   for (int pass = 0; pass < 2; pass++) {
-    const double* __restrict__ Bop = pass == 0 ? Ft : Gt;
     for (int s = 0; s < nJ; s++) {
       const int J = pass == 0 ? s : nJ - 1 - s;
-      const int J0 = J * kNB;
-      const int jb = min(kNB, n - J0);
+      const int J0 = J * kSB;
+      const int jb = min(kSB, n - J0);
       const int kbeg = pass == 0 ? 0 : J0;
       const int kend = pass == 0 ? J0 + jb : n;
-      d4 acc[TM][TN];
+      typename Tile::Acc acc;
+      Tile::zero(acc);
+      const double* Bop = pass == 0 ? Ft : Gt;
+      Tile::run(Tm + (long long)kbeg * ldt, ldt, Bop + (long long)kbeg * n + J0, n, kend - kbeg,
+                mlim, jb, lds, acc);
+      // Tile::run ends with a barrier: every read of block column J is done.
 #pragma unroll
-      for (int i = 0; i < TM; i++)
+      for (int j = 0; j < Tile::TN; j++) {
+        const int c = Tile::col(j);
+        if (c >= jb) continue;
+        double* Tc = Tm + (long long)(J0 + c) * ldt;
 #pragma unroll
-        for (int j = 0; j < TN; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-      bool nv[TN];
-#pragma unroll
-      for (int j = 0; j < TN; j++) nv[j] = (16 * j + lr) < jb;
-      // Forward: K runs over columns [0, J0+jb); backward: over [J0, n).
-      const double* Ap = T + (long long)kbeg * ldt;
-      const double* Bp = Bop + (long long)kbeg * n + J0;
-      mfma_panel<TM, TN>(Ap, ldt, Bp, n, kend - kbeg, m0, 0, lr, lk, mv, nv, acc);
-#pragma unroll
-      for (int i = 0; i < TM; i++)
-#pragma unroll
-        for (int j = 0; j < TN; j++) {
-          if (!nv[j]) continue;
+        for (int i = 0; i < Tile::TM; i++)
 #pragma unroll
           for (int r = 0; r < 4; r++) {
-            const long long m = m0 + 16 * i + lk + 4 * r;
-            if (m < ng) T[m + (long long)(J0 + 16 * j + lr) * ldt] = acc[i][j][r];
+            const int m = Tile::row(i, r);
+            if (m < mlim) Tc[m] = acc[i][j][r];
           }
-        }
-      // The next block column reads what this one wrote (same wave, same rows).
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      }
+      // The next block column reads what this one wrote.
+      __syncthreads();
     }
   }
 }
+
+// Tile variants of the solve (chosen per shape, see launch_potrs).
+typedef MfmaTile<256, kSB, 16, 4, 2> PotrsTile0;  // 8 waves of 64 x 64
+typedef MfmaTile<256, kSB, 16, 4, 1> PotrsTile1;  // 4 waves of 64 x 128
+typedef MfmaTile<128, kSB, 16, 2, 2> PotrsTile2;  // 4 waves of 64 x 64
+typedef MfmaTile<128, kSB, 32, 2, 2> PotrsTile3;  // 4 waves of 64 x 64, 32-deep chunks
 
 }  // namespace
 
@@ -393,23 +461,28 @@ int launch_gather_psimu(const double* psi, long long ld, int nstate, const int* 
   return 0;
 }
 
+int zct_variants() { return 3; }
+
 int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, int rk, int s1,
-               int K1, int s2, int K2, bool dual, double* C, long long ldc, hipStream_t st) {
+               int K1, int s2, int K2, bool dual, double* C, long long ldc, int variant,
+               hipStream_t st) {
   const double* A1 = psi + (long long)s1 * ld;
   const double* A2 = psi + (long long)s2 * ld;
   const double* B1 = psimuT + (long long)s1 * rk;
   const double* B2 = psimuT + (long long)s2 * rk;
+  if (variant < 0) variant = (ng >= 256LL * 256) ? 0 : 2;
+#define ISDF_ZCT_LAUNCH(TL, DUALV)                                                              \
+  hipLaunchKernelGGL((k_zct<TL, DUALV>), dim3(cdiv(ng, TL::BM) * cdiv(rk, TL::BN)), dim3(TL::NT), \
+                     0, st, ng, rk, A1, A2, ld, K1, K2, B1, B2, rk, C, ldc)
   if (!dual) {
-    constexpr int TM = 4, TN = 4;
-    dim3 g(cdiv(ng, 32 * TM), cdiv(rk, 32 * TN));
-    hipLaunchKernelGGL((k_zct<TM, TN, false>), g, dim3(256), 0, st, ng, rk, A1, A1, ld, K1, K1, B1,
-                       B1, rk, C, ldc);
+    if (variant == 0) ISDF_ZCT_LAUNCH(ZctTile0, false);
+    else if (variant == 1) ISDF_ZCT_LAUNCH(ZctTile1, false);
+    else ISDF_ZCT_LAUNCH(ZctTile2, false);
   } else {
-    constexpr int TM = 4, TN = 2;
-    dim3 g(cdiv(ng, 32 * TM), cdiv(rk, 32 * TN));
-    hipLaunchKernelGGL((k_zct<TM, TN, true>), g, dim3(256), 0, st, ng, rk, A1, A2, ld, K1, K2, B1,
-                       B2, rk, C, ldc);
+    if (variant == 2) ISDF_ZCT_LAUNCH(ZctTileDual1, true);
+    else ISDF_ZCT_LAUNCH(ZctTileDual0, true);
   }
+#undef ISDF_ZCT_LAUNCH
   ISDF_HIP_LAUNCHED("k_zct");
   return 0;
 }
@@ -433,28 +506,46 @@ int launch_potrf(double* P, int rk, double* Dinv, int* info_dev, hipStream_t st)
     ISDF_HIP_LAUNCHED("k_potrf_diag");
     const int rest = rk - J0 - jb;
     if (rest > 0) {
-      hipLaunchKernelGGL(k_potrf_panel, dim3(cdiv(rest, kNB)), dim3(64), 0, st, P, rk, J0, Dinv,
-                         info_dev);
+      hipLaunchKernelGGL(k_potrf_panel, dim3(cdiv(rest, PanelTile::BM)), dim3(PanelTile::NT), 0,
+                         st, P, rk, J0, Dinv, info_dev);
       ISDF_HIP_LAUNCHED("k_potrf_panel");
-      const unsigned t = cdiv(rest, kNB);
-      hipLaunchKernelGGL(k_potrf_syrk, dim3(t, t), dim3(256), 0, st, P, rk, J0, info_dev);
+      const unsigned t = cdiv(rest, SyrkTile::BM);
+      hipLaunchKernelGGL(k_potrf_syrk, dim3(t, t), dim3(SyrkTile::NT), 0, st, P, rk, J0, info_dev);
       ISDF_HIP_LAUNCHED("k_potrf_syrk");
     }
   }
   return 0;
 }
 
-int launch_make_fg(const double* L, int rk, const double* Dinv, double* Ft, double* Gt,
-                   hipStream_t st) {
-  const unsigned nJ = cdiv(rk, kNB);
-  hipLaunchKernelGGL(k_make_fg, dim3(nJ, nJ), dim3(256), 0, st, L, rk, Dinv, Ft, Gt);
+int launch_make_fg(const double* L, int rk, const double* Dinv, double* Dinv128, double* Ft,
+                   double* Gt, hipStream_t st) {
+  const unsigned nJ = cdiv(rk, kSB);
+  hipLaunchKernelGGL(k_dinv128, dim3(nJ), dim3(256), 0, st, L, rk, Dinv, Dinv128);
+  ISDF_HIP_LAUNCHED("k_dinv128");
+  hipLaunchKernelGGL(k_make_fg, dim3(nJ, nJ), dim3(256), 0, st, L, rk, Dinv128, Ft, Gt);
   ISDF_HIP_LAUNCHED("k_make_fg");
   return 0;
 }
 
+int potrs_variants() { return 4; }
+
 int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* Ft, const double* Gt,
-                 hipStream_t st) {
-  hipLaunchKernelGGL(k_potrs, dim3(cdiv(ng, 32)), dim3(64), 0, st, T, ng, ldt, rk, Ft, Gt);
+                 int variant, hipStream_t st) {
+  if (variant < 0) variant = (ng >= 256LL * 512) ? 0 : 2;
+  switch (variant) {
+#define ISDF_POTRS_CASE(V)                                                                      \
+  case V:                                                                                       \
+    hipLaunchKernelGGL((k_potrs<PotrsTile##V>), dim3(cdiv(ng, PotrsTile##V::BM)),               \
+                       dim3(PotrsTile##V::NT), 0, st, T, ng, ldt, rk, Ft, Gt);                  \
+    break;
+    ISDF_POTRS_CASE(0)
+    ISDF_POTRS_CASE(1)
+    ISDF_POTRS_CASE(2)
+    ISDF_POTRS_CASE(3)
+#undef ISDF_POTRS_CASE
+    default:
+      return set_error(1, "unknown solve tile variant");
+  }
   ISDF_HIP_LAUNCHED("k_potrs");
   return 0;
 }

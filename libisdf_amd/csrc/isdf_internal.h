@@ -36,6 +36,10 @@ int set_error(int code, const std::string& msg);
 // Device buffers of one K-means run.  n = Ng points, rkmax = initial rk.
 struct KmeansBufs {
   int n, rkmax, nchunk_sort, nchunk_final, chunk_sort, chunk_final, nblk_flag;
+  int ngrid[3];      // grid dimensions (x fastest)
+  double dx[3];      // grid spacing L/n
+  int nblk_pts;      // point blocks of the assignment (bricks, or chunks of 256)
+  bool grid_mode;    // every grid point is above the threshold (npt == Ng)
   double* w;         // [n]   density weights
   uint8_t* flags;    // [n]   threshold / center flags
   int* index;        // [n]   points above the threshold (ascending)
@@ -56,6 +60,9 @@ struct KmeansBufs {
   int* imin;         // [rkmax]
   int* centers;      // [rkmax]
   int* trace;        // [max_iter]
+  int* porder;       // [nblk_pts * 256] point of each brick slot (-1: none)
+  int* changed_blk;  // [nblk_pts] labels changed per point block
+  int* bbox;         // [6 * rkmax] cluster bounding boxes in grid indices
   int* scal;         // small device scalars, see KS_* below
 };
 
@@ -70,8 +77,8 @@ enum {
   KS_COUNT = 8
 };
 
-size_t kmeans_bytes(int n, int rkmax, int max_iter);
-void kmeans_carve(KmeansBufs* b, void* base, int n, int rkmax, int max_iter);
+size_t kmeans_bytes(int n, const int ngrid[3], int rkmax, int max_iter);
+void kmeans_carve(KmeansBufs* b, void* base, int n, const int ngrid[3], int rkmax, int max_iter);
 
 // Weights w(r) = (sum_{S2} psi^2)(sum_{S1} psi^2) for all grid points.
 int launch_weights(const double* psi, long long ld, int ng, int s1lo, int s1hi, int s2lo,
@@ -94,16 +101,20 @@ int kmeans_finish(KmeansBufs* b, int* piv_dev, hipStream_t st);
 // ---------------------------------------------------------------------------
 int launch_gather_psimu(const double* psi, long long ld, int nstate, const int* piv, int rk,
                         double* psimuT, hipStream_t st);
+// `variant` < 0 picks the default tile for the shape; 0..zct_variants()-1 forces one.
+int zct_variants();
 int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, int rk, int s1,
-               int K1, int s2, int K2, bool dual, double* C, long long ldc, hipStream_t st);
+               int K1, int s2, int K2, bool dual, double* C, long long ldc, int variant,
+               hipStream_t st);
 int launch_cct(const double* C, long long ldc, const int* piv, int rk, double* P,
                double* eps_dev, hipStream_t st);
 int launch_potrf(double* P, int rk, double* Dinv, int* info_dev, hipStream_t st);
-int launch_make_fg(const double* L, int rk, const double* Dinv, double* Ft, double* Gt,
-                   hipStream_t st);
+int launch_make_fg(const double* L, int rk, const double* Dinv, double* Dinv128, double* Ft,
+                   double* Gt, hipStream_t st);
+int potrs_variants();
 int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* Ft,
-                 const double* Gt, hipStream_t st);
+                 const double* Gt, int variant, hipStream_t st);
 
-constexpr int kNB = 64;  // block size of the Cholesky and of the triangular solves
+constexpr int kNB = 64;  // block size of the Cholesky (the solves use 128-blocks)
 
 }  // namespace isdf_hip

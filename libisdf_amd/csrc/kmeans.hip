@@ -230,38 +230,177 @@ __global__ __launch_bounds__(256) void k_init_centroids(int rk, int n, int npt,
   cz[c] = z;
 }
 
-// Fused pdist2 + findMin(Dim=1) (utility.cpp:598-615, :535-562): first index
-// of the minimum over the current centroids.  Optionally counts changed labels.
-__global__ __launch_bounds__(256) void k_assign(int n, const double* __restrict__ X,
-                                                const double* __restrict__ Y,
-                                                const double* __restrict__ Z,
-                                                const double* __restrict__ cx,
-                                                const double* __restrict__ cy,
-                                                const double* __restrict__ cz,
-                                                int* __restrict__ label, int* scal,
-                                                int count_changes) {
+// Brick order of the grid for the assignment: brick b (8 x 8 x 4 points,
+// bricks numbered x fastest), slot s -> grid point or -1 off the grid.  When
+// some grid points are below the density threshold the blocks are plain runs
+// of 256 local point indices instead.
+__global__ __launch_bounds__(256) void k_build_blocks(int n, int grid_mode, int n0, int n1, int n2,
+                                                     int* __restrict__ porder) {
+  const int b = blockIdx.x, s = threadIdx.x;
+  int v;
+  if (grid_mode) {
+    const int nbx = (n0 + 7) / 8, nby = (n1 + 7) / 8;
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int x = bx * 8 + (s & 7), y = by * 8 + ((s >> 3) & 7), z = bz * 4 + (s >> 6);
+    v = (x < n0 && y < n1 && z < n2) ? x + n0 * (y + n1 * z) : -1;
+  } else {
+    const int i = b * 256 + s;
+    v = i < n ? i : -1;
+  }
+  porder[(long long)b * 256 + s] = v;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+  for (int d = 32; d > 0; d >>= 1) v = fmin(v, __shfl_xor(v, d));
+  return v;
+}
+
+// Minimum of one double per thread over a 256-thread block.
+__device__ double block_min256(double v, double* red) {
+  v = wave_min(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  const double r = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
+  __syncthreads();
+  return r;
+}
+
+constexpr int kCandCap = 256;
+
+// Fused pdist2 + findMin(Dim=1) (utility.cpp:598-615, :535-562) with
+// conservative culling.  A workgroup takes one block of up to 256 points,
+// bounds their coordinates by a box, and keeps only the centroids whose lower
+// bound LB on the distance to the box is <= U (1 + 1e-9), U the smallest upper
+// bound over all centroids.  A centroid left out is, for every point of the
+// block, farther than the nearest centroid by a relative 1e-9, far beyond the
+// few ulps of rounding in the distance, so it can neither win nor tie; the
+// first minimum over the candidates, kept in ascending order, is therefore
+// the reference's first minimum over all centroids.
+// In grid mode the workgroup also folds its points into the bounding boxes
+// (grid indices) of their clusters, which the centroid sums walk.
+__global__ __launch_bounds__(256) void k_assign_culled(
+    const int* __restrict__ porder, const double* __restrict__ X, const double* __restrict__ Y,
+    const double* __restrict__ Z, const double* __restrict__ cx, const double* __restrict__ cy,
+    const double* __restrict__ cz, int* __restrict__ label, int* scal, int count_changes,
+    int* __restrict__ bbox, int rkmax, int grid_mode, int n0, int n1,
+    int* __restrict__ changed_blk) {
+  __shared__ int cand[kCandCap];
+  __shared__ double ccx[kCandCap], ccy[kCandCap], ccz[kCandCap];
+  __shared__ int cbox[6][kCandCap];
+  __shared__ double red[8];
+  __shared__ double rbox[6][4];
   if (count_changes && scal[KS_ACTIVE] == 0) return;
   const int rk = scal[KS_RK];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = i < n;
-  int ii = valid ? i : 0;
-  double px = X[ii], py = Y[ii], pz = Z[ii];
-  double best = dist2(px, py, pz, cx[0], cy[0], cz[0]);
-  int bi = 0;
-  int c = 1;
-  for (; c + 3 < rk; c += 4) {
-    double d0 = dist2(px, py, pz, cx[c], cy[c], cz[c]);
-    double d1 = dist2(px, py, pz, cx[c + 1], cy[c + 1], cz[c + 1]);
-    double d2 = dist2(px, py, pz, cx[c + 2], cy[c + 2], cz[c + 2]);
-    double d3 = dist2(px, py, pz, cx[c + 3], cy[c + 3], cz[c + 3]);
-    if (d0 < best) { best = d0; bi = c; }
-    if (d1 < best) { best = d1; bi = c + 1; }
-    if (d2 < best) { best = d2; bi = c + 2; }
-    if (d3 < best) { best = d3; bi = c + 3; }
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int i = porder[(long long)blockIdx.x * 256 + t];
+  const bool valid = i >= 0;
+  const double px = valid ? X[i] : 0.0, py = valid ? Y[i] : 0.0, pz = valid ? Z[i] : 0.0;
+
+  // Bounding box of the block's points.
+  {
+    double v[6] = {valid ? px : 1e300,  valid ? py : 1e300,  valid ? pz : 1e300,
+                   valid ? -px : 1e300, valid ? -py : 1e300, valid ? -pz : 1e300};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const double m = wave_min(v[k]);
+      if (lane == 0) rbox[k][wave] = m;
+    }
   }
-  for (; c < rk; c++) {
-    double d = dist2(px, py, pz, cx[c], cy[c], cz[c]);
-    if (d < best) { best = d; bi = c; }
+  __syncthreads();
+  double lo[3], hi[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    lo[k] = fmin(fmin(rbox[k][0], rbox[k][1]), fmin(rbox[k][2], rbox[k][3]));
+    hi[k] = -fmin(fmin(rbox[k + 3][0], rbox[k + 3][1]), fmin(rbox[k + 3][2], rbox[k + 3][3]));
+  }
+  if (lo[0] > 1e299) {  // no point in this block (uniform)
+    if (count_changes && t == 0) changed_blk[blockIdx.x] = 0;
+    return;
+  }
+
+  // Thread t bounds the contiguous centroid range [t*q, t*q + q): one pass for
+  // the smallest upper bound, one for the lower-bound test, one scan, so the
+  // candidate list comes out in ascending centroid order.
+  const int q = (rk + 255) / 256;
+  const int c0 = min(rk, t * q), c1 = min(rk, c0 + q);
+  double ub = 1e300;
+  for (int c = c0; c < c1; c++) {
+    const double ex = fmax(fabs(cx[c] - lo[0]), fabs(cx[c] - hi[0]));
+    const double ey = fmax(fabs(cy[c] - lo[1]), fabs(cy[c] - hi[1]));
+    const double ez = fmax(fabs(cz[c] - lo[2]), fabs(cz[c] - hi[2]));
+    ub = fmin(ub, ex * ex + ey * ey + ez * ez);
+  }
+  const double thr = block_min256(ub, red) * (1.0 + 1e-9);
+  unsigned long long keepmask = 0ull;  // q <= 32 (rk <= 8192)
+  int mycnt = 0;
+  for (int c = c0; c < c1; c++) {
+    const double ex = fmax(0.0, fmax(lo[0] - cx[c], cx[c] - hi[0]));
+    const double ey = fmax(0.0, fmax(lo[1] - cy[c], cy[c] - hi[1]));
+    const double ez = fmax(0.0, fmax(lo[2] - cz[c], cz[c] - hi[2]));
+    if (ex * ex + ey * ey + ez * ez <= thr) {
+      keepmask |= 1ull << (c - c0);
+      mycnt++;
+    }
+  }
+  int nc;
+  {
+    int pos = block_excl_scan(mycnt, &nc);
+    for (int c = c0; c < c1; c++) {
+      if (!((keepmask >> (c - c0)) & 1ull)) continue;
+      if (pos < kCandCap) {
+        cand[pos] = c;
+        ccx[pos] = cx[c];
+        ccy[pos] = cy[c];
+        ccz[pos] = cz[c];
+      }
+      pos++;
+    }
+  }
+  const bool culled = nc <= kCandCap;
+  if (culled && grid_mode) {
+    for (int k = t; k < nc; k += 256) {
+      cbox[0][k] = 0x7fffffff;
+      cbox[1][k] = 0x7fffffff;
+      cbox[2][k] = 0x7fffffff;
+      cbox[3][k] = -1;
+      cbox[4][k] = -1;
+      cbox[5][k] = -1;
+    }
+  }
+  __syncthreads();
+
+  int bk = 0, bi = 0;
+  if (culled) {
+    double best = dist2(px, py, pz, ccx[0], ccy[0], ccz[0]);
+    int k = 1;
+    for (; k + 3 < nc; k += 4) {
+      const double d0 = dist2(px, py, pz, ccx[k], ccy[k], ccz[k]);
+      const double d1 = dist2(px, py, pz, ccx[k + 1], ccy[k + 1], ccz[k + 1]);
+      const double d2 = dist2(px, py, pz, ccx[k + 2], ccy[k + 2], ccz[k + 2]);
+      const double d3 = dist2(px, py, pz, ccx[k + 3], ccy[k + 3], ccz[k + 3]);
+      if (d0 < best) { best = d0; bk = k; }
+      if (d1 < best) { best = d1; bk = k + 1; }
+      if (d2 < best) { best = d2; bk = k + 2; }
+      if (d3 < best) { best = d3; bk = k + 3; }
+    }
+    for (; k < nc; k++) {
+      const double d = dist2(px, py, pz, ccx[k], ccy[k], ccz[k]);
+      if (d < best) {
+        best = d;
+        bk = k;
+      }
+    }
+    bi = cand[bk];
+  } else {
+    double best = dist2(px, py, pz, cx[0], cy[0], cz[0]);
+    for (int c = 1; c < rk; c++) {
+      const double d = dist2(px, py, pz, cx[c], cy[c], cz[c]);
+      if (d < best) {
+        best = d;
+        bi = c;
+      }
+    }
   }
   bool changed = false;
   if (valid) {
@@ -269,9 +408,178 @@ __global__ __launch_bounds__(256) void k_assign(int n, const double* __restrict_
     label[i] = bi;
   }
   if (count_changes) {
-    unsigned long long m = __ballot(changed);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&scal[KS_CHANGED], (int)__popcll(m));
+    // Per-block count; the loop control sums the blocks (no contended atomics).
+    int tot;
+    block_excl_scan(changed ? 1 : 0, &tot);
+    if (t == 0) changed_blk[blockIdx.x] = tot;
   }
+  if (!grid_mode) return;
+  int gx = 0, gy = 0, gz = 0;
+  if (valid) {
+    gx = i % n0;
+    gy = (i / n0) % n1;
+    gz = i / (n0 * n1);
+  }
+  if (culled) {
+    if (valid) {
+      atomicMin(&cbox[0][bk], gx);
+      atomicMin(&cbox[1][bk], gy);
+      atomicMin(&cbox[2][bk], gz);
+      atomicMax(&cbox[3][bk], gx);
+      atomicMax(&cbox[4][bk], gy);
+      atomicMax(&cbox[5][bk], gz);
+    }
+    __syncthreads();
+    for (int k = t; k < nc; k += 256) {
+      if (cbox[3][k] < 0) continue;
+      const int c = cand[k];
+      atomicMin(&bbox[c], cbox[0][k]);
+      atomicMin(&bbox[rkmax + c], cbox[1][k]);
+      atomicMin(&bbox[2 * rkmax + c], cbox[2][k]);
+      atomicMax(&bbox[3 * rkmax + c], cbox[3][k]);
+      atomicMax(&bbox[4 * rkmax + c], cbox[4][k]);
+      atomicMax(&bbox[5 * rkmax + c], cbox[5][k]);
+    }
+  } else if (valid) {
+    atomicMin(&bbox[bi], gx);
+    atomicMin(&bbox[rkmax + bi], gy);
+    atomicMin(&bbox[2 * rkmax + bi], gz);
+    atomicMax(&bbox[3 * rkmax + bi], gx);
+    atomicMax(&bbox[4 * rkmax + bi], gy);
+    atomicMax(&bbox[5 * rkmax + bi], gz);
+  }
+}
+
+__global__ void k_reset_boxes(int* bbox, int rkmax) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < 6 * rkmax; e += gridDim.x * blockDim.x)
+    bbox[e] = (e < 3 * rkmax) ? 0x7fffffff : -1;
+}
+
+// Centroid sums, utility.cpp:816-822, when every grid point is a K-means point
+// (local index == grid index): one workgroup per cluster walks the cluster's
+// bounding box in ascending grid order (z, y, x), 1024 positions per round.
+// Thread t tests positions 4t..4t+3 of the round, members are compacted in
+// position order through a block scan, and their products w*x (one rounding
+// each, as in the reference) land in LDS; threads 0..3 then carry the four
+// sequential sums over the round's members in that order.
+__global__ __launch_bounds__(256) void k_centroid_sums_box(
+    int n0, int n1, const int* __restrict__ label, const double* __restrict__ WL,
+    const double* __restrict__ X, const double* __restrict__ Y, const double* __restrict__ Z,
+    const int* __restrict__ bbox, int rkmax, double* __restrict__ sw, double* __restrict__ sx,
+    double* __restrict__ sy, double* __restrict__ sz, const int* scal) {
+  __shared__ double buf[4][1024];
+  if (scal[KS_ACTIVE] == 0) return;
+  const int rk = scal[KS_RK];
+  const int c = blockIdx.x;
+  if (c >= rk) return;
+  const int t = threadIdx.x;
+  const int x0 = bbox[c], y0 = bbox[rkmax + c], z0 = bbox[2 * rkmax + c];
+  const int x1 = bbox[3 * rkmax + c], y1 = bbox[4 * rkmax + c], z1 = bbox[5 * rkmax + c];
+  double acc = 0.0;
+  if (x1 >= x0) {
+    const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
+    const long long plane = (long long)bw * bh;
+    const long long total = plane * (z1 - z0 + 1);
+    for (long long j0 = 0; j0 < total; j0 += 1024) {
+      int g[4];
+      bool mem[4];
+      int cnt = 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const long long j = j0 + 4 * t + q;
+        mem[q] = false;
+        g[q] = 0;
+        if (j < total) {
+          const int z = z0 + (int)(j / plane);
+          const long long r = j % plane;
+          const int y = y0 + (int)(r / bw);
+          const int x = x0 + (int)(r % bw);
+          g[q] = x + n0 * (y + n1 * z);
+          mem[q] = label[g[q]] == c;
+          cnt += mem[q] ? 1 : 0;
+        }
+      }
+      int tot;
+      int pos = block_excl_scan(cnt, &tot);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (mem[q]) {
+          const int gg = g[q];
+          const double wv = WL[gg];
+          buf[0][pos] = wv;
+          buf[1][pos] = wv * X[gg];
+          buf[2][pos] = wv * Y[gg];
+          buf[3][pos] = wv * Z[gg];
+          pos++;
+        }
+      }
+      __syncthreads();
+      if (t < 4) {
+        const double* qv = buf[t];
+        int e = 0;
+        for (; e + 3 < tot; e += 4) {
+          acc = acc + qv[e];
+          acc = acc + qv[e + 1];
+          acc = acc + qv[e + 2];
+          acc = acc + qv[e + 3];
+        }
+        for (; e < tot; e++) acc = acc + qv[e];
+      }
+      __syncthreads();
+    }
+  }
+  if (t == 0) sw[c] = acc;
+  if (t == 1) sx[c] = acc;
+  if (t == 2) sy[c] = acc;
+  if (t == 3) sz[c] = acc;
+}
+
+// findMin(Dim=0) (utility.cpp:926-945) when every grid point is a K-means
+// point.  The grid point nearest in each coordinate is within dx/2, so the
+// nearest point of centroid c lies within sqrt(3)/2 max(dx) of it in every
+// coordinate, and so does any point at the same distance: one wave scans that
+// window (plus one cell for rounding) and keeps the lexicographic minimum of
+// (distance, grid index), i.e. the first minimum in grid order.
+__global__ __launch_bounds__(256) void k_final_window(int n0, int n1, int n2, double dx0,
+                                                      double dx1, double dx2,
+                                                      const double* __restrict__ X,
+                                                      const double* __restrict__ Y,
+                                                      const double* __restrict__ Z,
+                                                      const double* __restrict__ cx,
+                                                      const double* __restrict__ cy,
+                                                      const double* __restrict__ cz,
+                                                      int* __restrict__ imin, const int* scal) {
+  const int rk = scal[KS_RK];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + wave;
+  if (c >= rk) return;
+  const double qx = cx[c], qy = cy[c], qz = cz[c];
+  const double rho = 0.8660254037844387 * fmax(dx0, fmax(dx1, dx2)) * (1.0 + 1e-9);
+  const int x0 = max(0, (int)floor((qx - rho) / dx0) - 1), x1 = min(n0 - 1, (int)ceil((qx + rho) / dx0) + 1);
+  const int y0 = max(0, (int)floor((qy - rho) / dx1) - 1), y1 = min(n1 - 1, (int)ceil((qy + rho) / dx1) + 1);
+  const int z0 = max(0, (int)floor((qz - rho) / dx2) - 1), z1 = min(n2 - 1, (int)ceil((qz + rho) / dx2) + 1);
+  const int bw = max(0, x1 - x0 + 1), bh = max(0, y1 - y0 + 1), bd = max(0, z1 - z0 + 1);
+  const int total = bw * bh * bd;
+  double best = 1e300;
+  int bg = 0x7fffffff;
+  for (int j = lane; j < total; j += 64) {
+    const int x = x0 + j % bw, y = y0 + (j / bw) % bh, z = z0 + j / (bw * bh);
+    const int g = x + n0 * (y + n1 * z);
+    const double d = dist2(X[g], Y[g], Z[g], qx, qy, qz);
+    if (d < best || (d == best && g < bg)) {
+      best = d;
+      bg = g;
+    }
+  }
+  for (int dd = 32; dd > 0; dd >>= 1) {
+    const double ob = __shfl_xor(best, dd);
+    const int og = __shfl_xor(bg, dd);
+    if (ob < best || (ob == best && og < bg)) {
+      best = ob;
+      bg = og;
+    }
+  }
+  if (lane == 0) imin[c] = bg;
 }
 
 // Stable counting sort of point indices by label: chunk histograms.
@@ -424,10 +732,13 @@ __global__ __launch_bounds__(1024) void k_update(const double* __restrict__ sw,
                                                  const double* __restrict__ sz,
                                                  double* __restrict__ cx, double* __restrict__ cy,
                                                  double* __restrict__ cz, int* __restrict__ map,
-                                                 int* scal) {
+                                                 int* scal, int* __restrict__ bbox, int rkmax) {
   __shared__ int nempty, rknew;
   if (scal[KS_ACTIVE] == 0) return;
   const int rk = scal[KS_RK];
+  if (bbox)  // the next assignment rebuilds the cluster boxes
+    for (int e = threadIdx.x; e < 6 * rkmax; e += blockDim.x)
+      bbox[e] = (e < 3 * rkmax) ? 0x7fffffff : -1;
   if (threadIdx.x == 0) nempty = 0;
   __syncthreads();
   int e = 0;
@@ -473,13 +784,21 @@ __global__ __launch_bounds__(1024) void k_update(const double* __restrict__ sw,
   }
 }
 
-// Loop control, utility.cpp:813 and :906-921.
-__global__ void k_control(int s, int max_iter, double maxF, int* trace, int* scal) {
+// Loop control, utility.cpp:813 and :906-921: the changed-label count is the
+// sum of the assignment's per-block counts.
+__global__ __launch_bounds__(1024) void k_control(int s, int max_iter, double maxF, int nblk,
+                                                  const int* __restrict__ changed_blk, int* trace,
+                                                  int* scal) {
   if (scal[KS_ACTIVE] == 0) return;
-  int f = scal[KS_CHANGED];
-  trace[s] = f;
-  scal[KS_ITERS] = s + 1;
-  if (!((double)f > maxF && s + 1 < max_iter)) scal[KS_ACTIVE] = 0;
+  int v = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) v += changed_blk[b];
+  int f;
+  block_excl_scan(v, &f);
+  if (threadIdx.x == 0) {
+    trace[s] = f;
+    scal[KS_ITERS] = s + 1;
+    if (!((double)f > maxF && s + 1 < max_iter)) scal[KS_ACTIVE] = 0;
+  }
 }
 
 // findMin(Dim=0), utility.cpp:926-945: per centroid, the first point at the
@@ -614,13 +933,13 @@ int compact_flags(KmeansBufs* b, int want, int base, int* out, hipStream_t st) {
 
 }  // namespace
 
-size_t kmeans_bytes(int n, int rkmax, int max_iter) {
+size_t kmeans_bytes(int n, const int ngrid[3], int rkmax, int max_iter) {
   KmeansBufs b;
-  kmeans_carve(&b, nullptr, n, rkmax, max_iter);
+  kmeans_carve(&b, nullptr, n, ngrid, rkmax, max_iter);
   return (size_t)(uintptr_t)b.scal + 64;
 }
 
-void kmeans_carve(KmeansBufs* b, void* basep, int n, int rkmax, int max_iter) {
+void kmeans_carve(KmeansBufs* b, void* basep, int n, const int ngrid[3], int rkmax, int max_iter) {
   uintptr_t p = (uintptr_t)basep;
   auto take = [&](size_t bytes) {
     uintptr_t r = p;
@@ -664,6 +983,16 @@ void kmeans_carve(KmeansBufs* b, void* basep, int n, int rkmax, int max_iter) {
   b->imin = (int*)take(sizeof(int) * rkmax);
   b->centers = (int*)take(sizeof(int) * rkmax);
   b->trace = (int*)take(sizeof(int) * std::max(1, max_iter));
+  b->nblk_pts = std::max((int)cdiv(n, 256), 1);
+  {
+    // point-block table: the larger of the brick tiling and 256-point runs
+    const long long bricks =
+        (long long)((ngrid[0] + 7) / 8) * ((ngrid[1] + 7) / 8) * ((ngrid[2] + 3) / 4);
+    const long long blocks = std::max<long long>(bricks, (long long)cdiv(n, 256));
+    b->porder = (int*)take(sizeof(int) * (size_t)(blocks * 256));
+    b->changed_blk = (int*)take(sizeof(int) * (size_t)blocks);
+  }
+  b->bbox = (int*)take(sizeof(int) * 6 * (size_t)rkmax);
   b->scal = (int*)take(sizeof(int) * KS_COUNT);
 }
 
@@ -713,15 +1042,33 @@ int kmeans_setup_points(KmeansBufs* b, const int ngrid[3], const double length[3
                         int rk, hipStream_t st) {
   const int n = b->n;
   double dx0 = length[0] / ngrid[0], dx1 = length[1] / ngrid[1], dx2 = length[2] / ngrid[2];
+  for (int d = 0; d < 3; d++) {
+    b->ngrid[d] = ngrid[d];
+    b->dx[d] = length[d] / ngrid[d];
+  }
+  b->grid_mode = (npt == n);
+  if (b->grid_mode) {
+    const int nbx = (ngrid[0] + 7) / 8, nby = (ngrid[1] + 7) / 8, nbz = (ngrid[2] + 3) / 4;
+    b->nblk_pts = nbx * nby * nbz;
+  } else {
+    b->nblk_pts = (int)cdiv(n, 256);
+  }
   hipLaunchKernelGGL(k_setup_points, dim3(cdiv(n, 256)), dim3(256), 0, st, n, npt, b->index, b->w,
                      ngrid[0], ngrid[1], dx0, dx1, dx2, b->gidx, b->X, b->Y, b->Z, b->WL);
   ISDF_HIP_LAUNCHED("k_setup_points");
   hipLaunchKernelGGL(k_init_centroids, dim3(cdiv(rk, 256)), dim3(256), 0, st, rk, n, npt, b->index,
                      ngrid[0], ngrid[1], dx0, dx1, dx2, b->cx, b->cy, b->cz, b->scal);
   ISDF_HIP_LAUNCHED("k_init_centroids");
-  hipLaunchKernelGGL(k_assign, dim3(cdiv(n, 256)), dim3(256), 0, st, n, b->X, b->Y, b->Z, b->cx,
-                     b->cy, b->cz, b->label, b->scal, 0);
-  ISDF_HIP_LAUNCHED("k_assign");
+  hipLaunchKernelGGL(k_build_blocks, dim3(b->nblk_pts), dim3(256), 0, st, n, b->grid_mode ? 1 : 0,
+                     ngrid[0], ngrid[1], ngrid[2], b->porder);
+  ISDF_HIP_LAUNCHED("k_build_blocks");
+  hipLaunchKernelGGL(k_reset_boxes, dim3(cdiv(6 * (long long)b->rkmax, 256)), dim3(256), 0, st,
+                     b->bbox, b->rkmax);
+  ISDF_HIP_LAUNCHED("k_reset_boxes");
+  hipLaunchKernelGGL(k_assign_culled, dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X, b->Y,
+                     b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 0, b->bbox, b->rkmax,
+                     b->grid_mode ? 1 : 0, ngrid[0], ngrid[1], b->changed_blk);
+  ISDF_HIP_LAUNCHED("k_assign_culled");
   return 0;
 }
 
@@ -738,21 +1085,31 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
   const size_t lds_rk = sizeof(int) * (size_t)rk;
   const int check_every = 4;
   for (int s = 0; s < max_iter; s++) {
-    hipLaunchKernelGGL(k_sort_hist, dim3(b->nchunk_sort), dim3(256), lds_rk, st, n, b->label,
-                       b->hist, b->scal);
-    hipLaunchKernelGGL(k_sort_scan_chunks, dim3(cdiv(rk, 256)), dim3(256), 0, st, b->nchunk_sort,
-                       b->hist, b->cnt_pts, b->scal);
-    hipLaunchKernelGGL(k_sort_scan_clusters, dim3(1), dim3(1024), 0, st, b->cnt_pts, b->base,
-                       b->scal);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(b->nchunk_sort), dim3(64), lds_rk, st, n, b->label,
-                       b->hist, b->base, b->sorted, b->scal);
-    hipLaunchKernelGGL(k_centroid_sums, dim3(cdiv(rk, 4)), dim3(256), 0, st, b->sorted, b->base,
-                       b->cnt_pts, b->WL, b->X, b->Y, b->Z, b->sw, b->sx, b->sy, b->sz, b->scal);
+    if (b->grid_mode) {
+      hipLaunchKernelGGL(k_centroid_sums_box, dim3(rk), dim3(256), 0, st, b->ngrid[0],
+                         b->ngrid[1], b->label, b->WL, b->X, b->Y, b->Z, b->bbox, b->rkmax, b->sw,
+                         b->sx, b->sy, b->sz, b->scal);
+    } else {
+      hipLaunchKernelGGL(k_sort_hist, dim3(b->nchunk_sort), dim3(256), lds_rk, st, n, b->label,
+                         b->hist, b->scal);
+      hipLaunchKernelGGL(k_sort_scan_chunks, dim3(cdiv(rk, 256)), dim3(256), 0, st,
+                         b->nchunk_sort, b->hist, b->cnt_pts, b->scal);
+      hipLaunchKernelGGL(k_sort_scan_clusters, dim3(1), dim3(1024), 0, st, b->cnt_pts, b->base,
+                         b->scal);
+      hipLaunchKernelGGL(k_sort_scatter, dim3(b->nchunk_sort), dim3(64), lds_rk, st, n, b->label,
+                         b->hist, b->base, b->sorted, b->scal);
+      hipLaunchKernelGGL(k_centroid_sums, dim3(cdiv(rk, 4)), dim3(256), 0, st, b->sorted,
+                         b->base, b->cnt_pts, b->WL, b->X, b->Y, b->Z, b->sw, b->sx, b->sy, b->sz,
+                         b->scal);
+    }
     hipLaunchKernelGGL(k_update, dim3(1), dim3(1024), 0, st, b->sw, b->sx, b->sy, b->sz, b->cx,
-                       b->cy, b->cz, b->map, b->scal);
-    hipLaunchKernelGGL(k_assign, dim3(cdiv(n, 256)), dim3(256), 0, st, n, b->X, b->Y, b->Z, b->cx,
-                       b->cy, b->cz, b->label, b->scal, 1);
-    hipLaunchKernelGGL(k_control, dim3(1), dim3(1), 0, st, s, max_iter, maxF, b->trace, b->scal);
+                       b->cy, b->cz, b->map, b->scal, b->grid_mode ? b->bbox : (int*)nullptr,
+                       b->rkmax);
+    hipLaunchKernelGGL(k_assign_culled, dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X,
+                       b->Y, b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 1, b->bbox, b->rkmax,
+                       b->grid_mode ? 1 : 0, b->ngrid[0], b->ngrid[1], b->changed_blk);
+    hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, st, s, max_iter, maxF, b->nblk_pts,
+                       b->changed_blk, b->trace, b->scal);
     ISDF_HIP_LAUNCHED("k-means iteration");
     if ((s + 1) % check_every == 0 && s + 1 < max_iter) {
       int act = 1;
@@ -772,13 +1129,20 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
 int kmeans_finish(KmeansBufs* b, int* piv_dev, hipStream_t st) {
   const int n = b->n;
   const int rkmax = b->rkmax;
-  dim3 g(b->nchunk_final, cdiv(rkmax, 256));
-  hipLaunchKernelGGL(k_final_partial, g, dim3(256), 0, st, n, b->chunk_final, b->X, b->Y, b->Z,
-                     b->cx, b->cy, b->cz, b->pbest, b->pidx, b->scal);
-  ISDF_HIP_LAUNCHED("k_final_partial");
-  hipLaunchKernelGGL(k_final_reduce, dim3(cdiv(rkmax, 256)), dim3(256), 0, st, b->nchunk_final,
-                     b->pbest, b->pidx, b->gidx, b->imin, b->scal);
-  ISDF_HIP_LAUNCHED("k_final_reduce");
+  if (b->grid_mode) {
+    hipLaunchKernelGGL(k_final_window, dim3(cdiv(rkmax, 4)), dim3(256), 0, st, b->ngrid[0],
+                       b->ngrid[1], b->ngrid[2], b->dx[0], b->dx[1], b->dx[2], b->X, b->Y, b->Z,
+                       b->cx, b->cy, b->cz, b->imin, b->scal);
+    ISDF_HIP_LAUNCHED("k_final_window");
+  } else {
+    dim3 g(b->nchunk_final, cdiv(rkmax, 256));
+    hipLaunchKernelGGL(k_final_partial, g, dim3(256), 0, st, n, b->chunk_final, b->X, b->Y, b->Z,
+                       b->cx, b->cy, b->cz, b->pbest, b->pidx, b->scal);
+    ISDF_HIP_LAUNCHED("k_final_partial");
+    hipLaunchKernelGGL(k_final_reduce, dim3(cdiv(rkmax, 256)), dim3(256), 0, st, b->nchunk_final,
+                       b->pbest, b->pidx, b->gidx, b->imin, b->scal);
+    ISDF_HIP_LAUNCHED("k_final_reduce");
+  }
   int P = 1;
   while (P < rkmax) P <<= 1;
   hipLaunchKernelGGL(k_unique, dim3(1), dim3(1024), sizeof(int) * (size_t)P, st, b->imin,

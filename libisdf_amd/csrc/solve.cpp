@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -135,6 +136,12 @@ int validate(const isdf_problem* p, long long* ng_out) {
   return 0;
 }
 
+// Tile variants forced through the environment (tuning runs); -1 = default.
+int env_variant(const char* name) {
+  const char* v = std::getenv(name);
+  return v ? std::atoi(v) : -1;
+}
+
 int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, int* h_piv,
                  hipStream_t st, isdf_report* rep, const isdf_debug* dbg) {
   long long ng64 = 0;
@@ -148,12 +155,12 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   const int s2 = p->nocc - p->nv2, K2 = p->nv2 + p->nc2;
   const bool dual = (p->nv1 != p->nv2 || p->nc1 != p->nc2);
   const int max_iter = p->kmeans_max_iter;
-  const int nJ = (rk + kNB - 1) / kNB;
+  const int nJ = (rk + kNB - 1) / kNB + 1;  // 64-blocks, plus one so 128-blocks pair up
 
   // Workspace layout: K-means buffers | piv | psimuT | P | Dinv | Ft | Gt | scalars.
   size_t off = 0;
   const size_t off_km = off;
-  off += align256(kmeans_bytes(ng, rk, max_iter));
+  off += align256(kmeans_bytes(ng, p->ngrid, rk, max_iter));
   const size_t off_piv = off;
   off += align256(sizeof(int) * (size_t)ng);
   const size_t off_psimu = off;
@@ -162,6 +169,8 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   off += align256(sizeof(double) * (size_t)rk * rk);
   const size_t off_Dinv = off;
   off += align256(sizeof(double) * (size_t)nJ * kNB * kNB);
+  const size_t off_D128 = off;
+  off += align256(sizeof(double) * (size_t)((rk + 127) / 128) * 128 * 128);
   const size_t off_Ft = off;
   off += align256(sizeof(double) * (size_t)rk * rk);
   const size_t off_Gt = off;
@@ -172,11 +181,12 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   if (int rc = workspace(off, &wsp)) return rc;
   char* base = (char*)wsp;
   KmeansBufs kb;
-  kmeans_carve(&kb, base + off_km, ng, rk, max_iter);
+  kmeans_carve(&kb, base + off_km, ng, p->ngrid, rk, max_iter);
   int* piv_dev = (int*)(base + off_piv);
   double* psimuT = (double*)(base + off_psimu);
   double* P = (double*)(base + off_P);
   double* Dinv = (double*)(base + off_Dinv);
+  double* Dinv128 = (double*)(base + off_D128);
   double* Ft = (double*)(base + off_Ft);
   double* Gt = (double*)(base + off_Gt);
   int* info_dev = (int*)(base + off_sc);
@@ -251,7 +261,8 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   if (int rc = launch_gather_psimu(d_psi, ld, nstate, piv_dev, rk, psimuT, st)) return rc;
   if (int rc = ev.rec(4, st)) return rc;
   // 4. Z C^T into d_theta (getbasis_scalapack.cpp:297-369).
-  if (int rc = launch_zct(d_psi, ld, ng, psimuT, rk, s1, K1, s2, K2, dual, d_theta, ng, st))
+  if (int rc = launch_zct(d_psi, ld, ng, psimuT, rk, s1, K1, s2, K2, dual, d_theta, ng,
+                          env_variant("ISDF_ZCT_TILE"), st))
     return rc;
   if (int rc = ev.rec(5, st)) return rc;
   // 5. C C^T + eps (getbasis_scalapack.cpp:390-428).
@@ -265,9 +276,9 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   if (int rc = launch_potrf(P, rk, Dinv, info_dev, st)) return rc;
   if (int rc = ev.rec(7, st)) return rc;
   // 7. Two right-side triangular solves in place (getbasis_scalapack.cpp:469-478).
-  if (int rc = launch_make_fg(P, rk, Dinv, Ft, Gt, st)) return rc;
+  if (int rc = launch_make_fg(P, rk, Dinv, Dinv128, Ft, Gt, st)) return rc;
   if (int rc = ev.rec(8, st)) return rc;
-  if (int rc = launch_potrs(d_theta, ng64, ng64, rk, Ft, Gt, st)) return rc;
+  if (int rc = launch_potrs(d_theta, ng64, ng64, rk, Ft, Gt, env_variant("ISDF_POTRS_TILE"), st)) return rc;
   if (int rc = ev.rec(9, st)) return rc;
 
   int info = 0;

@@ -131,6 +131,25 @@ def test_small_synthetic_parity(case):
     compare(g, o, cct_tol=1e-12)
 
 
+def test_partial_density_sorted_path_and_empty_clusters():
+    # psi vanishes on the two lowest z planes (grid point 0 included), so npt <
+    # Ng: the K-means points are the threshold list, the tail reads grid point
+    # 0 with zero weight (utility.cpp:708), zero-mass clusters are dropped
+    # (:834-845), and the centroid sums take the sorted path.
+    grid, L = (12, 10, 14), (5.0, 4.0, 6.0)
+    ng = grid[0] * grid[1] * grid[2]
+    rng = np.random.default_rng(21)
+    psi = rng.standard_normal((16, ng))
+    psi = synth.cholesky_qr2(psi, L[0] * L[1] * L[2] / ng)
+    psi[:, : 2 * grid[0] * grid[1]] = 0.0
+    args = (L, grid, 8, 16, 8, 8, 8, 8, 120, 40)
+    g = gpu_solve(psi, *args)
+    o = oracle_solve(psi, *args)
+    assert o["km"]["npt"] == ng - 2 * grid[0] * grid[1]
+    assert g["rep"]["npt"] == o["km"]["npt"]
+    compare(g, o)
+
+
 def test_kmeans_maxiter_zero():
     w = synth.WORKLOADS["cfg1"]
     psi = synth.wavefunctions(w)
