@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ psi, 
 // product; 256 x 64 with two accumulators when the pair sets differ.
 typedef MfmaTile<256, 128, 16, 4, 2> ZctTile0;      // 8 waves of 64 x 64
 typedef MfmaTile<256, 128, 16, 4, 1> ZctTile1;      // 4 waves of 64 x 128
-typedef MfmaTile<128, 128, 16, 2, 2> ZctTile2;      // 4 waves of 64 x 64
+typedef ValuTile<256, 128, 16, 1, 8> ZctTile2;      // vector FMA, 8 waves of 256 x 16
 typedef MfmaTile<256, 64, 16, 4, 2> ZctTileDual0;   // two accumulators
 typedef MfmaTile<128, 64, 16, 2, 2> ZctTileDual1;
 
@@ -85,24 +85,14 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct(int ng, int rk, const doubl
     Tile::zero(acc2);
     Tile::run(A2 + m0, lda, B2 + n0, ldb, K2, mlim, nlim, lds, acc2);
   }
-#pragma unroll
-  for (int j = 0; j < Tile::TN; j++) {
-    const int n = Tile::col(j);
-    if (n >= nlim) continue;
-    double* Cc = C + (long long)(n0 + n) * ldc + m0;
-#pragma unroll
-    for (int i = 0; i < Tile::TM; i++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int m = Tile::row(i, r);
-        if (m < mlim) {
-          const double v = acc1[i][j][r];
-          double o;
-          if constexpr (DUAL) o = v * acc2[i][j][r];
-          else o = v * v;
-          Cc[m] = o;
-        }
-      }
+  if constexpr (DUAL) {
+    Tile::epilogue2(acc1, acc2, [&](int m, int n, double v1, double v2) {
+      if (m < mlim && n < nlim) C[(long long)(n0 + n) * ldc + m0 + m] = v1 * v2;
+    });
+  } else {
+    Tile::epilogue(acc1, [&](int m, int n, double v) {
+      if (m < mlim && n < nlim) C[(long long)(n0 + n) * ldc + m0 + m] = v * v;
+    });
   }
 }
 
@@ -426,19 +416,9 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_potrs(double* __restrict__ T, l
       Tile::run(Tm + (long long)kbeg * ldt, ldt, Bop + (long long)kbeg * n + J0, n, kend - kbeg,
                 mlim, jb, lds, acc);
       // Tile::run ends with a barrier: every read of block column J is done.
-#pragma unroll
-      for (int j = 0; j < Tile::TN; j++) {
-        const int c = Tile::col(j);
-        if (c >= jb) continue;
-        double* Tc = Tm + (long long)(J0 + c) * ldt;
-#pragma unroll
-        for (int i = 0; i < Tile::TM; i++)
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int m = Tile::row(i, r);
-            if (m < mlim) Tc[m] = acc[i][j][r];
-          }
-      }
+      Tile::epilogue(acc, [&](int m, int c, double v) {
+        if (m < mlim && c < jb) Tm[(long long)(J0 + c) * ldt + m] = v;
+      });
       // The next block column reads what this one wrote.
       __syncthreads();
     }
@@ -449,7 +429,7 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_potrs(double* __restrict__ T, l
 typedef MfmaTile<256, kSB, 16, 4, 2> PotrsTile0;  // 8 waves of 64 x 64
 typedef MfmaTile<256, kSB, 16, 4, 1> PotrsTile1;  // 4 waves of 64 x 128
 typedef MfmaTile<128, kSB, 16, 2, 2> PotrsTile2;  // 4 waves of 64 x 64
-typedef MfmaTile<128, kSB, 32, 2, 2> PotrsTile3;  // 4 waves of 64 x 64, 32-deep chunks
+typedef ValuTile<256, kSB, 16, 1, 8> PotrsTile3;  // vector FMA, 8 waves of 256 x 16
 
 }  // namespace
 

@@ -136,6 +136,27 @@ struct MfmaTile {
     }
   }
 
+  // Calls f(m, n, value) for every accumulator element of this lane (m, n
+  // relative to the workgroup tile origin).
+  template <class F>
+  __device__ static void epilogue(const Acc& acc, F f) {
+#pragma unroll
+    for (int j = 0; j < TN; j++)
+#pragma unroll
+      for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) f(row(i, r), col(j), acc[i][j][r]);
+  }
+  template <class F>
+  __device__ static void epilogue2(const Acc& a1, const Acc& a2, F f) {
+#pragma unroll
+    for (int j = 0; j < TN; j++)
+#pragma unroll
+      for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) f(row(i, r), col(j), a1[i][j][r], a2[i][j][r]);
+  }
+
   // Row / column of accumulator element (i, j, r) for this lane, relative to
   // the workgroup tile origin.
   __device__ static int row(int i, int r) {
@@ -145,6 +166,126 @@ struct MfmaTile {
   __device__ static int col(int j) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     return (wave / WGM) * (TN * 16) + 16 * j + (lane & 15);
+  }
+};
+
+// The same staged GEMM on the vector FMA pipe (v_fma_f64), which sustains a
+// higher FP64 rate than v_mfma_f64_16x16x4f64 on gfx950.  Lane l of wave
+// (wm, wn) owns rows wm*64*TM + 64*i + l (i < TM) and the wave-uniform columns
+// wn*TN + j (j < TN): A values come from LDS per lane, B values as LDS
+// broadcasts, so each k step is TM*TN FMAs for TM + TN LDS reads.
+template <int BM_, int BN_, int KB_, int WGM_, int WGN_>
+struct ValuTile {
+  static constexpr int BM = BM_, BN = BN_, KB = KB_, WGM = WGM_, WGN = WGN_;
+  static constexpr int NT = 64 * WGM * WGN;
+  static constexpr int TM = BM / WGM / 64;
+  static constexpr int TN = BN / WGN;
+  static constexpr int APT = BM * KB / NT;
+  static constexpr int BPT = BN * KB / NT;
+  static constexpr int LDS_DOUBLES = 2 * KB * (BM + BN);
+  static_assert(TM >= 1 && TN >= 1 && BM % (64 * WGM) == 0 && BN % WGN == 0, "tile shape");
+  static_assert(BM * KB % NT == 0 && BN * KB % NT == 0, "chunk must split evenly");
+  typedef MfmaTile<BM, BN, KB, WGM, 1> Stager;  // same global->LDS staging (B k-major)
+  typedef double Acc[TM][TN];
+
+  __device__ static void zero(Acc& acc) {
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+      for (int j = 0; j < TN; j++) acc[i][j] = 0.0;
+  }
+
+  __device__ static void load(const double* __restrict__ A, long long lda,
+                              const double* __restrict__ B, long long ldb, int K, int k0,
+                              int mlim, int nlim, double (&ra)[APT], double (&rb)[BPT]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < APT; q++) {
+      const int e = t + q * NT;
+      const int kk = e / BM, m = e % BM;
+      const int k = k0 + kk;
+      ra[q] = (k < K && m < mlim) ? A[(long long)k * lda + m] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < BPT; q++) {
+      const int e = t + q * NT;
+      const int kk = e / BN, n = e % BN;
+      const int k = k0 + kk;
+      rb[q] = (k < K && n < nlim) ? B[(long long)k * ldb + n] : 0.0;
+    }
+  }
+
+  __device__ static void store(double* lds, int buf, const double (&ra)[APT],
+                               const double (&rb)[BPT]) {
+    double* As = lds + buf * KB * (BM + BN);
+    double* Bs = As + KB * BM;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < APT; q++) As[t + q * NT] = ra[q];
+#pragma unroll
+    for (int q = 0; q < BPT; q++) Bs[t + q * NT] = rb[q];
+  }
+
+  __device__ static void compute(const double* lds, int buf, Acc& acc) {
+    const double* As = lds + buf * KB * (BM + BN);
+    const double* Bs = As + KB * BM;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave % WGM, wn = wave / WGM;
+    const double* Aw = As + wm * (TM * 64) + lane;
+    const double* Bw = Bs + wn * TN;
+#pragma unroll 2
+    for (int kk = 0; kk < KB; kk++) {
+      double a[TM];
+#pragma unroll
+      for (int i = 0; i < TM; i++) a[i] = Aw[kk * BM + 64 * i];
+#pragma unroll
+      for (int j = 0; j < TN; j++) {
+        const double b = Bw[kk * BN + j];
+#pragma unroll
+        for (int i = 0; i < TM; i++) acc[i][j] = __builtin_fma(a[i], b, acc[i][j]);
+      }
+    }
+  }
+
+  __device__ static void run(const double* __restrict__ A, long long lda,
+                             const double* __restrict__ B, long long ldb, int K, int mlim,
+                             int nlim, double* lds, Acc& acc) {
+    double ra[APT], rb[BPT];
+    const int nch = (K + KB - 1) / KB;
+    if (nch == 0) return;
+    load(A, lda, B, ldb, K, 0, mlim, nlim, ra, rb);
+    store(lds, 0, ra, rb);
+    __syncthreads();
+    for (int c = 0; c < nch; c++) {
+      const bool more = c + 1 < nch;
+      if (more) load(A, lda, B, ldb, K, (c + 1) * KB, mlim, nlim, ra, rb);
+      compute(lds, c & 1, acc);
+      if (more) store(lds, (c + 1) & 1, ra, rb);
+      __syncthreads();
+    }
+  }
+
+  __device__ static int row(int i) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    return (wave % WGM) * (TM * 64) + 64 * i + lane;
+  }
+  __device__ static int col(int j) {
+    const int wave = threadIdx.x >> 6;
+    return (wave / WGM) * TN + j;
+  }
+  template <class F>
+  __device__ static void epilogue(const Acc& acc, F f) {
+#pragma unroll
+    for (int j = 0; j < TN; j++)
+#pragma unroll
+      for (int i = 0; i < TM; i++) f(row(i), col(j), acc[i][j]);
+  }
+  template <class F>
+  __device__ static void epilogue2(const Acc& a1, const Acc& a2, F f) {
+#pragma unroll
+    for (int j = 0; j < TN; j++)
+#pragma unroll
+      for (int i = 0; i < TM; i++) f(row(i), col(j), a1[i][j], a2[i][j]);
   }
 };
 
