@@ -30,6 +30,14 @@ METRIC = "ISDF solves/s and ZC^T TFLOP/s, mixed vs FP64 plain at fixed fit error
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per k_potrs launch from two rocprofv3 counter passes (FETCH_SIZE x 2.000,
+# WRITE_SIZE x 1.000, factors from a copy kernel of known size), per workload.
+TRAFFIC = {
+    "cfg2": {"bytes": 2.487e10,
+             "source": "profiles/k_potrs_k_zct_cfg2_fp64_hbm_bytes.md (20.49 GB read + "
+                       "4.383 GB written per launch)"},
+}
+
 
 def flops(w, rk):
     k1 = w.nv1 + w.nc1
@@ -92,7 +100,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", dest="mixed", action="store_false",
+                    help="skip the mixed-precision solves (FP32 and 3-way-split BF16)")
+    ap.add_argument("--fit-rows", type=int, default=2048,
+                    help="grid rows sampled for the fit-error comparison")
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                    help="run only a plain device copy of GIB GiB five times (the known-size "
+                         "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
+    if a.copy_probe > 0:
+        import torch
+        n = int(a.copy_probe * (1 << 30)) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.add(src, 0.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n,
+                          "launches": 5}))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs the ISDF path as a single rank with no "
               "collectives (every GPU box here has one GPU); --gpus %d is not supported." % a.gpus,
@@ -108,31 +134,35 @@ def main():
     rk = isdf.isdf_rank(w.nv1, w.nc1, w.nv2, w.nc2, w.mu_points)
     dev = torch.device("cuda:0")
     d_psi = torch.from_numpy(psi).to(dev)
-    d_theta = torch.empty((rk, w.ng), dtype=torch.float64, device=dev)
-    piv = np.zeros(w.ng, np.int32)
     dom = isdf.Domain(w.length, w.ngrid)
     stream = torch.cuda.current_stream()
-
-    def step():
-        return isdf.solve(d_psi, dom, w.nocc, w.nstate, w.nv1, w.nc1, w.nv2, w.nc2, w.mu_points,
-                          d_theta, piv, w.kmeans_iters, stream=stream)
-
-    for _ in range(a.warmup):
-        step()
-    torch.cuda.synchronize()
-    reps = []
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        reps.append(step())
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_step = dt * 1e3 / a.steps
-
     fl = flops(w, rk)
-    avg = {k: float(np.mean([r[k] for r in reps])) for k in reps[0] if k.startswith("ms_")}
+
+    def timed(precision, d_theta, piv, steps, warmup):
+        def step():
+            return isdf.solve(d_psi, dom, w.nocc, w.nstate, w.nv1, w.nc1, w.nv2, w.nc2,
+                              w.mu_points, d_theta, piv, w.kmeans_iters, stream=stream,
+                              precision=precision)
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        reps = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            reps.append(step())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        avg = {k: float(np.mean([r[k] for r in reps])) for k in reps[0] if k.startswith("ms_")}
+        return dt, avg, reps[-1]
+
+    # The drop-in FP64 path: the timed value.
+    d_theta = torch.empty((rk, w.ng), dtype=torch.float64, device=dev)
+    piv = np.zeros(w.ng, np.int32)
+    dt, avg, r0 = timed(isdf.ISDF_PREC_FP64, d_theta, piv, a.steps, a.warmup)
+    ms_step = dt * 1e3 / a.steps
     potrs_tf = fl["potrs"] / (avg["ms_potrs"] * 1e-3) / 1e12
     zct_tf = fl["zct"] / (avg["ms_zct"] * 1e-3) / 1e12
-    r0 = reps[-1]
+    traffic = TRAFFIC.get(a.workload)
     out = {
         "metric": METRIC,
         "value": a.steps / dt,
@@ -165,14 +195,59 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": potrs_tf / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic["bytes"] if traffic else None,
+            "traffic_source": traffic["source"] if traffic else None,
             "algorithmic_flops_per_launch": fl["potrs"],
             "avg_launch_ms": avg["ms_potrs"],
         },
     }
+
+    # Mixed forms on the same input, against the FP64 result of the timed run.
+    if a.mixed:
+        rows = torch.from_numpy(np.sort(np.random.default_rng(0).choice(
+            w.ng, min(w.ng, a.fit_rows), replace=False))).to(dev)
+        fit64 = sampled_fit_error(torch, d_psi, d_theta, piv, w, rk, rows)
+        theta_norm = torch.linalg.norm(d_theta)
+        out["fit_error_fp64_sampled"] = fit64
+        mixed = []
+        d_mix = torch.empty_like(d_theta)
+        piv_m = np.zeros(w.ng, np.int32)
+        for name, prec in (("bf16x3", isdf.ISDF_PREC_BF16X3), ("fp32", isdf.ISDF_PREC_FP32)):
+            dtm, avgm, _ = timed(prec, d_mix, piv_m, max(1, a.steps // 2), 1)
+            err = float(torch.linalg.norm(d_mix - d_theta) / theta_norm)
+            fitm = sampled_fit_error(torch, d_psi, d_mix, piv_m, w, rk, rows)
+            steps_m = max(1, a.steps // 2)
+            mixed.append({
+                "precision": name,
+                "solves_per_s": steps_m / dtm,
+                "speedup_vs_fp64": (steps_m / dtm) / (a.steps / dt),
+                "theta_rel_err_vs_fp64": err,
+                "fit_error_sampled": fitm,
+                "fit_error_ratio_vs_fp64": fitm / fit64,
+                "piv_equal_fp64": bool(np.array_equal(piv_m, piv)),
+                "zct_tflops": fl["zct"] / (avgm["ms_zct"] * 1e-3) / 1e12,
+                "theta_tflops": fl["potrs"] / (avgm["ms_potrs"] * 1e-3) / 1e12,
+                "stage_ms": {k[3:]: round(v, 4) for k, v in avgm.items()},
+            })
+        del d_mix
+        out["mixed"] = mixed
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(psi, w, r0["kmeans_iters"])
     print(json.dumps(out))
+
+
+def sampled_fit_error(torch, d_psi, d_theta, piv, w, rk, rows):
+    """||Z - Theta C||_F / ||Z||_F over a fixed sample of grid rows (rows are
+    independent in both norms), Z(r, (i,j)) = psi_i(r) psi_j(r), C = the same
+    pair products at the pivots (src/common/checkcode.cpp:62-166)."""
+    s1, k1 = w.nocc - w.nv1, w.nv1 + w.nc1
+    s2, k2 = w.nocc - w.nv2, w.nv2 + w.nc2
+    pv = torch.from_numpy(piv[:rk].astype(np.int64)).to(d_psi.device)
+    a1, a2 = d_psi[s1:s1 + k1], d_psi[s2:s2 + k2]
+    zr = (a1[:, rows].T[:, :, None] * a2[:, rows].T[:, None, :]).reshape(len(rows), -1)
+    cm = (a1[:, pv].T[:, :, None] * a2[:, pv].T[:, None, :]).reshape(rk, -1)
+    th = d_theta[:, rows].T.contiguous()
+    return float(torch.linalg.norm(zr - th @ cm) / torch.linalg.norm(zr))
 
 
 if __name__ == "__main__":

@@ -44,9 +44,21 @@ enum {
 };
 
 /* Arithmetic of the Z C^T / C C^T contractions and of the Theta solve.
- * FP64 is the drop-in path. */
+ * FP64 is the drop-in path and follows the reference.  The mixed forms have
+ * no counterpart in the reference: K-means, the Cholesky of C C^T + eps I and
+ * its inverse W stay FP64; Z C^T and Theta = Z C^T W run on low-precision MFMA
+ * with FP32 accumulation.
+ *   ISDF_PREC_FP32    FP32 MFMA (v_mfma_f32_16x16x4f32)
+ *   ISDF_PREC_BF16X3  every FP32 operand split exactly into three bf16 parts,
+ *                     six partial products on v_mfma_f32_16x16x32_bf16
+ * OR ISDF_PREC_REFINE into a mixed mode for one FP64 residual-correction step
+ * Theta += (Z C^T - Theta (C C^T + eps I)) W; its residual is an FP64 GEMM of
+ * the size of the FP64 solve (2 Ng rk^2 flops). */
 enum {
-  ISDF_PREC_FP64 = 0
+  ISDF_PREC_FP64 = 0,
+  ISDF_PREC_FP32 = 1,
+  ISDF_PREC_BF16X3 = 2,
+  ISDF_PREC_REFINE = 16
 };
 
 typedef struct isdf_problem {
@@ -70,8 +82,9 @@ typedef struct isdf_report {
   int reserved;
   double eps;        /* 1e-10 * trace(C C^T) / rk added to the diagonal */
   /* Stage times in milliseconds (HIP events on the call's stream).  ms_potrf
-   * includes the preparation of the blocked solve operands; ms_potrs is the
-   * in-place triangular-solve kernel alone. */
+   * includes the preparation of the blocked solve operands (mixed forms: and
+   * the FP64 inverse W); ms_potrs is the in-place triangular-solve kernel alone
+   * (mixed forms: the Theta = Z C^T W product and the optional correction). */
   double ms_weights, ms_kmeans, ms_select, ms_gather, ms_zct, ms_cct, ms_potrf, ms_potrs,
       ms_total;
 } isdf_report;

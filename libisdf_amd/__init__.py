@@ -16,9 +16,10 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ISDF_EHIP, ISDF_EINVAL, ISDF_ENOMEM, ISDF_ENOTSPD, ISDF_OK,
-                   ISDF_PREC_FP64)
+                   ISDF_PREC_BF16X3, ISDF_PREC_FP32, ISDF_PREC_FP64, ISDF_PREC_REFINE)
 
-__all__ = ["Domain", "ISDF", "isdf_rank", "solve", "check", "ISDF_PREC_FP64"]
+__all__ = ["Domain", "ISDF", "isdf_rank", "solve", "check", "ISDF_PREC_FP64", "ISDF_PREC_FP32",
+           "ISDF_PREC_BF16X3", "ISDF_PREC_REFINE"]
 
 
 class Domain:

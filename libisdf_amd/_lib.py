@@ -10,7 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libisdf_hip.so")
 
 ISDF_OK, ISDF_EINVAL, ISDF_ENOTSPD, ISDF_EHIP, ISDF_ENOMEM = 0, 1, 2, 3, 4
-ISDF_PREC_FP64 = 0
+ISDF_PREC_FP64, ISDF_PREC_FP32, ISDF_PREC_BF16X3, ISDF_PREC_REFINE = 0, 1, 2, 16
 
 # Every symbol the public header declares; tests check the library exports them.
 EXPORTS = (

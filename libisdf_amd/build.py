@@ -33,6 +33,7 @@ COMMON = [
 SOURCES = {
     "kmeans.hip": ["-ffp-contract=off"],
     "basis.hip": [],
+    "lowprec.hip": [],
     "solve.cpp": ["-x", "hip"],
     "check.cpp": [],
     "dropin.cpp": ["-x", "hip"],

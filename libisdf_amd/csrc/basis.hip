@@ -430,6 +430,7 @@ typedef MfmaTile<256, kSB, 16, 4, 2> PotrsTile0;  // 8 waves of 64 x 64
 typedef MfmaTile<256, kSB, 16, 4, 1> PotrsTile1;  // 4 waves of 64 x 128
 typedef MfmaTile<128, kSB, 16, 2, 2> PotrsTile2;  // 4 waves of 64 x 64
 typedef ValuTile<256, kSB, 16, 1, 8> PotrsTile3;  // vector FMA, 8 waves of 256 x 16
+typedef MfmaTile<32, kSB, 16, 2, 2> PotrsTile4;   // 4 waves of 16 x 64: few rows, many workgroups
 
 }  // namespace
 
@@ -472,6 +473,10 @@ int launch_cct(const double* C, long long ldc, const int* piv, int rk, double* P
   hipLaunchKernelGGL(k_cct_gather, dim3(cdiv((long long)rk * rk, 256)), dim3(256), 0, st, C, ldc,
                      piv, rk, P);
   ISDF_HIP_LAUNCHED("k_cct_gather");
+  return launch_trace_eps(P, rk, eps_dev, st);
+}
+
+int launch_trace_eps(double* P, int rk, double* eps_dev, hipStream_t st) {
   hipLaunchKernelGGL(k_trace_eps, dim3(1), dim3(256), 0, st, P, rk, eps_dev);
   ISDF_HIP_LAUNCHED("k_trace_eps");
   return 0;
@@ -507,7 +512,7 @@ int launch_make_fg(const double* L, int rk, const double* Dinv, double* Dinv128,
   return 0;
 }
 
-int potrs_variants() { return 4; }
+int potrs_variants() { return 5; }
 
 int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* Ft, const double* Gt,
                  int variant, hipStream_t st) {
@@ -522,6 +527,7 @@ int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* F
     ISDF_POTRS_CASE(1)
     ISDF_POTRS_CASE(2)
     ISDF_POTRS_CASE(3)
+    ISDF_POTRS_CASE(4)
 #undef ISDF_POTRS_CASE
     default:
       return set_error(1, "unknown solve tile variant");

@@ -115,6 +115,40 @@ int potrs_variants();
 int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* Ft,
                  const double* Gt, int variant, hipStream_t st);
 
+// trace of P summed in ascending order and eps = 1e-10 trace / rk added to the
+// diagonal (one block).
+int launch_trace_eps(double* P, int rk, double* eps_dev, hipStream_t st);
+
 constexpr int kNB = 64;  // block size of the Cholesky (the solves use 128-blocks)
+
+// ---------------------------------------------------------------------------
+// Mixed-precision contractions (lowprec.hip).  mode 1 = FP32 MFMA, mode 2 =
+// 3-way-split BF16 MFMA; FP32 accumulation in both.  Operands are FP32 rows
+// with k contiguous, k padded to a multiple of 32 with zeros.
+// ---------------------------------------------------------------------------
+int lp_variants(int mode);
+// out[g*kp + k] = psi(g, c0 + k) as FP32 (zero for K <= k < kp).
+int launch_pack_psi(const double* psi, long long ld, int ng, int c0, int K, int kp, float* out,
+                    hipStream_t st);
+// out[nu*kp + k] = psi(piv[nu], c0 + k) as FP32 (zero for K <= k < kp).
+int launch_gather_rows(const double* psi, long long ld, const int* piv, int rk, int c0, int K,
+                       int kp, float* out, hipStream_t st);
+// Zt[g*ldz + nu] = P1(g,nu) P2(g,nu), zero for rk <= nu < ldz.
+int launch_zct_lp(int mode, int ng, int rk, int ldz, const float* A1, const float* A2, int kp1,
+                  int kp2, const float* B1, const float* B2, bool dual, float* Zt, int variant,
+                  hipStream_t st);
+// P(mu, nu) = Zt[piv[mu]][nu] (column-major FP64) and the eps shift.
+int launch_cct_lp(const float* Zt, int ldz, const int* piv, int rk, double* P, double* eps_dev,
+                  hipStream_t st);
+// W = (L L^T)^{-1} (FP64, column-major rk x rk) from the solve operands, and
+// Wf[nu*ldw + k] = W(k, nu) as FP32 (zero for k >= rk).
+int launch_inverse(int rk, const double* Ft, const double* Gt, double* W, int ldw, float* Wf,
+                   hipStream_t st);
+// theta(g, nu) = (or +=) sum_k Wf[nu][k] Zt[g][k], k < ldk.
+int launch_theta_lp(int mode, int ng, int rk, int ldk, const float* Wf, const float* Zt,
+                    double* theta, long long ldt, bool add, int variant, hipStream_t st);
+// Zt[g][nu] <- Zt[g][nu] - (theta Pe)(g, nu) in FP64 (the correction step's residual).
+int launch_resid(int ng, int rk, const double* theta, long long ldt, const double* Pe, float* Zt,
+                 int ldz, hipStream_t st);
 
 }  // namespace isdf_hip

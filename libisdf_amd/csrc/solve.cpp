@@ -127,7 +127,13 @@ int validate(const isdf_problem* p, long long* ng_out) {
                      "pair ranges [nocc-nv, nocc+nc) must be non-empty and inside [0, nstate)");
   if (p->kmeans_max_iter < 0) return set_error(ISDF_EINVAL, "kmeans_max_iter must be >= 0");
   if (p->ld_psi != 0 && p->ld_psi < ng) return set_error(ISDF_EINVAL, "ld_psi < Ng");
-  if (p->precision != ISDF_PREC_FP64) return set_error(ISDF_EINVAL, "unknown precision mode");
+  {
+    const int mode = p->precision & ~ISDF_PREC_REFINE;
+    if (mode != ISDF_PREC_FP64 && mode != ISDF_PREC_FP32 && mode != ISDF_PREC_BF16X3)
+      return set_error(ISDF_EINVAL, "unknown precision mode");
+    if (mode == ISDF_PREC_FP64 && (p->precision & ISDF_PREC_REFINE))
+      return set_error(ISDF_EINVAL, "ISDF_PREC_REFINE applies to the mixed forms only");
+  }
   int rk = irank(p);
   if (rk <= 0) return set_error(ISDF_EINVAL, "rk must be positive");
   if (rk > ng) return set_error(ISDF_EINVAL, "mu_points exceeds the number of grid points");
@@ -140,6 +146,63 @@ int validate(const isdf_problem* p, long long* ng_out) {
 int env_variant(const char* name) {
   const char* v = std::getenv(name);
   return v ? std::atoi(v) : -1;
+}
+
+// Basis construction of the mixed forms (records events 4..9 like the FP64
+// path): packed operands -> low-precision Z C^T (Zt, g-major FP32) -> C C^T +
+// eps from the pivot rows of Zt -> FP64 Cholesky -> FP64 inverse W ->
+// low-precision Theta = Zt W into d_theta [-> FP64 residual, Theta += R W].
+int basis_mixed(const isdf_problem* p, const double* d_psi, long long ld, int ng, int rk, int mode,
+                bool refine, char* base, size_t off_A1, size_t off_A2, size_t off_B1,
+                size_t off_B2, size_t off_Zt, size_t off_W, size_t off_Wf, size_t off_Pe, int kp1,
+                int kp2, int ldz, const int* piv_dev, double* P, double* Dinv, double* Dinv128,
+                double* Ft, double* Gt, int* info_dev, double* eps_dev, double* d_theta,
+                const isdf_debug* dbg, Events& ev, hipStream_t st) {
+  const int s1 = p->nocc - p->nv1, K1 = p->nv1 + p->nc1;
+  const int s2 = p->nocc - p->nv2, K2 = p->nv2 + p->nc2;
+  const bool dual = (p->nv1 != p->nv2 || p->nc1 != p->nc2);
+  float* A1 = (float*)(base + off_A1);
+  float* A2 = dual ? (float*)(base + off_A2) : A1;
+  float* B1 = (float*)(base + off_B1);
+  float* B2 = dual ? (float*)(base + off_B2) : B1;
+  float* Zt = (float*)(base + off_Zt);
+  double* W = (double*)(base + off_W);
+  float* Wf = (float*)(base + off_Wf);
+  double* Pe = refine ? (double*)(base + off_Pe) : nullptr;
+  // Operands: psi rows of the pair sets and their values at the pivots.
+  if (int rc = launch_pack_psi(d_psi, ld, ng, s1, K1, kp1, A1, st)) return rc;
+  if (int rc = launch_gather_rows(d_psi, ld, piv_dev, rk, s1, K1, kp1, B1, st)) return rc;
+  if (dual) {
+    if (int rc = launch_pack_psi(d_psi, ld, ng, s2, K2, kp2, A2, st)) return rc;
+    if (int rc = launch_gather_rows(d_psi, ld, piv_dev, rk, s2, K2, kp2, B2, st)) return rc;
+  }
+  if (int rc = ev.rec(4, st)) return rc;
+  if (int rc = launch_zct_lp(mode, ng, rk, ldz, A1, A2, kp1, dual ? kp2 : kp1, B1, B2, dual, Zt,
+                             env_variant("ISDF_LP_ZCT_TILE"), st))
+    return rc;
+  if (int rc = ev.rec(5, st)) return rc;
+  if (int rc = launch_cct_lp(Zt, ldz, piv_dev, rk, P, eps_dev, st)) return rc;
+  if (dbg && dbg->d_cct)
+    ISDF_HIP_TRY(hipMemcpyAsync(dbg->d_cct, P, sizeof(double) * (size_t)rk * rk,
+                                hipMemcpyDeviceToDevice, st));
+  if (refine)
+    ISDF_HIP_TRY(hipMemcpyAsync(Pe, P, sizeof(double) * (size_t)rk * rk, hipMemcpyDeviceToDevice,
+                                st));
+  if (int rc = ev.rec(6, st)) return rc;
+  ISDF_HIP_TRY(hipMemsetAsync(info_dev, 0, sizeof(int), st));
+  if (int rc = launch_potrf(P, rk, Dinv, info_dev, st)) return rc;
+  if (int rc = ev.rec(7, st)) return rc;
+  if (int rc = launch_make_fg(P, rk, Dinv, Dinv128, Ft, Gt, st)) return rc;
+  if (int rc = launch_inverse(rk, Ft, Gt, W, ldz, Wf, st)) return rc;
+  if (int rc = ev.rec(8, st)) return rc;
+  const int tv = env_variant("ISDF_LP_THETA_TILE");
+  if (int rc = launch_theta_lp(mode, ng, rk, ldz, Wf, Zt, d_theta, ng, false, tv, st)) return rc;
+  if (refine) {
+    if (int rc = launch_resid(ng, rk, d_theta, ng, Pe, Zt, ldz, st)) return rc;
+    if (int rc = launch_theta_lp(mode, ng, rk, ldz, Wf, Zt, d_theta, ng, true, tv, st)) return rc;
+  }
+  if (int rc = ev.rec(9, st)) return rc;
+  return 0;
 }
 
 int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, int* h_piv,
@@ -156,6 +219,12 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   const bool dual = (p->nv1 != p->nv2 || p->nc1 != p->nc2);
   const int max_iter = p->kmeans_max_iter;
   const int nJ = (rk + kNB - 1) / kNB + 1;  // 64-blocks, plus one so 128-blocks pair up
+  const int mode = p->precision & ~ISDF_PREC_REFINE;
+  const bool mixed = mode != ISDF_PREC_FP64;
+  const bool refine = (p->precision & ISDF_PREC_REFINE) != 0;
+  // Mixed forms: FP32 operand rows padded to multiples of 32 in k.
+  auto pad32 = [](int v) { return (v + 31) / 32 * 32; };
+  const int kp1 = pad32(K1), kp2 = dual ? pad32(K2) : 0, ldz = pad32(rk);
 
   // Workspace layout: K-means buffers | piv | psimuT | P | Dinv | Ft | Gt | scalars.
   size_t off = 0;
@@ -177,6 +246,30 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   off += align256(sizeof(double) * (size_t)rk * rk);
   const size_t off_sc = off;
   off += 256;
+  // Mixed forms: packed psi rows, gathered Psi(mu) rows, Zt, W (FP64), Wf, and
+  // the copy of C C^T + eps I the correction step multiplies by.
+  size_t off_A1 = 0, off_A2 = 0, off_B1 = 0, off_B2 = 0, off_Zt = 0, off_W = 0, off_Wf = 0,
+         off_Pe = 0;
+  if (mixed) {
+    off_A1 = off;
+    off += align256(sizeof(float) * (size_t)ng * kp1);
+    off_A2 = off;
+    off += align256(sizeof(float) * (size_t)ng * kp2);
+    off_B1 = off;
+    off += align256(sizeof(float) * (size_t)rk * kp1);
+    off_B2 = off;
+    off += align256(sizeof(float) * (size_t)rk * kp2);
+    off_Zt = off;
+    off += align256(sizeof(float) * (size_t)ng * ldz);
+    off_W = off;
+    off += align256(sizeof(double) * (size_t)rk * rk);
+    off_Wf = off;
+    off += align256(sizeof(float) * (size_t)rk * ldz);
+    if (refine) {
+      off_Pe = off;
+      off += align256(sizeof(double) * (size_t)rk * rk);
+    }
+  }
   void* wsp = nullptr;
   if (int rc = workspace(off, &wsp)) return rc;
   char* base = (char*)wsp;
@@ -257,6 +350,12 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
     ISDF_HIP_TRY(hipMemcpyAsync(dbg->h_trace, kb.trace, sizeof(int) * std::max(iters, 0),
                                 hipMemcpyDeviceToHost, st));
 
+  if (mixed) {
+    if (int rc = basis_mixed(p, d_psi, ld, ng, rk, mode, refine, base, off_A1, off_A2, off_B1,
+                             off_B2, off_Zt, off_W, off_Wf, off_Pe, kp1, kp2, ldz, piv_dev, P,
+                             Dinv, Dinv128, Ft, Gt, info_dev, eps_dev, d_theta, dbg, ev, st))
+      return rc;
+  } else {
   // 3. Psi(mu) gather over the caller's rk (getbasis_scalapack.cpp:249-269).
   if (int rc = launch_gather_psimu(d_psi, ld, nstate, piv_dev, rk, psimuT, st)) return rc;
   if (int rc = ev.rec(4, st)) return rc;
@@ -280,6 +379,7 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   if (int rc = ev.rec(8, st)) return rc;
   if (int rc = launch_potrs(d_theta, ng64, ng64, rk, Ft, Gt, env_variant("ISDF_POTRS_TILE"), st)) return rc;
   if (int rc = ev.rec(9, st)) return rc;
+  }
 
   int info = 0;
   double eps = 0.0;
