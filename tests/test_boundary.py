@@ -61,15 +61,18 @@ def test_rank_rule_matches_oracle():
     (dict(nv1=9), "pair ranges"),
     (dict(nc2=20), "pair ranges"),
     (dict(max_iter=-1), "kmeans_max_iter"),
+    (dict(precision=7), "unknown precision"),
+    (dict(precision=isdf.ISDF_PREC_REFINE), "mixed forms only"),
 ])
 def test_invalid_arguments_rejected_before_device_work(kw, msg):
     dom = isdf.Domain((10.0,) * 3, (16, 16, 16))
-    a = dict(nocc=8, nstate=16, nv1=8, nc1=4, nv2=8, nc2=4, mu_points=64, max_iter=5)
+    a = dict(nocc=8, nstate=16, nv1=8, nc1=4, nv2=8, nc2=4, mu_points=64, max_iter=5,
+             precision=isdf.ISDF_PREC_FP64)
     a.update(kw)
     piv = np.zeros(4096, np.int32)
     with pytest.raises(ValueError, match=msg):
         isdf.solve(1, dom, a["nocc"], a["nstate"], a["nv1"], a["nc1"], a["nv2"], a["nc2"],
-                   a["mu_points"], 1, piv, a["max_iter"], stream=0)
+                   a["mu_points"], 1, piv, a["max_iter"], stream=0, precision=a["precision"])
 
 
 def test_check_validator_reproduces_reference_si8(si8):
