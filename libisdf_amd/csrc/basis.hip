@@ -96,16 +96,6 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct(int ng, int rk, const doubl
   }
 }
 
-// P(mu, nu) = Xi(piv[mu], nu), src/cpu/getbasis_scalapack.cpp:390-396.
-__global__ __launch_bounds__(256) void k_cct_gather(const double* __restrict__ C, long long ldc,
-                                                    const int* __restrict__ piv, int rk,
-                                                    double* __restrict__ P) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)rk * rk) return;
-  int mu = (int)(t % rk), nu = (int)(t / rk);
-  P[t] = C[(long long)nu * ldc + piv[mu]];
-}
-
 // trace summed in ascending order, eps = 1e-10 * trace / rk added to the
 // diagonal (getbasis_scalapack.cpp:398-428).  One block.
 __global__ __launch_bounds__(256) void k_trace_eps(double* __restrict__ P, int rk,
@@ -392,23 +382,29 @@ __global__ __launch_bounds__(256, 1) void k_make_fg(const double* __restrict__ L
 }
 
 // The in-place solve.  A workgroup owns Tile::BM rows; Tile::BN == kSB.
+// unit_lower: T is the identity (ng == n), so row i of the forward result
+// (column i of L^{-1}) vanishes left of column i: block columns before the
+// strip's own are zero and stay so, and the forward GEMMs start there.
 template <class Tile>
 __global__ __launch_bounds__(Tile::NT, 1) void k_potrs(double* __restrict__ T, long long ng,
                                                        long long ldt, int n,
                                                        const double* __restrict__ Ft,
-                                                       const double* __restrict__ Gt) {
+                                                       const double* __restrict__ Gt,
+                                                       int unit_lower) {
   static_assert(Tile::BN == kSB, "block column width");
+  static_assert(kSB % Tile::BM == 0 || Tile::BM % kSB == 0, "strip / block alignment");
   __shared__ double lds[Tile::LDS_DOUBLES];
   const long long m0 = (long long)blockIdx.x * Tile::BM;
   const int mlim = (int)std::min<long long>(Tile::BM, ng - m0);
   double* Tm = T + m0;
   const int nJ = (n + kSB - 1) / kSB;
+  const int Js = unit_lower ? (int)(m0 / kSB) : 0;
   for (int pass = 0; pass < 2; pass++) {
-    for (int s = 0; s < nJ; s++) {
+    for (int s = pass == 0 ? Js : 0; s < nJ; s++) {
       const int J = pass == 0 ? s : nJ - 1 - s;
       const int J0 = J * kSB;
       const int jb = min(kSB, n - J0);
-      const int kbeg = pass == 0 ? 0 : J0;
+      const int kbeg = pass == 0 ? Js * kSB : J0;
       const int kend = pass == 0 ? J0 + jb : n;
       typename Tile::Acc acc;
       Tile::zero(acc);
@@ -431,6 +427,7 @@ typedef MfmaTile<256, kSB, 16, 4, 1> PotrsTile1;  // 4 waves of 64 x 128
 typedef MfmaTile<128, kSB, 16, 2, 2> PotrsTile2;  // 4 waves of 64 x 64
 typedef ValuTile<256, kSB, 16, 1, 8> PotrsTile3;  // vector FMA, 8 waves of 256 x 16
 typedef MfmaTile<32, kSB, 16, 2, 2> PotrsTile4;   // 4 waves of 16 x 64: few rows, many workgroups
+typedef MfmaTile<16, kSB, 16, 1, 4> PotrsTile5;   // 4 waves of 16 x 32
 
 }  // namespace
 
@@ -444,6 +441,10 @@ int launch_gather_psimu(const double* psi, long long ld, int nstate, const int* 
 
 int zct_variants() { return 3; }
 
+int zct_pick_variant(long long ng, int requested) {
+  return requested >= 0 ? requested : (ng >= 256LL * 256 ? 0 : 2);
+}
+
 int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, int rk, int s1,
                int K1, int s2, int K2, bool dual, double* C, long long ldc, int variant,
                hipStream_t st) {
@@ -451,7 +452,7 @@ int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, in
   const double* A2 = psi + (long long)s2 * ld;
   const double* B1 = psimuT + (long long)s1 * rk;
   const double* B2 = psimuT + (long long)s2 * rk;
-  if (variant < 0) variant = (ng >= 256LL * 256) ? 0 : 2;
+  variant = zct_pick_variant(ng, variant);
 #define ISDF_ZCT_LAUNCH(TL, DUALV)                                                              \
   hipLaunchKernelGGL((k_zct<TL, DUALV>), dim3(cdiv(ng, TL::BM) * cdiv(rk, TL::BN)), dim3(TL::NT), \
                      0, st, ng, rk, A1, A2, ld, K1, K2, B1, B2, rk, C, ldc)
@@ -466,14 +467,6 @@ int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, in
 #undef ISDF_ZCT_LAUNCH
   ISDF_HIP_LAUNCHED("k_zct");
   return 0;
-}
-
-int launch_cct(const double* C, long long ldc, const int* piv, int rk, double* P, double* eps_dev,
-               hipStream_t st) {
-  hipLaunchKernelGGL(k_cct_gather, dim3(cdiv((long long)rk * rk, 256)), dim3(256), 0, st, C, ldc,
-                     piv, rk, P);
-  ISDF_HIP_LAUNCHED("k_cct_gather");
-  return launch_trace_eps(P, rk, eps_dev, st);
 }
 
 int launch_trace_eps(double* P, int rk, double* eps_dev, hipStream_t st) {
@@ -512,22 +505,24 @@ int launch_make_fg(const double* L, int rk, const double* Dinv, double* Dinv128,
   return 0;
 }
 
-int potrs_variants() { return 5; }
+int potrs_variants() { return 6; }
 
 int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* Ft, const double* Gt,
-                 int variant, hipStream_t st) {
+                 int variant, hipStream_t st, bool unit_lower) {
+  if (unit_lower && ng != rk) return set_error(1, "unit_lower needs a square identity operand");
   if (variant < 0) variant = (ng >= 256LL * 512) ? 0 : 2;
   switch (variant) {
 #define ISDF_POTRS_CASE(V)                                                                      \
   case V:                                                                                       \
     hipLaunchKernelGGL((k_potrs<PotrsTile##V>), dim3(cdiv(ng, PotrsTile##V::BM)),               \
-                       dim3(PotrsTile##V::NT), 0, st, T, ng, ldt, rk, Ft, Gt);                  \
+                       dim3(PotrsTile##V::NT), 0, st, T, ng, ldt, rk, Ft, Gt, (int)unit_lower); \
     break;
     ISDF_POTRS_CASE(0)
     ISDF_POTRS_CASE(1)
     ISDF_POTRS_CASE(2)
     ISDF_POTRS_CASE(3)
     ISDF_POTRS_CASE(4)
+    ISDF_POTRS_CASE(5)
 #undef ISDF_POTRS_CASE
     default:
       return set_error(1, "unknown solve tile variant");

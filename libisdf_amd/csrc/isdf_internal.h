@@ -103,17 +103,19 @@ int launch_gather_psimu(const double* psi, long long ld, int nstate, const int* 
                         double* psimuT, hipStream_t st);
 // `variant` < 0 picks the default tile for the shape; 0..zct_variants()-1 forces one.
 int zct_variants();
+// The tile variant launch_zct uses for ng rows when `requested` < 0.
+int zct_pick_variant(long long ng, int requested);
 int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, int rk, int s1,
                int K1, int s2, int K2, bool dual, double* C, long long ldc, int variant,
                hipStream_t st);
-int launch_cct(const double* C, long long ldc, const int* piv, int rk, double* P,
-               double* eps_dev, hipStream_t st);
 int launch_potrf(double* P, int rk, double* Dinv, int* info_dev, hipStream_t st);
 int launch_make_fg(const double* L, int rk, const double* Dinv, double* Dinv128, double* Ft,
                    double* Gt, hipStream_t st);
 int potrs_variants();
+// T (ng x rk, ld ldt) <- T (L L^T)^{-1} in place.  unit_lower: T is the rk x rk
+// identity, whose forward result is upper triangular (skips the zero blocks).
 int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* Ft,
-                 const double* Gt, int variant, hipStream_t st);
+                 const double* Gt, int variant, hipStream_t st, bool unit_lower = false);
 
 // trace of P summed in ascending order and eps = 1e-10 trace / rk added to the
 // diagonal (one block).
@@ -127,25 +129,28 @@ constexpr int kNB = 64;  // block size of the Cholesky (the solves use 128-block
 // with k contiguous, k padded to a multiple of 32 with zeros.
 // ---------------------------------------------------------------------------
 int lp_variants(int mode);
-// out[g*kp + k] = psi(g, c0 + k) as FP32 (zero for K <= k < kp).
-int launch_pack_psi(const double* psi, long long ld, int ng, int c0, int K, int kp, float* out,
-                    hipStream_t st);
-// out[nu*kp + k] = psi(piv[nu], c0 + k) as FP32 (zero for K <= k < kp).
-int launch_gather_rows(const double* psi, long long ld, const int* piv, int rk, int c0, int K,
-                       int kp, float* out, hipStream_t st);
-// Zt[g*ldz + nu] = P1(g,nu) P2(g,nu), zero for rk <= nu < ldz.
-int launch_zct_lp(int mode, int ng, int rk, int ldz, const float* A1, const float* A2, int kp1,
-                  int kp2, const float* B1, const float* B2, bool dual, float* Zt, int variant,
+// Bytes per operand element: 4 (FP32 rows) or 6 (BF16X3: three pre-split bf16
+// planes interleaved per 32-wide k chunk).
+int lp_operand_bytes(int mode);
+// Operand rows g of psi(g, c0 + k), k < kp (zero for k >= K), in the mode's format.
+int launch_pack_psi(int mode, const double* psi, long long ld, int ng, int c0, int K, int kp,
+                    void* out, hipStream_t st);
+// Operand rows nu of psi(piv[nu], c0 + k) in the mode's format.
+int launch_gather_rows(int mode, const double* psi, long long ld, const int* piv, int rk, int c0,
+                       int K, int kp, void* out, hipStream_t st);
+// Zt[g*ldz + nu] = P1(g,nu) P2(g,nu) (FP32), zero for rk <= nu < ldz.
+int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void* A2, int kp1,
+                  int kp2, const void* B1, const void* B2, bool dual, float* Zt, int variant,
                   hipStream_t st);
-// P(mu, nu) = Zt[piv[mu]][nu] (column-major FP64) and the eps shift.
+// P(mu, nu) = Zt[piv[mu]][nu] (column-major FP64; piv NULL: row mu) and the eps shift.
 int launch_cct_lp(const float* Zt, int ldz, const int* piv, int rk, double* P, double* eps_dev,
                   hipStream_t st);
 // W = (L L^T)^{-1} (FP64, column-major rk x rk) from the solve operands, and
-// Wf[nu*ldw + k] = W(k, nu) as FP32 (zero for k >= rk).
-int launch_inverse(int rk, const double* Ft, const double* Gt, double* W, int ldw, float* Wf,
-                   hipStream_t st);
+// operand rows nu of W(k, nu), k < ldw (zero for k >= rk), in the mode's format.
+int launch_inverse(int mode, int rk, const double* Ft, const double* Gt, double* W, int ldw,
+                   void* Wf, hipStream_t st);
 // theta(g, nu) = (or +=) sum_k Wf[nu][k] Zt[g][k], k < ldk.
-int launch_theta_lp(int mode, int ng, int rk, int ldk, const float* Wf, const float* Zt,
+int launch_theta_lp(int mode, int ng, int rk, int ldk, const void* Wf, const float* Zt,
                     double* theta, long long ldt, bool add, int variant, hipStream_t st);
 // Zt[g][nu] <- Zt[g][nu] - (theta Pe)(g, nu) in FP64 (the correction step's residual).
 int launch_resid(int ng, int rk, const double* theta, long long ldt, const double* Pe, float* Zt,

@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "isdf_internal.h"
 #include "mfma_gemm.h"
@@ -56,29 +57,98 @@ __device__ __forceinline__ void split3(f4 x, bf4& h, bf4& m, bf4& l) {
   }
 }
 
+// One GEMM operand of LpTile: ROWS rows of a 32-wide k chunk, staged from HBM
+// into LDS by NT threads in 16-byte pieces.
+//   FP32 operand: X(r, k) = ((const float*)X)[r*ld + k].  In the FP32 mode it
+//     lands in LDS as FP32 rows of stride RS32; in the BF16X3 mode each value
+//     is split (split3) into the three bf16 planes on the way.
+//   Pre-split operand (BF16X3 only): the three planes interleaved per chunk,
+//     plane p of X(r, k) at ((const __bf16*)X)[r*ld + (k/32)*96 + p*32 + k%32]
+//     (ld = 3 K), copied to the planes unchanged: no conversion work in the loop.
+// LDS planes: 64-byte rows of four 16-byte chunks, chunk c of row r stored at
+// chunk c ^ ((r >> 2) & 3), so the 16 rows a lane group reads at one k chunk
+// cover all 64 banks once.
+template <int ROWS, int NT, bool BF16, bool PRE>
+struct LpOperand {
+  static_assert(BF16 || !PRE, "pre-split operands are bf16");
+  static constexpr int RS32 = 36;  // FP32 LDS row stride (floats): conflict-free reads
+  static constexpr int RS16 = 32;  // bf16 plane row (elements)
+  static constexpr int PIECES = PRE ? ROWS * 12 : ROWS * 8;
+  static constexpr int PT = PIECES / NT;
+  static constexpr int LDS_BYTES = BF16 ? 3 * ROWS * RS16 * 2 : ROWS * RS32 * 4;
+  static_assert(PT >= 1 && PIECES % NT == 0, "staging split");
+  typedef int i4 __attribute__((ext_vector_type(4)));
+  typedef typename std::conditional<PRE, i4, f4>::type Reg;
+
+  __device__ static int swz(int r, int c8) {
+    return r * RS16 + 8 * ((c8 >> 1) ^ ((r >> 2) & 3)) + 4 * (c8 & 1);
+  }
+
+  __device__ static void load(const void* __restrict__ X, long long ld, int K, int k0, int lim,
+                              Reg (&reg)[PT]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < PT; q++) {
+      const int e = t + q * NT;
+      if constexpr (PRE) {
+        const int r = e / 12, pc = e % 12;
+        reg[q] = r < lim ? *(const i4*)((const __bf16*)X + (long long)r * ld + (k0 / 32) * 96 +
+                                        8 * pc)
+                         : i4{0, 0, 0, 0};
+      } else {
+        const int r = e >> 3, k = k0 + 4 * (e & 7);
+        reg[q] = (r < lim && k < K) ? *(const f4*)((const float*)X + (long long)r * ld + k)
+                                    : f4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+  }
+
+  __device__ static void store(char* base, const Reg (&reg)[PT]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < PT; q++) {
+      const int e = t + q * NT;
+      if constexpr (PRE) {
+        const int r = e / 12, pc = e % 12, p = pc >> 2, c16 = pc & 3;
+        __bf16* P = (__bf16*)base + p * ROWS * RS16;
+        *(i4*)(P + r * RS16 + 8 * (c16 ^ ((r >> 2) & 3))) = reg[q];
+      } else if constexpr (BF16) {
+        __bf16* P = (__bf16*)base;
+        const int o = swz(e >> 3, e & 7);
+        bf4 h, m, l;
+        split3(reg[q], h, m, l);
+        *(bf4*)(P + o) = h;
+        *(bf4*)(P + ROWS * RS16 + o) = m;
+        *(bf4*)(P + 2 * ROWS * RS16 + o) = l;
+      } else {
+        *(f4*)((float*)base + (e >> 3) * RS32 + 4 * (e & 7)) = reg[q];
+      }
+    }
+  }
+};
+
 // C[m][n] = sum_k A[m][k] B[n][k] over a BM x BN workgroup tile, K streamed in
-// chunks of 32 through double-buffered LDS.  A(m, k) = A[m*lda + k],
-// B(n, k) = B[n*ldb + k]; K, lda and ldb are multiples of 4 (K of 32) and the
-// operands are 16-byte aligned.  Rows m >= mlim and n >= nlim read as zero.
+// chunks of 32 through double-buffered LDS.  Operand formats as LpOperand
+// (A_PRE / B_PRE: pre-split bf16 planes); K is a multiple of 32, FP32 operand
+// rows are 16-byte aligned.  Rows m >= mlim and n >= nlim read as zero.
 //
 // 16x16 MFMA output layout (FP32 accumulators, measured on gfx950): lane l
 // holds D[4*(l/16) + r][l % 16] in register r.
-template <int BM_, int BN_, int WGM_, int WGN_, bool BF16_>
+template <int BM_, int BN_, int WGM_, int WGN_, bool BF16_, bool A_PRE_ = false,
+          bool B_PRE_ = false>
 struct LpTile {
   static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_;
-  static constexpr bool BF16 = BF16_;
+  static constexpr bool BF16 = BF16_, A_PRE = A_PRE_, B_PRE = B_PRE_;
   static constexpr int NT = 64 * WGM * WGN;
   static constexpr int TM = BM / WGM / 16;
   static constexpr int TN = BN / WGN / 16;
   static constexpr int KB = 32;
-  static constexpr int APT = BM * (KB / 4) / NT;  // float4 per thread per chunk
-  static constexpr int BPT = BN * (KB / 4) / NT;
-  static constexpr int RS32 = KB + 4;  // FP32 row stride in LDS (floats): no bank conflicts
-  static constexpr int RS16 = KB + 8;  // bf16 plane row stride (elements), 80 bytes
-  static constexpr int BUF_BYTES = BF16 ? 3 * (BM + BN) * RS16 * 2 : (BM + BN) * RS32 * 4;
+  typedef LpOperand<BM, NT, BF16, A_PRE> OpA;
+  typedef LpOperand<BN, NT, BF16, B_PRE> OpB;
+  static constexpr int RS32 = OpA::RS32, RS16 = OpA::RS16;
+  static constexpr int BUF_BYTES = OpA::LDS_BYTES + OpB::LDS_BYTES;
   static constexpr int LDS_BYTES = 2 * BUF_BYTES;
   static_assert(TM >= 1 && TN >= 1, "tile too small for the wave grid");
-  static_assert(APT >= 1 && BPT >= 1 && BM * 8 % NT == 0 && BN * 8 % NT == 0, "staging split");
   static_assert(LDS_BYTES <= 160 * 1024, "LDS");
 
   typedef f4 Acc[TM][TN];
@@ -90,76 +160,13 @@ struct LpTile {
       for (int j = 0; j < TN; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
   }
 
-  // Element q of a thread is float4 e = t + q*NT of the chunk: row e/8,
-  // k offset 4*(e%8); eight consecutive threads read one 128-byte row piece.
-  __device__ static void load(const float* __restrict__ A, long long lda,
-                              const float* __restrict__ B, long long ldb, int K, int k0, int mlim,
-                              int nlim, f4 (&ra)[APT], f4 (&rb)[BPT]) {
-    const int t = threadIdx.x;
-#pragma unroll
-    for (int q = 0; q < APT; q++) {
-      const int e = t + q * NT;
-      const int m = e >> 3, k = k0 + 4 * (e & 7);
-      ra[q] = (m < mlim && k < K) ? *(const f4*)(A + (long long)m * lda + k)
-                                  : f4{0.f, 0.f, 0.f, 0.f};
-    }
-#pragma unroll
-    for (int q = 0; q < BPT; q++) {
-      const int e = t + q * NT;
-      const int n = e >> 3, k = k0 + 4 * (e & 7);
-      rb[q] = (n < nlim && k < K) ? *(const f4*)(B + (long long)n * ldb + k)
-                                  : f4{0.f, 0.f, 0.f, 0.f};
-    }
-  }
-
-  __device__ static void store(char* lds, int buf, const f4 (&ra)[APT], const f4 (&rb)[BPT]) {
-    const int t = threadIdx.x;
-    if constexpr (!BF16) {
-      float* As = (float*)(lds + buf * BUF_BYTES);
-      float* Bs = As + BM * RS32;
-#pragma unroll
-      for (int q = 0; q < APT; q++) {
-        const int e = t + q * NT;
-        *(f4*)(As + (e >> 3) * RS32 + 4 * (e & 7)) = ra[q];
-      }
-#pragma unroll
-      for (int q = 0; q < BPT; q++) {
-        const int e = t + q * NT;
-        *(f4*)(Bs + (e >> 3) * RS32 + 4 * (e & 7)) = rb[q];
-      }
-    } else {
-      __bf16* Ap = (__bf16*)(lds + buf * BUF_BYTES);  // planes h, m, l of A
-      __bf16* Bp = Ap + 3 * BM * RS16;                // planes h, m, l of B
-#pragma unroll
-      for (int q = 0; q < APT; q++) {
-        const int e = t + q * NT;
-        const int o = (e >> 3) * RS16 + 4 * (e & 7);
-        bf4 h, m, l;
-        split3(ra[q], h, m, l);
-        *(bf4*)(Ap + o) = h;
-        *(bf4*)(Ap + BM * RS16 + o) = m;
-        *(bf4*)(Ap + 2 * BM * RS16 + o) = l;
-      }
-#pragma unroll
-      for (int q = 0; q < BPT; q++) {
-        const int e = t + q * NT;
-        const int o = (e >> 3) * RS16 + 4 * (e & 7);
-        bf4 h, m, l;
-        split3(rb[q], h, m, l);
-        *(bf4*)(Bp + o) = h;
-        *(bf4*)(Bp + BN * RS16 + o) = m;
-        *(bf4*)(Bp + 2 * BN * RS16 + o) = l;
-      }
-    }
-  }
-
   __device__ static void compute(const char* lds, int buf, Acc& acc) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave % WGM, wn = wave / WGM;
     const int lr = lane & 15, lk = lane >> 4;
     if constexpr (!BF16) {
       const float* As = (const float*)(lds + buf * BUF_BYTES);
-      const float* Bs = As + BM * RS32;
+      const float* Bs = (const float*)(lds + buf * BUF_BYTES + OpA::LDS_BYTES);
       const float* Aw = As + (wm * TM * 16 + lr) * RS32 + lk;
       const float* Bw = Bs + (wn * TN * 16 + lr) * RS32 + lk;
 #pragma unroll
@@ -177,52 +184,58 @@ struct LpTile {
       }
     } else {
       const __bf16* Ap = (const __bf16*)(lds + buf * BUF_BYTES);
-      const __bf16* Bp = Ap + 3 * BM * RS16;
-      const __bf16* Aw = Ap + (wm * TM * 16 + lr) * RS16 + 8 * lk;
-      const __bf16* Bw = Bp + (wn * TN * 16 + lr) * RS16 + 8 * lk;
-      bf8 ah[TM], am[TM], al[TM];
+      const __bf16* Bp = (const __bf16*)(lds + buf * BUF_BYTES + OpA::LDS_BYTES);
+      // Rows wm*TM*16 + 16 i + lr: (row >> 2) & 3 == (lr >> 2) & 3.
+      const int co = lr * RS16 + 8 * (lk ^ ((lr >> 2) & 3));
+      const __bf16* Aw = Ap + wm * TM * 16 * RS16 + co;
+      const __bf16* Bw = Bp + wn * TN * 16 * RS16 + co;
+      bf8 a[3][TM], b[3][TN];
 #pragma unroll
-      for (int i = 0; i < TM; i++) {
-        ah[i] = *(const bf8*)(Aw + 16 * i * RS16);
-        am[i] = *(const bf8*)(Aw + BM * RS16 + 16 * i * RS16);
-        al[i] = *(const bf8*)(Aw + 2 * BM * RS16 + 16 * i * RS16);
+      for (int p = 0; p < 3; p++) {
+#pragma unroll
+        for (int i = 0; i < TM; i++) a[p][i] = *(const bf8*)(Aw + (p * BM + 16 * i) * RS16);
+#pragma unroll
+        for (int j = 0; j < TN; j++) b[p][j] = *(const bf8*)(Bw + (p * BN + 16 * j) * RS16);
       }
+      // Six partial products, smallest first; each pass issues TM*TN independent MFMAs.
+      constexpr int PA[6] = {0, 2, 1, 0, 1, 0};
+      constexpr int PB[6] = {2, 0, 1, 1, 0, 0};
 #pragma unroll
-      for (int j = 0; j < TN; j++) {
-        const bf8 bh = *(const bf8*)(Bw + 16 * j * RS16);
-        const bf8 bm = *(const bf8*)(Bw + BN * RS16 + 16 * j * RS16);
-        const bf8 bl = *(const bf8*)(Bw + 2 * BN * RS16 + 16 * j * RS16);
+      for (int p = 0; p < 6; p++)
 #pragma unroll
-        for (int i = 0; i < TM; i++) {
-          // smallest terms first
-          f4 c = acc[i][j];
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[i], bm, c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bm, c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[i], bh, c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, c, 0, 0, 0);
-          acc[i][j] = c;
-        }
-      }
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+          for (int j = 0; j < TN; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[PA[p]][i], b[PB[p]][j],
+                                                                acc[i][j], 0, 0, 0);
     }
   }
 
   // acc += A[0:mlim, 0:K] B[0:nlim, 0:K]^T.  Ends with a workgroup barrier.
-  __device__ static void run(const float* __restrict__ A, long long lda,
-                             const float* __restrict__ B, long long ldb, int K, int mlim,
+  __device__ static void run(const void* __restrict__ A, long long lda,
+                             const void* __restrict__ B, long long ldb, int K, int mlim,
                              int nlim, char* lds, Acc& acc) {
-    f4 ra[APT], rb[BPT];
+    typename OpA::Reg ra[OpA::PT];
+    typename OpB::Reg rb[OpB::PT];
     const int nch = (K + KB - 1) / KB;
     if (nch == 0) return;
-    load(A, lda, B, ldb, K, 0, mlim, nlim, ra, rb);
-    store(lds, 0, ra, rb);
+    OpA::load(A, lda, K, 0, mlim, ra);
+    OpB::load(B, ldb, K, 0, nlim, rb);
+    OpA::store(lds, ra);
+    OpB::store(lds + OpA::LDS_BYTES, rb);
     __syncthreads();
     for (int c = 0; c < nch; c++) {
       const bool more = c + 1 < nch;
-      if (more) load(A, lda, B, ldb, K, (c + 1) * KB, mlim, nlim, ra, rb);
+      if (more) {
+        OpA::load(A, lda, K, (c + 1) * KB, mlim, ra);
+        OpB::load(B, ldb, K, (c + 1) * KB, nlim, rb);
+      }
       compute(lds, c & 1, acc);
-      if (more) store(lds, (c + 1) & 1, ra, rb);
+      if (more) {
+        char* nb = lds + ((c + 1) & 1) * BUF_BYTES;
+        OpA::store(nb, ra);
+        OpB::store(nb + OpA::LDS_BYTES, rb);
+      }
       __syncthreads();
     }
   }
@@ -253,12 +266,31 @@ __device__ __forceinline__ void xcd_map(int b, int ntile, int nstrip, int* s, in
   }
 }
 
-// Packs columns [c0, c0+K) of the FP64 column-major psi (ld) into FP32 rows
-// out[g*kp + k], zero for K <= k < kp.  64 grid points x 32 states per block,
-// transposed through LDS so that both sides are coalesced.
+// Writes operand element (r, k) of a row-major operand with kp (multiple of
+// 32) columns: FP32 at out[r*kp + k], or, SPLIT, its three bf16 parts in the
+// pre-split layout of LpOperand (row stride 3 kp).
+template <bool SPLIT>
+__device__ __forceinline__ void put_operand(void* out, int kp, long long r, int k, float v) {
+  if constexpr (SPLIT) {
+    __bf16* o = (__bf16*)out + r * 3 * kp + (k >> 5) * 96 + (k & 31);
+    const __bf16 h = (__bf16)v;
+    const float r1 = v - (float)h;
+    const __bf16 m = (__bf16)r1;
+    o[0] = h;
+    o[32] = m;
+    o[64] = (__bf16)(r1 - (float)m);
+  } else {
+    ((float*)out)[r * kp + k] = v;
+  }
+}
+
+// Packs columns [c0, c0+K) of the FP64 column-major psi (ld) into operand rows
+// g (FP32 or pre-split), zero for K <= k < kp.  64 grid points x 32 states per
+// block, transposed through LDS so that both sides are coalesced.
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_pack_rows(const double* __restrict__ psi, long long ld,
                                                    int ng, int c0, int K, int kp,
-                                                   float* __restrict__ out) {
+                                                   void* __restrict__ out) {
   __shared__ float tile[32][65];
   const long long g0 = (long long)blockIdx.x * 64;
   const int k0 = blockIdx.y * 32;
@@ -275,28 +307,33 @@ __global__ __launch_bounds__(256) void k_pack_rows(const double* __restrict__ ps
   for (int q = 0; q < 8; q++) {
     const int gg = (t >> 5) + 8 * q, kk = t & 31;
     const long long g = g0 + gg;
-    if (g < ng) out[g * kp + k0 + kk] = tile[kk][gg];
+    if (g < ng) put_operand<SPLIT>(out, kp, g, k0 + kk, tile[kk][gg]);
   }
 }
 
-// out[nu*kp + k] = psi(piv[nu], c0 + k) in FP32, zero for k >= K.
+// Operand row nu = psi(piv[nu], c0 + k) (FP32 or pre-split), zero for k >= K.
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_gather_rows(const double* __restrict__ psi, long long ld,
                                                      const int* __restrict__ piv, int rk, int c0,
-                                                     int K, int kp, float* __restrict__ out) {
+                                                     int K, int kp, void* __restrict__ out) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)rk * kp) return;
   const int k = (int)(t % kp), nu = (int)(t / kp);
-  out[t] = k < K ? (float)psi[(long long)(c0 + k) * ld + piv[nu]] : 0.f;
+  put_operand<SPLIT>(out, kp, nu, k,
+                     k < K ? (float)psi[(long long)(c0 + k) * ld + piv[nu]] : 0.f);
 }
 
 // Zt[g][nu] = P1(g, nu) * P2(g, nu) (P2 = P1 when the pair sets agree), for
 // nu < ldz (columns rk..ldz come out zero, as the Theta GEMM's k padding).
+// A1/A2 (grid rows) and B1/B2 (pivot rows) in the Tile's operand formats, row
+// strides lda1/lda2 (elements of the format).
 template <class Tile, bool DUAL>
 __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lp(int ng, int rk, int ldz,
-                                                        const float* __restrict__ A1,
-                                                        const float* __restrict__ A2, int kp1,
-                                                        int kp2, const float* __restrict__ B1,
-                                                        const float* __restrict__ B2,
+                                                        const void* __restrict__ A1,
+                                                        const void* __restrict__ A2, int K1,
+                                                        int K2, long long lda1, long long lda2,
+                                                        const void* __restrict__ B1,
+                                                        const void* __restrict__ B2,
                                                         float* __restrict__ Zt) {
   __shared__ __attribute__((aligned(16))) char lds[Tile::LDS_BYTES];
   const int nstrip = (ng + Tile::BM - 1) / Tile::BM;
@@ -309,12 +346,15 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lp(int ng, int rk, int ldz,
   const int mlim = (int)std::min<long long>(Tile::BM, ng - m0);
   const int nlim = std::min(Tile::BN, rk - n0);   // operand rows that exist
   const int wlim = std::min(Tile::BN, ldz - n0);  // columns written (zeros past rk)
+  constexpr int ESA = Tile::A_PRE ? 2 : 4, ESB = Tile::B_PRE ? 2 : 4;  // element bytes
   typename Tile::Acc acc1, acc2;
   Tile::zero(acc1);
-  Tile::run(A1 + m0 * kp1, kp1, B1 + (long long)n0 * kp1, kp1, kp1, mlim, nlim, lds, acc1);
+  Tile::run((const char*)A1 + m0 * lda1 * ESA, lda1, (const char*)B1 + (long long)n0 * lda1 * ESB,
+            lda1, K1, mlim, nlim, lds, acc1);
   if constexpr (DUAL) {
     Tile::zero(acc2);
-    Tile::run(A2 + m0 * kp2, kp2, B2 + (long long)n0 * kp2, kp2, kp2, mlim, nlim, lds, acc2);
+    Tile::run((const char*)A2 + m0 * lda2 * ESA, lda2,
+              (const char*)B2 + (long long)n0 * lda2 * ESB, lda2, K2, mlim, nlim, lds, acc2);
   }
   float* Zm = Zt + m0 * ldz + n0;
 #pragma unroll
@@ -335,14 +375,16 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lp(int ng, int rk, int ldz,
 }
 
 // P(mu, nu) = Zt[piv[mu]][nu] (column-major rk x rk FP64),
-// src/cpu/getbasis_scalapack.cpp:390-396.
+// src/cpu/getbasis_scalapack.cpp:390-396; piv == NULL reads row mu (Zt holds
+// the pivot rows only).
 __global__ __launch_bounds__(256) void k_cct_lp(const float* __restrict__ Zt, int ldz,
                                                 const int* __restrict__ piv, int rk,
                                                 double* __restrict__ P) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)rk * rk) return;
   const int mu = (int)(t % rk), nu = (int)(t / rk);
-  P[t] = (double)Zt[(long long)piv[mu] * ldz + nu];
+  const long long row = piv ? piv[mu] : mu;
+  P[t] = (double)Zt[row * ldz + nu];
 }
 
 // T = I (n x n, column-major FP64).
@@ -352,21 +394,25 @@ __global__ __launch_bounds__(256) void k_identity(double* __restrict__ T, int n)
   T[t] = (t % n == t / n) ? 1.0 : 0.0;
 }
 
-// Wf[nu*ldw + k] = W(k, nu) in FP32, zero for k >= rk (W column-major FP64).
+// Operand row nu of W: W(k, nu) (FP32 or pre-split, ldw columns), zero for
+// k >= rk (W column-major FP64).
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_pack_w(const double* __restrict__ W, int rk, int ldw,
-                                                float* __restrict__ Wf) {
+                                                void* __restrict__ Wf) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)rk * ldw) return;
   const int k = (int)(t % ldw), nu = (int)(t / ldw);
-  Wf[t] = k < rk ? (float)W[(long long)nu * rk + k] : 0.f;
+  put_operand<SPLIT>(Wf, ldw, nu, k, k < rk ? (float)W[(long long)nu * rk + k] : 0.f);
 }
 
-// theta(g, nu) (=|+=) sum_k Wf[nu][k] Zt[g][k]: rows nu of the tile are the A
-// operand, grid points g the B operand; each lane group writes 16 consecutive
-// g of one column of the column-major theta.
+// theta(g, nu) (=|+=) sum_k W(k, nu) Zt[g][k]: rows nu of the tile are the A
+// operand (Wf, row stride ldwr elements of its format), grid points g the B
+// operand (FP32 Zt, row stride ldk); each lane group writes 16 consecutive g
+// of one column of the column-major theta.
 template <class Tile, bool ADD>
 __global__ __launch_bounds__(Tile::NT, 1) void k_theta_lp(int ng, int rk, int ldk,
-                                                          const float* __restrict__ Wf,
+                                                          const void* __restrict__ Wf,
+                                                          long long ldwr,
                                                           const float* __restrict__ Zt,
                                                           double* __restrict__ theta,
                                                           long long ldt) {
@@ -380,9 +426,11 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_theta_lp(int ng, int rk, int ld
   const int v0 = vt * Tile::BM;
   const int mlim = std::min(Tile::BM, rk - v0);
   const int nlim = (int)std::min<long long>(Tile::BN, ng - g0);
+  constexpr int ESA = Tile::A_PRE ? 2 : 4;
   typename Tile::Acc acc;
   Tile::zero(acc);
-  Tile::run(Wf + (long long)v0 * ldk, ldk, Zt + g0 * ldk, ldk, ldk, mlim, nlim, lds, acc);
+  Tile::run((const char*)Wf + (long long)v0 * ldwr * ESA, ldwr, Zt + g0 * ldk, ldk, ldk, mlim,
+            nlim, lds, acc);
   double* Tt = theta + (long long)v0 * ldt + g0;
 #pragma unroll
   for (int i = 0; i < Tile::TM; i++)
@@ -432,58 +480,75 @@ __global__ __launch_bounds__(ResidTile::NT, 1) void k_resid(int ng, int rk,
   });
 }
 
-// Tile variants.  Both contractions use the same shapes per mode.
-typedef LpTile<256, 128, 4, 2, false> LpF32_0;   // 8 waves of 64 x 64
-typedef LpTile<128, 128, 2, 2, false> LpF32_1;   // 4 waves of 64 x 64
-typedef LpTile<256, 256, 4, 2, false> LpF32_2;   // 8 waves of 64 x 128
-typedef LpTile<128, 128, 2, 2, true> LpB16_0;    // 4 waves of 64 x 64
-typedef LpTile<192, 128, 2, 2, true> LpB16_1;    // 4 waves of 96 x 64
-typedef LpTile<128, 64, 2, 2, true> LpB16_2;     // 4 waves of 64 x 32
+// Tile variants.  FP32 mode: FP32 operands.  BF16X3 mode: Z C^T takes both
+// operands pre-split, Theta takes W pre-split and splits Zt while staging.
+typedef LpTile<256, 128, 4, 2, false> LpF32_0;              // 8 waves of 64 x 64
+typedef LpTile<128, 128, 2, 2, false> LpF32_1;              // 4 waves of 64 x 64
+typedef LpTile<256, 256, 4, 2, false> LpF32_2;              // 8 waves of 64 x 128
+typedef LpTile<256, 128, 4, 2, true, true, true> LpZB16_0;  // 8 waves of 64 x 64
+typedef LpTile<128, 256, 2, 4, true, true, true> LpZB16_1;
+typedef LpTile<128, 128, 2, 2, true, true, true> LpZB16_2;  // 4 waves
+typedef LpTile<256, 128, 4, 2, true, true, false> LpTB16_0;
+typedef LpTile<128, 256, 2, 4, true, true, false> LpTB16_1;
+typedef LpTile<128, 128, 2, 2, true, true, false> LpTB16_2;
 
 }  // namespace
 
 int lp_variants(int mode) { return 3; }
 
-int launch_pack_psi(const double* psi, long long ld, int ng, int c0, int K, int kp, float* out,
-                    hipStream_t st) {
-  hipLaunchKernelGGL(k_pack_rows, dim3(cdiv(ng, 64), cdiv(kp, 32)), dim3(256), 0, st, psi, ld, ng,
-                     c0, K, kp, out);
+int lp_operand_bytes(int mode) { return mode == 2 ? 6 : 4; }
+
+int launch_pack_psi(int mode, const double* psi, long long ld, int ng, int c0, int K, int kp,
+                    void* out, hipStream_t st) {
+  const dim3 grid(cdiv(ng, 64), cdiv(kp, 32));
+  if (mode == 2)
+    hipLaunchKernelGGL(k_pack_rows<true>, grid, dim3(256), 0, st, psi, ld, ng, c0, K, kp, out);
+  else
+    hipLaunchKernelGGL(k_pack_rows<false>, grid, dim3(256), 0, st, psi, ld, ng, c0, K, kp, out);
   ISDF_HIP_LAUNCHED("k_pack_rows");
   return 0;
 }
 
-int launch_gather_rows(const double* psi, long long ld, const int* piv, int rk, int c0, int K,
-                       int kp, float* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_gather_rows, dim3(cdiv((long long)rk * kp, 256)), dim3(256), 0, st, psi,
-                     ld, piv, rk, c0, K, kp, out);
+int launch_gather_rows(int mode, const double* psi, long long ld, const int* piv, int rk, int c0,
+                       int K, int kp, void* out, hipStream_t st) {
+  const dim3 grid(cdiv((long long)rk * kp, 256));
+  if (mode == 2)
+    hipLaunchKernelGGL(k_gather_rows<true>, grid, dim3(256), 0, st, psi, ld, piv, rk, c0, K, kp,
+                       out);
+  else
+    hipLaunchKernelGGL(k_gather_rows<false>, grid, dim3(256), 0, st, psi, ld, piv, rk, c0, K, kp,
+                       out);
   ISDF_HIP_LAUNCHED("k_gather_rows");
   return 0;
 }
 
-int launch_zct_lp(int mode, int ng, int rk, int ldz, const float* A1, const float* A2, int kp1,
-                  int kp2, const float* B1, const float* B2, bool dual, float* Zt, int variant,
+int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void* A2, int kp1,
+                  int kp2, const void* B1, const void* B2, bool dual, float* Zt, int variant,
                   hipStream_t st) {
-  if (variant < 0) variant = 0;
-  // Two 256 x 256 FP32 accumulators do not fit the register file.
+  if (variant < 0) variant = mode == 1 ? 2 : 0;  // measured best on the 96^3 workload
+  // Two accumulators: 256 x 256 (FP32) and the 8-wave BF16X3 tiles spill; use
+  // the smaller tiles.
   if (dual && mode == 1 && variant == 2) variant = 0;
+  if (dual && mode == 2 && variant != 2) variant = 2;
+  const long long lda1 = mode == 2 ? 3LL * kp1 : kp1, lda2 = mode == 2 ? 3LL * kp2 : kp2;
 #define ISDF_ZLP(TL)                                                                           \
   do {                                                                                         \
     const unsigned nb = cdiv(ng, TL::BM) * cdiv(ldz, TL::BN);                                  \
     if (dual)                                                                                  \
       hipLaunchKernelGGL((k_zct_lp<TL, true>), dim3(nb), dim3(TL::NT), 0, st, ng, rk, ldz, A1, \
-                         A2, kp1, kp2, B1, B2, Zt);                                            \
+                         A2, kp1, kp2, lda1, lda2, B1, B2, Zt);                                \
     else                                                                                       \
       hipLaunchKernelGGL((k_zct_lp<TL, false>), dim3(nb), dim3(TL::NT), 0, st, ng, rk, ldz,    \
-                         A1, A2, kp1, kp2, B1, B2, Zt);                                        \
+                         A1, A2, kp1, kp2, lda1, lda2, B1, B2, Zt);                            \
   } while (0)
   if (mode == 1) {
     if (variant == 1) ISDF_ZLP(LpF32_1);
     else if (variant == 2) ISDF_ZLP(LpF32_2);
     else ISDF_ZLP(LpF32_0);
   } else {
-    if (variant == 1) ISDF_ZLP(LpB16_1);
-    else if (variant == 2) ISDF_ZLP(LpB16_2);
-    else ISDF_ZLP(LpB16_0);
+    if (variant == 1) ISDF_ZLP(LpZB16_1);
+    else if (variant == 2) ISDF_ZLP(LpZB16_2);
+    else ISDF_ZLP(LpZB16_0);
   }
 #undef ISDF_ZLP
   ISDF_HIP_LAUNCHED("k_zct_lp");
@@ -498,39 +563,44 @@ int launch_cct_lp(const float* Zt, int ldz, const int* piv, int rk, double* P, d
   return launch_trace_eps(P, rk, eps_dev, st);
 }
 
-int launch_inverse(int rk, const double* Ft, const double* Gt, double* W, int ldw, float* Wf,
-                   hipStream_t st) {
+int launch_inverse(int mode, int rk, const double* Ft, const double* Gt, double* W, int ldw,
+                   void* Wf, hipStream_t st) {
   hipLaunchKernelGGL(k_identity, dim3(cdiv((long long)rk * rk, 256)), dim3(256), 0, st, W, rk);
   ISDF_HIP_LAUNCHED("k_identity");
-  // W = I (L L^T)^{-1}: the blocked in-place solve with its 32-row tile (rk/32 workgroups).
-  if (int rc = launch_potrs(W, rk, rk, rk, Ft, Gt, 4, st)) return rc;
-  hipLaunchKernelGGL(k_pack_w, dim3(cdiv((long long)rk * ldw, 256)), dim3(256), 0, st, W, rk, ldw,
-                     Wf);
+  // W = I (L L^T)^{-1}: the blocked in-place solve with its 16-row tile (rk/16
+  // workgroups), skipping the blocks the identity leaves zero.
+  if (int rc = launch_potrs(W, rk, rk, rk, Ft, Gt, 5, st, true)) return rc;
+  const dim3 grid(cdiv((long long)rk * ldw, 256));
+  if (mode == 2)
+    hipLaunchKernelGGL(k_pack_w<true>, grid, dim3(256), 0, st, W, rk, ldw, Wf);
+  else
+    hipLaunchKernelGGL(k_pack_w<false>, grid, dim3(256), 0, st, W, rk, ldw, Wf);
   ISDF_HIP_LAUNCHED("k_pack_w");
   return 0;
 }
 
-int launch_theta_lp(int mode, int ng, int rk, int ldk, const float* Wf, const float* Zt,
+int launch_theta_lp(int mode, int ng, int rk, int ldk, const void* Wf, const float* Zt,
                     double* theta, long long ldt, bool add, int variant, hipStream_t st) {
-  if (variant < 0) variant = 0;
+  if (variant < 0) variant = mode == 1 ? 2 : 0;  // measured best on the 96^3 workload
+  const long long ldwr = mode == 2 ? 3LL * ldk : ldk;
 #define ISDF_TLP(TL)                                                                            \
   do {                                                                                          \
     const unsigned nb = cdiv(ng, TL::BN) * cdiv(rk, TL::BM);                                    \
     if (add)                                                                                    \
       hipLaunchKernelGGL((k_theta_lp<TL, true>), dim3(nb), dim3(TL::NT), 0, st, ng, rk, ldk, Wf, \
-                         Zt, theta, ldt);                                                       \
+                         ldwr, Zt, theta, ldt);                                                 \
     else                                                                                        \
       hipLaunchKernelGGL((k_theta_lp<TL, false>), dim3(nb), dim3(TL::NT), 0, st, ng, rk, ldk,   \
-                         Wf, Zt, theta, ldt);                                                   \
+                         Wf, ldwr, Zt, theta, ldt);                                             \
   } while (0)
   if (mode == 1) {
     if (variant == 1) ISDF_TLP(LpF32_1);
     else if (variant == 2) ISDF_TLP(LpF32_2);
     else ISDF_TLP(LpF32_0);
   } else {
-    if (variant == 1) ISDF_TLP(LpB16_1);
-    else if (variant == 2) ISDF_TLP(LpB16_2);
-    else ISDF_TLP(LpB16_0);
+    if (variant == 1) ISDF_TLP(LpTB16_1);
+    else if (variant == 2) ISDF_TLP(LpTB16_2);
+    else ISDF_TLP(LpTB16_0);
   }
 #undef ISDF_TLP
   ISDF_HIP_LAUNCHED("k_theta_lp");

@@ -80,13 +80,17 @@ struct GlibcRand {
   }
 };
 
+// Stage timing events: 0 start, 1 weights, 2 K-means, 3 selection, 4 gather
+// (main stream), 5 Z C^T (main), 6/7/8 start / C C^T / end of the Cholesky
+// branch (helper stream), 9 join (main), 10 solve done (main).
+constexpr int kNumEvents = 11;
 struct Events {
   bool on = false;
-  hipEvent_t e[10];
+  hipEvent_t e[kNumEvents];
   int create(bool want) {
     on = want;
     if (!on) return 0;
-    for (int i = 0; i < 10; i++) ISDF_HIP_TRY(hipEventCreate(&e[i]));
+    for (int i = 0; i < kNumEvents; i++) ISDF_HIP_TRY(hipEventCreate(&e[i]));
     return 0;
   }
   int rec(int i, hipStream_t st) {
@@ -100,9 +104,33 @@ struct Events {
   }
   ~Events() {
     if (on)
-      for (int i = 0; i < 10; i++) (void)hipEventDestroy(e[i]);
+      for (int i = 0; i < kNumEvents; i++) (void)hipEventDestroy(e[i]);
   }
 };
+
+// Per-device helper stream (default priority) and fork/join events.  The
+// C C^T -> Cholesky -> solve-operand branch depends only on the pivots, not
+// on the Ng x rk Z C^T, so it runs on the helper stream beside that GEMM.
+struct Aux {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, pack = nullptr;
+};
+std::vector<Aux> g_aux;
+
+int aux_stream(Aux** out) {
+  int dev = 0;
+  ISDF_HIP_TRY(hipGetDevice(&dev));
+  if ((int)g_aux.size() <= dev) g_aux.resize(dev + 1);
+  Aux& a = g_aux[dev];
+  if (!a.s) {
+    ISDF_HIP_TRY(hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking));
+    ISDF_HIP_TRY(hipEventCreateWithFlags(&a.fork, hipEventDisableTiming));
+    ISDF_HIP_TRY(hipEventCreateWithFlags(&a.join, hipEventDisableTiming));
+    ISDF_HIP_TRY(hipEventCreateWithFlags(&a.pack, hipEventDisableTiming));
+  }
+  *out = &a;
+  return 0;
+}
 
 int irank(const isdf_problem* p) {
   if (p->mu_points != 0) return p->mu_points;
@@ -148,60 +176,63 @@ int env_variant(const char* name) {
   return v ? std::atoi(v) : -1;
 }
 
-// Basis construction of the mixed forms (records events 4..9 like the FP64
-// path): packed operands -> low-precision Z C^T (Zt, g-major FP32) -> C C^T +
-// eps from the pivot rows of Zt -> FP64 Cholesky -> FP64 inverse W ->
-// low-precision Theta = Zt W into d_theta [-> FP64 residual, Theta += R W].
-int basis_mixed(const isdf_problem* p, const double* d_psi, long long ld, int ng, int rk, int mode,
-                bool refine, char* base, size_t off_A1, size_t off_A2, size_t off_B1,
-                size_t off_B2, size_t off_Zt, size_t off_W, size_t off_Wf, size_t off_Pe, int kp1,
-                int kp2, int ldz, const int* piv_dev, double* P, double* Dinv, double* Dinv128,
-                double* Ft, double* Gt, int* info_dev, double* eps_dev, double* d_theta,
-                const isdf_debug* dbg, Events& ev, hipStream_t st) {
-  const int s1 = p->nocc - p->nv1, K1 = p->nv1 + p->nc1;
-  const int s2 = p->nocc - p->nv2, K2 = p->nv2 + p->nc2;
+// Basis construction of the mixed forms.  Helper stream: the operand rows of
+// psi (queued at the start of the solve, beside K-means), then, from the
+// pivots, C C^T + eps from a small low-precision contraction of the pivot rows
+// -> FP64 Cholesky -> FP64 inverse W.  Main stream: the Z C^T contraction
+// (Zt, g-major FP32) -> join -> Theta = Zt W into d_theta [-> FP64 residual,
+// Theta += R W].
+struct MixedBufs {
+  void *A1, *A2, *B1, *B2, *Wf;
+  float *Zt, *Pz;
+  double *W, *Pe;
+};
+
+int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, const MixedBufs& m,
+                int kp1, int kp2, int ldz, double* P, double* Dinv, double* Dinv128, double* Ft,
+                double* Gt, int* info_dev, double* eps_dev, double* d_theta,
+                const isdf_debug* dbg, Events& ev, Aux* aux, hipStream_t st) {
   const bool dual = (p->nv1 != p->nv2 || p->nc1 != p->nc2);
-  float* A1 = (float*)(base + off_A1);
-  float* A2 = dual ? (float*)(base + off_A2) : A1;
-  float* B1 = (float*)(base + off_B1);
-  float* B2 = dual ? (float*)(base + off_B2) : B1;
-  float* Zt = (float*)(base + off_Zt);
-  double* W = (double*)(base + off_W);
-  float* Wf = (float*)(base + off_Wf);
-  double* Pe = refine ? (double*)(base + off_Pe) : nullptr;
-  // Operands: psi rows of the pair sets and their values at the pivots.
-  if (int rc = launch_pack_psi(d_psi, ld, ng, s1, K1, kp1, A1, st)) return rc;
-  if (int rc = launch_gather_rows(d_psi, ld, piv_dev, rk, s1, K1, kp1, B1, st)) return rc;
-  if (dual) {
-    if (int rc = launch_pack_psi(d_psi, ld, ng, s2, K2, kp2, A2, st)) return rc;
-    if (int rc = launch_gather_rows(d_psi, ld, piv_dev, rk, s2, K2, kp2, B2, st)) return rc;
-  }
-  if (int rc = ev.rec(4, st)) return rc;
-  if (int rc = launch_zct_lp(mode, ng, rk, ldz, A1, A2, kp1, dual ? kp2 : kp1, B1, B2, dual, Zt,
-                             env_variant("ISDF_LP_ZCT_TILE"), st))
+  hipStream_t s2 = aux->s;
+  ISDF_HIP_TRY(hipEventRecord(aux->fork, st));
+  ISDF_HIP_TRY(hipStreamWaitEvent(s2, aux->fork, 0));
+  if (int rc = ev.rec(6, s2)) return rc;
+  // C C^T: the contraction on the pivot rows alone (rk x rk x K).
+  if (int rc = launch_zct_lp(mode, rk, rk, ldz, m.B1, m.B2, kp1, dual ? kp2 : kp1, m.B1, m.B2,
+                             dual, m.Pz, -1, s2))
     return rc;
-  if (int rc = ev.rec(5, st)) return rc;
-  if (int rc = launch_cct_lp(Zt, ldz, piv_dev, rk, P, eps_dev, st)) return rc;
+  if (int rc = launch_cct_lp(m.Pz, ldz, nullptr, rk, P, eps_dev, s2)) return rc;
   if (dbg && dbg->d_cct)
     ISDF_HIP_TRY(hipMemcpyAsync(dbg->d_cct, P, sizeof(double) * (size_t)rk * rk,
-                                hipMemcpyDeviceToDevice, st));
+                                hipMemcpyDeviceToDevice, s2));
   if (refine)
-    ISDF_HIP_TRY(hipMemcpyAsync(Pe, P, sizeof(double) * (size_t)rk * rk, hipMemcpyDeviceToDevice,
-                                st));
-  if (int rc = ev.rec(6, st)) return rc;
-  ISDF_HIP_TRY(hipMemsetAsync(info_dev, 0, sizeof(int), st));
-  if (int rc = launch_potrf(P, rk, Dinv, info_dev, st)) return rc;
-  if (int rc = ev.rec(7, st)) return rc;
-  if (int rc = launch_make_fg(P, rk, Dinv, Dinv128, Ft, Gt, st)) return rc;
-  if (int rc = launch_inverse(rk, Ft, Gt, W, ldz, Wf, st)) return rc;
-  if (int rc = ev.rec(8, st)) return rc;
-  const int tv = env_variant("ISDF_LP_THETA_TILE");
-  if (int rc = launch_theta_lp(mode, ng, rk, ldz, Wf, Zt, d_theta, ng, false, tv, st)) return rc;
-  if (refine) {
-    if (int rc = launch_resid(ng, rk, d_theta, ng, Pe, Zt, ldz, st)) return rc;
-    if (int rc = launch_theta_lp(mode, ng, rk, ldz, Wf, Zt, d_theta, ng, true, tv, st)) return rc;
-  }
+    ISDF_HIP_TRY(hipMemcpyAsync(m.Pe, P, sizeof(double) * (size_t)rk * rk,
+                                hipMemcpyDeviceToDevice, s2));
+  if (int rc = ev.rec(7, s2)) return rc;
+  ISDF_HIP_TRY(hipMemsetAsync(info_dev, 0, sizeof(int), s2));
+  if (int rc = launch_potrf(P, rk, Dinv, info_dev, s2)) return rc;
+  if (int rc = launch_make_fg(P, rk, Dinv, Dinv128, Ft, Gt, s2)) return rc;
+  if (int rc = launch_inverse(mode, rk, Ft, Gt, m.W, ldz, m.Wf, s2)) return rc;
+  if (int rc = ev.rec(8, s2)) return rc;
+  ISDF_HIP_TRY(hipEventRecord(aux->join, s2));
+
+  // Main stream: Z C^T once the packed psi rows are in place.
+  ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->pack, 0));
+  if (int rc = launch_zct_lp(mode, ng, rk, ldz, m.A1, m.A2, kp1, dual ? kp2 : kp1, m.B1, m.B2,
+                             dual, m.Zt, env_variant("ISDF_LP_ZCT_TILE"), st))
+    return rc;
+  if (int rc = ev.rec(5, st)) return rc;
+  ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->join, 0));
   if (int rc = ev.rec(9, st)) return rc;
+  const int tv = env_variant("ISDF_LP_THETA_TILE");
+  if (int rc = launch_theta_lp(mode, ng, rk, ldz, m.Wf, m.Zt, d_theta, ng, false, tv, st))
+    return rc;
+  if (refine) {
+    if (int rc = launch_resid(ng, rk, d_theta, ng, m.Pe, m.Zt, ldz, st)) return rc;
+    if (int rc = launch_theta_lp(mode, ng, rk, ldz, m.Wf, m.Zt, d_theta, ng, true, tv, st))
+      return rc;
+  }
+  if (int rc = ev.rec(10, st)) return rc;
   return 0;
 }
 
@@ -249,21 +280,24 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   // Mixed forms: packed psi rows, gathered Psi(mu) rows, Zt, W (FP64), Wf, and
   // the copy of C C^T + eps I the correction step multiplies by.
   size_t off_A1 = 0, off_A2 = 0, off_B1 = 0, off_B2 = 0, off_Zt = 0, off_W = 0, off_Wf = 0,
-         off_Pe = 0;
+         off_Pe = 0, off_Pz = 0;
   if (mixed) {
+    const size_t eb = (size_t)lp_operand_bytes(mode);
     off_A1 = off;
-    off += align256(sizeof(float) * (size_t)ng * kp1);
+    off += align256(eb * (size_t)ng * kp1);
     off_A2 = off;
-    off += align256(sizeof(float) * (size_t)ng * kp2);
+    off += align256(eb * (size_t)ng * kp2);
     off_B1 = off;
-    off += align256(sizeof(float) * (size_t)rk * kp1);
+    off += align256(eb * (size_t)rk * kp1);
     off_B2 = off;
-    off += align256(sizeof(float) * (size_t)rk * kp2);
+    off += align256(eb * (size_t)rk * kp2);
     off_Zt = off;
     off += align256(sizeof(float) * (size_t)ng * ldz);
     off_W = off;
     off += align256(sizeof(double) * (size_t)rk * rk);
     off_Wf = off;
+    off += align256(eb * (size_t)rk * ldz);
+    off_Pz = off;
     off += align256(sizeof(float) * (size_t)rk * ldz);
     if (refine) {
       off_Pe = off;
@@ -285,9 +319,33 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   int* info_dev = (int*)(base + off_sc);
   double* eps_dev = (double*)(base + off_sc + 64);
 
+  Aux* aux = nullptr;
+  if (int rc = aux_stream(&aux)) return rc;
+  MixedBufs mb{};
+  if (mixed) {
+    mb.A1 = base + off_A1;
+    mb.A2 = dual ? base + off_A2 : mb.A1;
+    mb.B1 = base + off_B1;
+    mb.B2 = dual ? base + off_B2 : mb.B1;
+    mb.Wf = base + off_Wf;
+    mb.Zt = (float*)(base + off_Zt);
+    mb.Pz = (float*)(base + off_Pz);
+    mb.W = (double*)(base + off_W);
+    mb.Pe = refine ? (double*)(base + off_Pe) : nullptr;
+  }
+
   Events ev;
   if (int rc = ev.create(rep != nullptr)) return rc;
   if (int rc = ev.rec(0, st)) return rc;
+  if (mixed) {
+    // Operand rows of psi on the helper stream, beside weights and K-means.
+    ISDF_HIP_TRY(hipEventRecord(aux->fork, st));
+    ISDF_HIP_TRY(hipStreamWaitEvent(aux->s, aux->fork, 0));
+    if (int rc = launch_pack_psi(mode, d_psi, ld, ng, s1, K1, kp1, mb.A1, aux->s)) return rc;
+    if (dual)
+      if (int rc = launch_pack_psi(mode, d_psi, ld, ng, s2, K2, kp2, mb.A2, aux->s)) return rc;
+    ISDF_HIP_TRY(hipEventRecord(aux->pack, aux->s));
+  }
 
   // piv zeroed first (src/common/isdf.cpp:174-175).
   std::memset(h_piv, 0, sizeof(int) * (size_t)ng);
@@ -351,34 +409,52 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
                                 hipMemcpyDeviceToHost, st));
 
   if (mixed) {
-    if (int rc = basis_mixed(p, d_psi, ld, ng, rk, mode, refine, base, off_A1, off_A2, off_B1,
-                             off_B2, off_Zt, off_W, off_Wf, off_Pe, kp1, kp2, ldz, piv_dev, P,
-                             Dinv, Dinv128, Ft, Gt, info_dev, eps_dev, d_theta, dbg, ev, st))
+    // Psi(mu) operand rows (main stream), then the two branches.
+    if (int rc = launch_gather_rows(mode, d_psi, ld, piv_dev, rk, s1, K1, kp1, mb.B1, st))
+      return rc;
+    if (dual)
+      if (int rc = launch_gather_rows(mode, d_psi, ld, piv_dev, rk, s2, K2, kp2, mb.B2, st))
+        return rc;
+    if (int rc = ev.rec(4, st)) return rc;
+    if (int rc = basis_mixed(p, ng, rk, mode, refine, mb, kp1, kp2, ldz, P, Dinv, Dinv128, Ft,
+                             Gt, info_dev, eps_dev, d_theta, dbg, ev, aux, st))
       return rc;
   } else {
-  // 3. Psi(mu) gather over the caller's rk (getbasis_scalapack.cpp:249-269).
-  if (int rc = launch_gather_psimu(d_psi, ld, nstate, piv_dev, rk, psimuT, st)) return rc;
-  if (int rc = ev.rec(4, st)) return rc;
-  // 4. Z C^T into d_theta (getbasis_scalapack.cpp:297-369).
-  if (int rc = launch_zct(d_psi, ld, ng, psimuT, rk, s1, K1, s2, K2, dual, d_theta, ng,
-                          env_variant("ISDF_ZCT_TILE"), st))
-    return rc;
-  if (int rc = ev.rec(5, st)) return rc;
-  // 5. C C^T + eps (getbasis_scalapack.cpp:390-428).
-  if (int rc = launch_cct(d_theta, ng, piv_dev, rk, P, eps_dev, st)) return rc;
-  if (dbg && dbg->d_cct)
-    ISDF_HIP_TRY(hipMemcpyAsync(dbg->d_cct, P, sizeof(double) * (size_t)rk * rk,
-                                hipMemcpyDeviceToDevice, st));
-  if (int rc = ev.rec(6, st)) return rc;
-  // 6. Cholesky (getbasis_scalapack.cpp:455).
-  ISDF_HIP_TRY(hipMemsetAsync(info_dev, 0, sizeof(int), st));
-  if (int rc = launch_potrf(P, rk, Dinv, info_dev, st)) return rc;
-  if (int rc = ev.rec(7, st)) return rc;
-  // 7. Two right-side triangular solves in place (getbasis_scalapack.cpp:469-478).
-  if (int rc = launch_make_fg(P, rk, Dinv, Dinv128, Ft, Gt, st)) return rc;
-  if (int rc = ev.rec(8, st)) return rc;
-  if (int rc = launch_potrs(d_theta, ng64, ng64, rk, Ft, Gt, env_variant("ISDF_POTRS_TILE"), st)) return rc;
-  if (int rc = ev.rec(9, st)) return rc;
+    // 3. Psi(mu) gather over the caller's rk (getbasis_scalapack.cpp:249-269).
+    if (int rc = launch_gather_psimu(d_psi, ld, nstate, piv_dev, rk, psimuT, st)) return rc;
+    if (int rc = ev.rec(4, st)) return rc;
+    // Helper stream: C C^T + eps (getbasis_scalapack.cpp:390-428) as the Z C^T
+    // contraction of the pivot rows, P(mu, nu) = Z C^T(piv[mu], nu), then the
+    // Cholesky (:455) and the blocked-solve operands.
+    hipStream_t s2s = aux->s;
+    ISDF_HIP_TRY(hipEventRecord(aux->fork, st));
+    ISDF_HIP_TRY(hipStreamWaitEvent(s2s, aux->fork, 0));
+    if (int rc = ev.rec(6, s2s)) return rc;
+    // Same tile as the Ng-row Z C^T below, so P equals its pivot rows.
+    const int zv = zct_pick_variant(ng64, env_variant("ISDF_ZCT_TILE"));
+    if (int rc = launch_zct(psimuT, rk, rk, psimuT, rk, s1, K1, s2, K2, dual, P, rk, zv, s2s))
+      return rc;
+    if (int rc = launch_trace_eps(P, rk, eps_dev, s2s)) return rc;
+    if (dbg && dbg->d_cct)
+      ISDF_HIP_TRY(hipMemcpyAsync(dbg->d_cct, P, sizeof(double) * (size_t)rk * rk,
+                                  hipMemcpyDeviceToDevice, s2s));
+    if (int rc = ev.rec(7, s2s)) return rc;
+    ISDF_HIP_TRY(hipMemsetAsync(info_dev, 0, sizeof(int), s2s));
+    if (int rc = launch_potrf(P, rk, Dinv, info_dev, s2s)) return rc;
+    if (int rc = launch_make_fg(P, rk, Dinv, Dinv128, Ft, Gt, s2s)) return rc;
+    if (int rc = ev.rec(8, s2s)) return rc;
+    ISDF_HIP_TRY(hipEventRecord(aux->join, s2s));
+    // Main stream: Z C^T into d_theta (getbasis_scalapack.cpp:297-369).
+    if (int rc = launch_zct(d_psi, ld, ng, psimuT, rk, s1, K1, s2, K2, dual, d_theta, ng, zv, st))
+      return rc;
+    if (int rc = ev.rec(5, st)) return rc;
+    ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->join, 0));
+    if (int rc = ev.rec(9, st)) return rc;
+    // Two right-side triangular solves in place (getbasis_scalapack.cpp:469-478).
+    if (int rc = launch_potrs(d_theta, ng64, ng64, rk, Ft, Gt, env_variant("ISDF_POTRS_TILE"),
+                              st))
+      return rc;
+    if (int rc = ev.rec(10, st)) return rc;
   }
 
   int info = 0;
@@ -399,10 +475,10 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
     rep->ms_select = ev.ms(2, 3);
     rep->ms_gather = ev.ms(3, 4);
     rep->ms_zct = ev.ms(4, 5);
-    rep->ms_cct = ev.ms(5, 6);
-    rep->ms_potrf = ev.ms(6, 8);
-    rep->ms_potrs = ev.ms(8, 9);
-    rep->ms_total = ev.ms(0, 9);
+    rep->ms_cct = ev.ms(6, 7);
+    rep->ms_potrf = ev.ms(7, 8);
+    rep->ms_potrs = ev.ms(9, 10);
+    rep->ms_total = ev.ms(0, 10);
   }
   if (info != 0) return set_error(ISDF_ENOTSPD, "Matrix is not positive definite.");
   return ISDF_OK;
@@ -424,7 +500,18 @@ int isdf_hip_solve(const isdf_problem* prob, const double* d_psi, double* d_thet
                    void* stream, isdf_report* rep, const isdf_debug* dbg) {
   std::lock_guard<std::mutex> lk(isdf_hip::g_mutex);
   isdf_hip::set_error(0, "");
-  return isdf_hip::solve_locked(prob, d_psi, d_theta, h_piv, (hipStream_t)stream, rep, dbg);
+  const int rc =
+      isdf_hip::solve_locked(prob, d_psi, d_theta, h_piv, (hipStream_t)stream, rep, dbg);
+  if (rc != ISDF_OK) {
+    // A failure may leave work queued on either stream; drain both before the
+    // workspace is reused.
+    (void)hipStreamSynchronize((hipStream_t)stream);
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess && (int)isdf_hip::g_aux.size() > dev &&
+        isdf_hip::g_aux[dev].s)
+      (void)hipStreamSynchronize(isdf_hip::g_aux[dev].s);
+  }
+  return rc;
 }
 
 int isdf_hip_rank(const isdf_problem* prob) {
