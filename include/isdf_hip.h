@@ -126,10 +126,36 @@ typedef struct isdf_check {
 int isdf_hip_check(const isdf_problem* prob, const double* h_psi, const double* h_theta,
                    const int* h_piv, int do_coulomb, isdf_check* out);
 
+/* Report of a batched solve. */
+typedef struct isdf_batch_report {
+  double ms_total;        /* the whole batch, HIP events on the call's stream */
+  int graph_replayed;     /* 1: the call replayed the captured hipGraph of this batch shape */
+  int zct_tile;           /* tile variants of the grouped Z C^T and Theta launches */
+  int theta_tile;
+  int reserved;
+  double zct_tile_ms[4];  /* autotuner timings of each variant (first call of a shape) */
+  double theta_tile_ms[4];
+} isdf_batch_report;
+
+/* Many independent ISDF instances in one call (no counterpart in LibISDF,
+ * whose ISDF() takes one instance): probs[i], d_psi[i], d_theta[i], h_piv[i]
+ * as in isdf_hip_solve.  Mixed forms only (ISDF_PREC_FP32 or ISDF_PREC_BF16X3,
+ * the same for every instance, equal pair sets).  K-means runs per instance on
+ * concurrent branches; the contractions of all instances run as persistent
+ * grouped launches and the Cholesky chains as batched launches.  The first
+ * call of a batch shape (problems and buffer addresses) tunes the tile
+ * variants and captures the schedule in a hipGraph; later calls with the same
+ * shape replay it.  Synchronous on return.  reps (nbatch entries) and brep may
+ * be NULL. */
+int isdf_hip_solve_batched(int nbatch, const isdf_problem* probs, const double* const* d_psi,
+                           double* const* d_theta, int* const* h_piv, void* stream,
+                           isdf_report* reps, isdf_batch_report* brep);
+
 /* Library version string. */
 const char* isdf_hip_version(void);
 
-/* Releases the cached device workspace of the calling device. */
+/* Releases the cached device workspace of the calling device and the batch
+ * plan (buffers and captured graph). */
 int isdf_hip_release_workspace(void);
 
 #ifdef __cplusplus

@@ -18,7 +18,7 @@ from . import _lib
 from ._lib import (ISDF_EHIP, ISDF_EINVAL, ISDF_ENOMEM, ISDF_ENOTSPD, ISDF_OK,
                    ISDF_PREC_BF16X3, ISDF_PREC_FP32, ISDF_PREC_FP64, ISDF_PREC_REFINE)
 
-__all__ = ["Domain", "ISDF", "isdf_rank", "solve", "check", "ISDF_PREC_FP64", "ISDF_PREC_FP32",
+__all__ = ["Domain", "ISDF", "isdf_rank", "solve", "solve_batched", "check", "ISDF_PREC_FP64", "ISDF_PREC_FP32",
            "ISDF_PREC_BF16X3", "ISDF_PREC_REFINE"]
 
 
@@ -126,6 +126,38 @@ def solve(d_psi, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_theta, p
     if rc != ISDF_OK:
         _raise(rc)
     return rep.as_dict()
+
+
+def solve_batched(instances, precision=ISDF_PREC_FP32, stream=None):
+    """Many independent instances in one call (isdf_hip_solve_batched; no
+    counterpart in the reference).  instances: list of dicts with keys d_psi,
+    domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_theta, piv,
+    max_iter (as in solve()).  Returns (list of per-instance reports, batch
+    report)."""
+    L = _lib.lib()
+    n = len(instances)
+    probs = (_lib.IsdfProblem * n)()
+    psis = (ctypes.c_void_p * n)()
+    thetas = (ctypes.c_void_p * n)()
+    pivs = (ctypes.POINTER(ctypes.c_int) * n)()
+    for i, a in enumerate(instances):
+        probs[i] = _problem(a["domain"], a["nocc"], a["nstate"], a["nv1"], a["nc1"], a["nv2"],
+                            a["nc2"], a["mu_points"], a["max_iter"], precision)
+        piv = a["piv"]
+        ng = a["domain"].NumGridTotal()
+        if not (isinstance(piv, np.ndarray) and piv.dtype == np.int32
+                and piv.flags.c_contiguous and piv.size >= ng):
+            raise ValueError("piv must be a contiguous numpy int32 array of Ng entries")
+        psis[i] = _dev_ptr(a["d_psi"])
+        thetas[i] = _dev_ptr(a["d_theta"])
+        pivs[i] = piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    reps = (_lib.IsdfReport * n)()
+    brep = _lib.IsdfBatchReport()
+    rc = L.isdf_hip_solve_batched(n, probs, psis, thetas, pivs,
+                                  ctypes.c_void_p(_stream_ptr(stream)), reps, ctypes.byref(brep))
+    if rc != ISDF_OK:
+        _raise(rc)
+    return [r.as_dict() for r in reps], brep.as_dict()
 
 
 def check(psi, theta, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, coulomb=True):

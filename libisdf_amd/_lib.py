@@ -20,6 +20,7 @@ EXPORTS = (
     "isdf_hip_version",
     "isdf_hip_release_workspace",
     "isdf_hip_check",
+    "isdf_hip_solve_batched",
 )
 
 
@@ -74,6 +75,23 @@ class IsdfDebug(ctypes.Structure):
     ]
 
 
+class IsdfBatchReport(ctypes.Structure):
+    _fields_ = [
+        ("ms_total", ctypes.c_double),
+        ("graph_replayed", ctypes.c_int),
+        ("zct_tile", ctypes.c_int),
+        ("theta_tile", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+        ("zct_tile_ms", ctypes.c_double * 4),
+        ("theta_tile_ms", ctypes.c_double * 4),
+    ]
+
+    def as_dict(self):
+        return dict(ms_total=self.ms_total, graph_replayed=self.graph_replayed,
+                    zct_tile=self.zct_tile, theta_tile=self.theta_tile,
+                    zct_tile_ms=list(self.zct_tile_ms), theta_tile_ms=list(self.theta_tile_ms))
+
+
 class IsdfCheck(ctypes.Structure):
     _fields_ = [
         ("fit_error", ctypes.c_double),
@@ -111,6 +129,11 @@ def lib():
     L.isdf_hip_check.argtypes = [ctypes.POINTER(IsdfProblem), ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
                                  ctypes.c_int, ctypes.POINTER(IsdfCheck)]
+    L.isdf_hip_solve_batched.restype = ctypes.c_int
+    L.isdf_hip_solve_batched.argtypes = [
+        ctypes.c_int, ctypes.POINTER(IsdfProblem), ctypes.POINTER(ctypes.c_void_p),
+        ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.POINTER(ctypes.c_int)),
+        ctypes.c_void_p, ctypes.POINTER(IsdfReport), ctypes.POINTER(IsdfBatchReport)]
     _lib = L
     return L
 

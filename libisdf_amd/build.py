@@ -35,6 +35,7 @@ SOURCES = {
     "basis.hip": [],
     "lowprec.hip": [],
     "solve.cpp": ["-x", "hip"],
+    "batch.cpp": ["-x", "hip"],
     "check.cpp": [],
     "dropin.cpp": ["-x", "hip"],
 }

@@ -99,9 +99,16 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct(int ng, int rk, const doubl
 // trace summed in ascending order, eps = 1e-10 * trace / rk added to the
 // diagonal (getbasis_scalapack.cpp:398-428).  One block.
 __global__ __launch_bounds__(256) void k_trace_eps(double* __restrict__ P, int rk,
-                                                   double* __restrict__ eps_out) {
+                                                   double* __restrict__ eps_out,
+                                                   const CholJob* jobs) {
   __shared__ double eps_s;
   __shared__ double diag[8192];
+  if (jobs) {
+    const CholJob& J = jobs[blockIdx.z];
+    P = J.P;
+    rk = J.n;
+    eps_out = J.eps;
+  }
   for (int nu = threadIdx.x; nu < rk; nu += blockDim.x) diag[nu] = P[(long long)nu * rk + nu];
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -126,10 +133,18 @@ __global__ __launch_bounds__(256) void k_trace_eps(double* __restrict__ P, int r
 // of each step and the finished factor pass through LDS as broadcasts.
 __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int n, int J0,
                                                    double* __restrict__ Dinv,
-                                                   int* __restrict__ info) {
+                                                   int* __restrict__ info, const CholJob* jobs) {
   __shared__ double colj[kNB];
   __shared__ double Lrow[kNB][kNB + 1];  // Lrow[i][t] = L[i][t]
   __shared__ int fail_s;
+  if (jobs) {
+    const CholJob& J = jobs[blockIdx.z];
+    A = J.P;
+    n = J.n;
+    Dinv = J.Dinv;
+    info = J.info;
+    if (J0 >= n) return;
+  }
   if (*info != 0) return;
   const int lane = threadIdx.x;
   const int jb = min(kNB, n - J0);
@@ -204,11 +219,21 @@ typedef MfmaTile<128, 128, 16, 2, 2> SyrkTile;
 // GEMM has read them.
 __global__ __launch_bounds__(256) void k_potrf_panel(double* __restrict__ A, int n, int J0,
                                                      const double* __restrict__ Dinv,
-                                                     const int* __restrict__ info) {
+                                                     const int* __restrict__ info,
+                                                     const CholJob* jobs) {
   __shared__ double lds[PanelTile::LDS_DOUBLES];
+  if (jobs) {
+    const CholJob& J = jobs[blockIdx.z];
+    A = J.P;
+    n = J.n;
+    Dinv = J.Dinv;
+    info = J.info;
+    if (J0 >= n) return;
+  }
   if (*info != 0) return;
   const int jb = min(kNB, n - J0);
   const int I0 = J0 + jb + blockIdx.x * PanelTile::BM;
+  if (I0 >= n) return;
   const int mlim = min(PanelTile::BM, n - I0);
   double* Ai = A + (long long)J0 * n + I0;
   const double* D = Dinv + (long long)(J0 / kNB) * kNB * kNB;
@@ -232,14 +257,23 @@ __global__ __launch_bounds__(256) void k_potrf_panel(double* __restrict__ A, int
 
 // Trailing update A_IK -= L_IJ L_KJ^T on the lower 128 x 128 tiles (MFMA).
 __global__ __launch_bounds__(256) void k_potrf_syrk(double* __restrict__ A, int n, int J0,
-                                                    const int* __restrict__ info) {
+                                                    const int* __restrict__ info,
+                                                    const CholJob* jobs) {
   __shared__ double lds[SyrkTile::LDS_DOUBLES];
+  if (jobs) {
+    const CholJob& J = jobs[blockIdx.z];
+    A = J.P;
+    n = J.n;
+    info = J.info;
+    if (J0 >= n) return;
+  }
   if (*info != 0) return;
   const int ti = blockIdx.x, tk = blockIdx.y;
   if (tk > ti) return;
   const int jb = min(kNB, n - J0);
   const int T0 = J0 + jb;
   const int I0 = T0 + ti * SyrkTile::BM, K0 = T0 + tk * SyrkTile::BN;
+  if (I0 >= n) return;
   const int mlim = min(SyrkTile::BM, n - I0), nlim = min(SyrkTile::BN, n - K0);
   typename SyrkTile::Acc acc;
   SyrkTile::zero(acc);
@@ -277,11 +311,20 @@ constexpr int kSB = 128;
 // Dinv128[J] = inv(L_JJ) for the 128-blocks: [[Da, 0], [-Db L_ba Da, Db]].
 __global__ __launch_bounds__(256) void k_dinv128(const double* __restrict__ L, int n,
                                                  const double* __restrict__ Dinv64,
-                                                 double* __restrict__ Dinv128) {
+                                                 double* __restrict__ Dinv128,
+                                                 const CholJob* jobs) {
   __shared__ double da[kNB][kNB];   // da[col][row]
   __shared__ double db[kNB][kNB];
   __shared__ double lt[kNB][kNB];   // L_ba Da, [col][row]
   const int J = blockIdx.x, J0 = J * kSB;
+  if (jobs) {
+    const CholJob& jb = jobs[blockIdx.z];
+    L = jb.P;
+    n = jb.n;
+    Dinv64 = jb.Dinv;
+    Dinv128 = jb.Dinv128;
+    if (J0 >= n) return;
+  }
   const int t = threadIdx.x;
   const double* Da = Dinv64 + (long long)(2 * J) * kNB * kNB;
   const double* Db = Dinv64 + (long long)(2 * J + 1) * kNB * kNB;
@@ -331,10 +374,20 @@ typedef MfmaTile<128, 128, 16, 2, 2, false> FgTile;
 __global__ __launch_bounds__(256, 1) void k_make_fg(const double* __restrict__ L, int n,
                                                     const double* __restrict__ Dinv128,
                                                     double* __restrict__ Ft,
-                                                    double* __restrict__ Gt) {
+                                                    double* __restrict__ Gt,
+                                                    const CholJob* jobs) {
   __shared__ double lds[FgTile::LDS_DOUBLES];
   const int P = blockIdx.x, Q = blockIdx.y;
   if (Q > P) return;
+  if (jobs) {
+    const CholJob& J = jobs[blockIdx.z];
+    L = J.P;
+    n = J.n;
+    Dinv128 = J.Dinv128;
+    Ft = J.Ft;
+    Gt = J.Gt;
+    if (P * kSB >= n) return;
+  }
   const int P0 = P * kSB, Q0 = Q * kSB;
   const int pb = min(kSB, n - P0), qb = min(kSB, n - Q0);
   const double* DP = Dinv128 + (long long)P * kSB * kSB;
@@ -390,11 +443,20 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_potrs(double* __restrict__ T, l
                                                        long long ldt, int n,
                                                        const double* __restrict__ Ft,
                                                        const double* __restrict__ Gt,
-                                                       int unit_lower) {
+                                                       int unit_lower, const CholJob* jobs) {
   static_assert(Tile::BN == kSB, "block column width");
   static_assert(kSB % Tile::BM == 0 || Tile::BM % kSB == 0, "strip / block alignment");
   __shared__ double lds[Tile::LDS_DOUBLES];
   const long long m0 = (long long)blockIdx.x * Tile::BM;
+  if (jobs) {  // T = W of the instance, which holds the identity
+    const CholJob& J = jobs[blockIdx.z];
+    T = J.W;
+    n = J.n;
+    ng = ldt = J.n;
+    Ft = J.Ft;
+    Gt = J.Gt;
+    if (m0 >= ng) return;
+  }
   const int mlim = (int)std::min<long long>(Tile::BM, ng - m0);
   double* Tm = T + m0;
   const int nJ = (n + kSB - 1) / kSB;
@@ -470,7 +532,8 @@ int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, in
 }
 
 int launch_trace_eps(double* P, int rk, double* eps_dev, hipStream_t st) {
-  hipLaunchKernelGGL(k_trace_eps, dim3(1), dim3(256), 0, st, P, rk, eps_dev);
+  hipLaunchKernelGGL(k_trace_eps, dim3(1), dim3(256), 0, st, P, rk, eps_dev,
+                     (const CholJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_trace_eps");
   return 0;
 }
@@ -480,15 +543,17 @@ int launch_potrf(double* P, int rk, double* Dinv, int* info_dev, hipStream_t st)
   for (int J = 0; J < nJ; J++) {
     const int J0 = J * kNB;
     const int jb = std::min(kNB, rk - J0);
-    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, st, P, rk, J0, Dinv, info_dev);
+    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, st, P, rk, J0, Dinv, info_dev,
+                       (const CholJob*)nullptr);
     ISDF_HIP_LAUNCHED("k_potrf_diag");
     const int rest = rk - J0 - jb;
     if (rest > 0) {
       hipLaunchKernelGGL(k_potrf_panel, dim3(cdiv(rest, PanelTile::BM)), dim3(PanelTile::NT), 0,
-                         st, P, rk, J0, Dinv, info_dev);
+                         st, P, rk, J0, Dinv, info_dev, (const CholJob*)nullptr);
       ISDF_HIP_LAUNCHED("k_potrf_panel");
       const unsigned t = cdiv(rest, SyrkTile::BM);
-      hipLaunchKernelGGL(k_potrf_syrk, dim3(t, t), dim3(SyrkTile::NT), 0, st, P, rk, J0, info_dev);
+      hipLaunchKernelGGL(k_potrf_syrk, dim3(t, t), dim3(SyrkTile::NT), 0, st, P, rk, J0, info_dev,
+                         (const CholJob*)nullptr);
       ISDF_HIP_LAUNCHED("k_potrf_syrk");
     }
   }
@@ -498,9 +563,11 @@ int launch_potrf(double* P, int rk, double* Dinv, int* info_dev, hipStream_t st)
 int launch_make_fg(const double* L, int rk, const double* Dinv, double* Dinv128, double* Ft,
                    double* Gt, hipStream_t st) {
   const unsigned nJ = cdiv(rk, kSB);
-  hipLaunchKernelGGL(k_dinv128, dim3(nJ), dim3(256), 0, st, L, rk, Dinv, Dinv128);
+  hipLaunchKernelGGL(k_dinv128, dim3(nJ), dim3(256), 0, st, L, rk, Dinv, Dinv128,
+                     (const CholJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_dinv128");
-  hipLaunchKernelGGL(k_make_fg, dim3(nJ, nJ), dim3(256), 0, st, L, rk, Dinv128, Ft, Gt);
+  hipLaunchKernelGGL(k_make_fg, dim3(nJ, nJ), dim3(256), 0, st, L, rk, Dinv128, Ft, Gt,
+                     (const CholJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_make_fg");
   return 0;
 }
@@ -515,7 +582,8 @@ int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* F
 #define ISDF_POTRS_CASE(V)                                                                      \
   case V:                                                                                       \
     hipLaunchKernelGGL((k_potrs<PotrsTile##V>), dim3(cdiv(ng, PotrsTile##V::BM)),               \
-                       dim3(PotrsTile##V::NT), 0, st, T, ng, ldt, rk, Ft, Gt, (int)unit_lower); \
+                       dim3(PotrsTile##V::NT), 0, st, T, ng, ldt, rk, Ft, Gt, (int)unit_lower,  \
+                       (const CholJob*)nullptr);                                                \
     break;
     ISDF_POTRS_CASE(0)
     ISDF_POTRS_CASE(1)
@@ -528,6 +596,55 @@ int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* F
       return set_error(1, "unknown solve tile variant");
   }
   ISDF_HIP_LAUNCHED("k_potrs");
+  return 0;
+}
+
+int launch_trace_eps_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st) {
+  (void)max_n;
+  hipLaunchKernelGGL(k_trace_eps, dim3(1, 1, nb), dim3(256), 0, st, (double*)nullptr, 0,
+                     (double*)nullptr, jobs);
+  ISDF_HIP_LAUNCHED("k_trace_eps (batched)");
+  return 0;
+}
+
+int launch_potrf_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st) {
+  const int nJ = (max_n + kNB - 1) / kNB;
+  for (int J = 0; J < nJ; J++) {
+    const int J0 = J * kNB;
+    hipLaunchKernelGGL(k_potrf_diag, dim3(1, 1, nb), dim3(64), 0, st, (double*)nullptr, 0, J0,
+                       (double*)nullptr, (int*)nullptr, jobs);
+    ISDF_HIP_LAUNCHED("k_potrf_diag (batched)");
+    const int rest = max_n - J0 - kNB;
+    if (rest > 0) {
+      hipLaunchKernelGGL(k_potrf_panel, dim3(cdiv(rest, PanelTile::BM), 1, nb),
+                         dim3(PanelTile::NT), 0, st, (double*)nullptr, 0, J0,
+                         (const double*)nullptr, (const int*)nullptr, jobs);
+      ISDF_HIP_LAUNCHED("k_potrf_panel (batched)");
+      const unsigned t = cdiv(rest, SyrkTile::BM);
+      hipLaunchKernelGGL(k_potrf_syrk, dim3(t, t, nb), dim3(SyrkTile::NT), 0, st,
+                         (double*)nullptr, 0, J0, (const int*)nullptr, jobs);
+      ISDF_HIP_LAUNCHED("k_potrf_syrk (batched)");
+    }
+  }
+  return 0;
+}
+
+int launch_make_fg_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st) {
+  const unsigned nJ = cdiv(max_n, kSB);
+  hipLaunchKernelGGL(k_dinv128, dim3(nJ, 1, nb), dim3(256), 0, st, (const double*)nullptr, 0,
+                     (const double*)nullptr, (double*)nullptr, jobs);
+  ISDF_HIP_LAUNCHED("k_dinv128 (batched)");
+  hipLaunchKernelGGL(k_make_fg, dim3(nJ, nJ, nb), dim3(256), 0, st, (const double*)nullptr, 0,
+                     (const double*)nullptr, (double*)nullptr, (double*)nullptr, jobs);
+  ISDF_HIP_LAUNCHED("k_make_fg (batched)");
+  return 0;
+}
+
+int launch_potrs_identity_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st) {
+  hipLaunchKernelGGL((k_potrs<PotrsTile5>), dim3(cdiv(max_n, PotrsTile5::BM), 1, nb),
+                     dim3(PotrsTile5::NT), 0, st, (double*)nullptr, 0LL, 0LL, 0,
+                     (const double*)nullptr, (const double*)nullptr, 1, jobs);
+  ISDF_HIP_LAUNCHED("k_potrs (batched identity)");
   return 0;
 }
 

@@ -84,17 +84,21 @@ void kmeans_carve(KmeansBufs* b, void* base, int n, const int ngrid[3], int rkma
 int launch_weights(const double* psi, long long ld, int ng, int s1lo, int s1hi, int s2lo,
                    int s2hi, double* w, hipStream_t st);
 
-// Threshold, point setup, initial centroids.  Leaves scal[KS_NPT] on device and
-// returns it through *npt_host (one synchronisation).
+// Threshold, point setup, initial centroids.  Returns npt through *npt_host
+// (one synchronisation); with npt_host NULL nothing is read back (npt stays in
+// bcount[nblk_flag] on the device) and the caller assumes npt == Ng.
 int kmeans_prepare(KmeansBufs* b, const int ngrid[3], const double length[3], double df_tol,
                    int* npt_host, hipStream_t st);
 int kmeans_setup_points(KmeansBufs* b, const int ngrid[3], const double length[3], int npt,
                         int rk, hipStream_t st);
-// The Lloyd loop; returns the number of iterations through *iters_host.
+// The Lloyd loop; returns the number of iterations through *iters_host.  With
+// iters_host NULL all max_iter iterations are queued without host reads (the
+// kernels stop working once converged; the count stays in scal[KS_ITERS]).
 int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* iters_host,
                    hipStream_t st);
 // Final nearest points, unique(), and the full pivot list piv_dev[n].
-int kmeans_finish(KmeansBufs* b, int* piv_dev, hipStream_t st);
+// nosync: no host read of the distinct-center count.
+int kmeans_finish(KmeansBufs* b, int* piv_dev, hipStream_t st, bool nosync = false);
 
 // ---------------------------------------------------------------------------
 // Basis construction (basis.hip)
@@ -122,6 +126,68 @@ int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* F
 int launch_trace_eps(double* P, int rk, double* eps_dev, hipStream_t st);
 
 constexpr int kNB = 64;  // block size of the Cholesky (the solves use 128-blocks)
+
+// One instance of a batched Cholesky / solve-operand chain (device array; the
+// batched launches take blockIdx.z as the instance).  Sizes of the buffers as
+// the single solve carves them for rk = n.
+struct CholJob {
+  double* P;        // n x n: C C^T, + eps I, then L in place (lower)
+  double* Dinv;     // inverses of the 64-blocks of L
+  double* Dinv128;  // inverses of the 128-blocks
+  double* Ft;       // forward-solve operand (n x n)
+  double* Gt;       // backward-solve operand (n x n)
+  double* W;        // (L L^T)^{-1}
+  void* Wf;         // operand rows of W in the mixed format (n x ldz)
+  float* Pz;        // pivot-row contraction, n x ldz FP32
+  int* info;        // Cholesky status
+  double* eps;      // the diagonal shift
+  int n, ldz;
+};
+
+// One job of the grouped low-precision Z C^T launch: C[m][c] = P1(m,c)^2 for
+// c < n (c < ldc written, zero past n), P1 = A (m rows) B^T (n rows), operand
+// rows of kp (padded) k values in the mode's format, row stride lda elements.
+struct LpGemmJob {
+  const void* A;
+  const void* B;
+  float* C;
+  long long lda;
+  int kp, m, n, ldc;
+  int tile0;  // first tile of the job in the grouped tile list (a multiple of 8)
+};
+// One job of the grouped Theta launch: theta(g, nu) = sum_k Wf[nu][k] Zt[g][k].
+struct LpThetaJob {
+  const void* Wf;
+  long long ldwr;  // row stride of Wf (elements of its format)
+  const float* Zt;
+  double* theta;
+  long long ldt;
+  int ldk, ng, rk;
+  int tile0;
+};
+// Tiles of one job (before rounding the prefix to a multiple of 8).
+long long lp_zct_job_tiles(int mode, int variant, int m, int ldc);
+long long lp_theta_job_tiles(int mode, int variant, int ng, int rk);
+// Persistent grouped launches over the jobs' tile lists; `grid` workgroups
+// stride over `total` tiles.
+int launch_zct_lp_grouped(int mode, int variant, const LpGemmJob* jobs, int njobs, long long total,
+                          int grid, hipStream_t st);
+int launch_theta_lp_grouped(int mode, int variant, const LpThetaJob* jobs, int njobs,
+                            long long total, int grid, hipStream_t st);
+// Batched helpers of the mixed chain: P from Pz (+ eps), W = I, Wf from W,
+// info = 0.
+int launch_cct_lp_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st);
+int launch_inverse_batched(int mode, const CholJob* jobs, int nb, int max_n, int max_ldz,
+                           hipStream_t st);
+int launch_clear_info_batched(const CholJob* jobs, int nb, hipStream_t st);
+
+// Batched forms (blockIdx.z = instance): the same kernels as the single
+// solve, sized for the largest instance; blocks past an instance's size exit.
+int launch_trace_eps_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st);
+int launch_potrf_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st);
+int launch_make_fg_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st);
+// W = I (L L^T)^{-1} per instance.
+int launch_potrs_identity_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Mixed-precision contractions (lowprec.hip).  mode 1 = FP32 MFMA, mode 2 =

@@ -175,9 +175,12 @@ __global__ __launch_bounds__(1024) void k_scan_single(int* a, int m) {
   if (threadIdx.x == 0) a[m] = carry;
 }
 
+// base_dev, when given, replaces base (an offset still on the device).
 __global__ __launch_bounds__(256) void k_flag_scatter(const uint8_t* __restrict__ flags, int n,
                                                       int want, const int* __restrict__ bexcl,
-                                                      int base, int* __restrict__ out) {
+                                                      int base, int* __restrict__ out,
+                                                      const int* base_dev) {
+  if (base_dev) base = *base_dev;
   long long i0 = (long long)blockIdx.x * kFlagTile + threadIdx.x * 8;
   int c = 0;
   for (int k = 0; k < 8; k++) {
@@ -917,7 +920,8 @@ inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) /
 
 // Ordered compaction of the indices whose flag equals `want` into out[base...].
 // Returns the count through the device slot `count_dev` (bcount[nblk]).
-int compact_flags(KmeansBufs* b, int want, int base, int* out, hipStream_t st) {
+int compact_flags(KmeansBufs* b, int want, int base, int* out, hipStream_t st,
+                  const int* base_dev = nullptr) {
   const int nblk = b->nblk_flag;
   hipLaunchKernelGGL(k_flag_count, dim3(nblk), dim3(256), 0, st, b->flags, b->n, want, b->bcount);
   ISDF_HIP_LAUNCHED("k_flag_count");
@@ -925,7 +929,7 @@ int compact_flags(KmeansBufs* b, int want, int base, int* out, hipStream_t st) {
   ISDF_HIP_LAUNCHED("k_scan_single");
   if (out) {
     hipLaunchKernelGGL(k_flag_scatter, dim3(nblk), dim3(256), 0, st, b->flags, b->n, want,
-                       b->bcount, base, out);
+                       b->bcount, base, out, base_dev);
     ISDF_HIP_LAUNCHED("k_flag_scatter");
   }
   return 0;
@@ -1025,6 +1029,9 @@ int kmeans_prepare(KmeansBufs* b, const int ngrid[3], const double length[3], do
   }
   ISDF_HIP_LAUNCHED("k_thresh");
   if (int rc = compact_flags(b, 1, 0, nullptr, st)) return rc;
+  // Without npt_host the caller assumes every grid point is a K-means point and
+  // checks npt (left in bcount[nblk_flag]) afterwards.
+  if (!npt_host) return 0;
   int npt = 0;
   ISDF_HIP_TRY(hipMemcpyAsync(&npt, b->bcount + b->nblk_flag, sizeof(int), hipMemcpyDeviceToHost,
                               st));
@@ -1032,7 +1039,7 @@ int kmeans_prepare(KmeansBufs* b, const int ngrid[3], const double length[3], do
   *npt_host = npt;
   if (npt < n) {
     hipLaunchKernelGGL(k_flag_scatter, dim3(b->nblk_flag), dim3(256), 0, st, b->flags, n, 1,
-                       b->bcount, 0, b->index);
+                       b->bcount, 0, b->index, (const int*)nullptr);
     ISDF_HIP_LAUNCHED("k_flag_scatter");
   }
   return 0;
@@ -1080,9 +1087,13 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
   const int active0 = ((double)n > maxF && 0 < max_iter) ? 1 : 0;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, b->scal + KS_ACTIVE, 1LL, active0);
   ISDF_HIP_LAUNCHED("k_fill_i32");
-  *iters_host = 0;
+  const bool poll = iters_host != nullptr;
+  if (poll) *iters_host = 0;
   if (!active0) return 0;
   const size_t lds_rk = sizeof(int) * (size_t)rk;
+  // Polled: the host reads the active flag every few iterations and stops
+  // queueing; unpolled (graph capture): every iteration is queued and the
+  // kernels return at once after convergence.
   const int check_every = 4;
   for (int s = 0; s < max_iter; s++) {
     if (b->grid_mode) {
@@ -1111,7 +1122,7 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
     hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, st, s, max_iter, maxF, b->nblk_pts,
                        b->changed_blk, b->trace, b->scal);
     ISDF_HIP_LAUNCHED("k-means iteration");
-    if ((s + 1) % check_every == 0 && s + 1 < max_iter) {
+    if (poll && (s + 1) % check_every == 0 && s + 1 < max_iter) {
       int act = 1;
       ISDF_HIP_TRY(hipMemcpyAsync(&act, b->scal + KS_ACTIVE, sizeof(int), hipMemcpyDeviceToHost,
                                   st));
@@ -1119,6 +1130,7 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
       if (!act) break;
     }
   }
+  if (!poll) return 0;
   int it = 0;
   ISDF_HIP_TRY(hipMemcpyAsync(&it, b->scal + KS_ITERS, sizeof(int), hipMemcpyDeviceToHost, st));
   ISDF_HIP_TRY(hipStreamSynchronize(st));
@@ -1126,7 +1138,7 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
   return 0;
 }
 
-int kmeans_finish(KmeansBufs* b, int* piv_dev, hipStream_t st) {
+int kmeans_finish(KmeansBufs* b, int* piv_dev, hipStream_t st, bool nosync) {
   const int n = b->n;
   const int rkmax = b->rkmax;
   if (b->grid_mode) {
@@ -1154,6 +1166,7 @@ int kmeans_finish(KmeansBufs* b, int* piv_dev, hipStream_t st) {
   hipLaunchKernelGGL(k_mark_centers, dim3(cdiv(rkmax, 256)), dim3(256), 0, st, b->centers,
                      b->flags, piv_dev, b->scal);
   ISDF_HIP_LAUNCHED("k_mark_centers");
+  if (nosync) return compact_flags(b, 0, 0, piv_dev, st, b->scal + KS_RKU);
   int rku = 0;
   ISDF_HIP_TRY(hipMemcpyAsync(&rku, b->scal + KS_RKU, sizeof(int), hipMemcpyDeviceToHost, st));
   ISDF_HIP_TRY(hipStreamSynchronize(st));

@@ -450,6 +450,120 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_theta_lp(int ng, int rk, int ld
     }
 }
 
+// Grouped (persistent) forms of the two contractions: `gridDim.x` workgroups
+// walk one tile list that spans every job; tile t goes to workgroup t % grid,
+// and with grid a multiple of 8 and each job's list starting at a multiple of
+// 8, a job's tiles meet the XCDs in the same pattern as in the single launch.
+template <class Tile>
+__global__ __launch_bounds__(Tile::NT, 1) void k_zct_lp_grouped(const LpGemmJob* __restrict__ jobs,
+                                                                int njobs, long long total) {
+  __shared__ __attribute__((aligned(16))) char lds[Tile::LDS_BYTES];
+  constexpr int ES = Tile::A_PRE ? 2 : 4;
+  for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    int j = 0;
+    while (j + 1 < njobs && jobs[j + 1].tile0 <= t) j++;
+    const LpGemmJob J = jobs[j];
+    const int nstrip = (J.m + Tile::BM - 1) / Tile::BM;
+    const int ntile = (J.ldc + Tile::BN - 1) / Tile::BN;
+    const long long local = t - J.tile0;
+    if (local >= (long long)nstrip * ntile) continue;  // padding of the job's list
+    int s, ct;
+    xcd_map((int)local, ntile, nstrip, &s, &ct);
+    const long long m0 = (long long)s * Tile::BM;
+    const int n0 = ct * Tile::BN;
+    const int mlim = (int)std::min<long long>(Tile::BM, J.m - m0);
+    const int nlim = std::min(Tile::BN, J.n - n0);
+    const int wlim = std::min(Tile::BN, J.ldc - n0);
+    typename Tile::Acc acc;
+    Tile::zero(acc);
+    Tile::run((const char*)J.A + m0 * J.lda * ES, J.lda, (const char*)J.B + n0 * J.lda * ES,
+              J.lda, J.kp, mlim, nlim, lds, acc);
+    float* Cm = J.C + m0 * J.ldc + n0;
+#pragma unroll
+    for (int jj = 0; jj < Tile::TN; jj++) {
+      const int n = Tile::col(jj);
+#pragma unroll
+      for (int i = 0; i < Tile::TM; i++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int m = Tile::row(i, r);
+          if (m < mlim && n < wlim) {
+            const float v = acc[i][jj][r];
+            Cm[(long long)m * J.ldc + n] = v * v;
+          }
+        }
+    }
+  }
+}
+
+template <class Tile>
+__global__ __launch_bounds__(Tile::NT, 1) void k_theta_lp_grouped(
+    const LpThetaJob* __restrict__ jobs, int njobs, long long total) {
+  __shared__ __attribute__((aligned(16))) char lds[Tile::LDS_BYTES];
+  constexpr int ESA = Tile::A_PRE ? 2 : 4;
+  for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    int j = 0;
+    while (j + 1 < njobs && jobs[j + 1].tile0 <= t) j++;
+    const LpThetaJob J = jobs[j];
+    const int nstrip = (J.ng + Tile::BN - 1) / Tile::BN;
+    const int ntile = (J.rk + Tile::BM - 1) / Tile::BM;
+    const long long local = t - J.tile0;
+    if (local >= (long long)nstrip * ntile) continue;
+    int s, vt;
+    xcd_map((int)local, ntile, nstrip, &s, &vt);
+    const long long g0 = (long long)s * Tile::BN;
+    const int v0 = vt * Tile::BM;
+    const int mlim = std::min(Tile::BM, J.rk - v0);
+    const int nlim = (int)std::min<long long>(Tile::BN, J.ng - g0);
+    typename Tile::Acc acc;
+    Tile::zero(acc);
+    Tile::run((const char*)J.Wf + (long long)v0 * J.ldwr * ESA, J.ldwr, J.Zt + g0 * J.ldk, J.ldk,
+              J.ldk, mlim, nlim, lds, acc);
+    double* Tt = J.theta + (long long)v0 * J.ldt + g0;
+#pragma unroll
+    for (int i = 0; i < Tile::TM; i++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = Tile::row(i, r);
+#pragma unroll
+        for (int jj = 0; jj < Tile::TN; jj++) {
+          const int n = Tile::col(jj);
+          if (m < mlim && n < nlim) Tt[(long long)m * J.ldt + n] = (double)acc[i][jj][r];
+        }
+      }
+  }
+}
+
+// Batched helpers (blockIdx.z = instance).
+__global__ __launch_bounds__(256) void k_cct_lp_batched(const CholJob* __restrict__ jobs) {
+  const CholJob J = jobs[blockIdx.z];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)J.n * J.n) return;
+  const int mu = (int)(t % J.n), nu = (int)(t / J.n);
+  J.P[t] = (double)J.Pz[(long long)mu * J.ldz + nu];
+}
+
+__global__ __launch_bounds__(256) void k_identity_batched(const CholJob* __restrict__ jobs) {
+  const CholJob J = jobs[blockIdx.z];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)J.n * J.n) return;
+  J.W[t] = (t % J.n == t / J.n) ? 1.0 : 0.0;
+}
+
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_pack_w_batched(const CholJob* __restrict__ jobs) {
+  const CholJob J = jobs[blockIdx.z];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)J.n * J.ldz) return;
+  const int k = (int)(t % J.ldz), nu = (int)(t / J.ldz);
+  put_operand<SPLIT>(J.Wf, J.ldz, nu, k, k < J.n ? (float)J.W[(long long)nu * J.n + k] : 0.f);
+}
+
+__global__ void k_clear_info_batched(const CholJob* __restrict__ jobs, int nb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nb) *jobs[i].info = 0;
+}
+
 // Residual of the FP64 correction step: Zt[g][nu] <- Zt[g][nu] - (Theta Pe)(g, nu)
 // with Theta FP64 (column-major, ldt) and Pe = C C^T + eps I (column-major).
 typedef MfmaTile<256, 128, 16, 4, 2, false> ResidTile;
@@ -613,6 +727,99 @@ int launch_resid(int ng, int rk, const double* theta, long long ldt, const doubl
   hipLaunchKernelGGL(k_resid, dim3(nb), dim3(ResidTile::NT), 0, st, ng, rk, theta, ldt, Pe, Zt,
                      ldz);
   ISDF_HIP_LAUNCHED("k_resid");
+  return 0;
+}
+
+long long lp_zct_job_tiles(int mode, int variant, int m, int ldc) {
+  if (variant < 0) variant = mode == 1 ? 2 : 0;
+  int bm, bn;
+  if (mode == 1) {
+    bm = variant == 1 ? LpF32_1::BM : variant == 2 ? LpF32_2::BM : LpF32_0::BM;
+    bn = variant == 1 ? LpF32_1::BN : variant == 2 ? LpF32_2::BN : LpF32_0::BN;
+  } else {
+    bm = variant == 1 ? LpZB16_1::BM : variant == 2 ? LpZB16_2::BM : LpZB16_0::BM;
+    bn = variant == 1 ? LpZB16_1::BN : variant == 2 ? LpZB16_2::BN : LpZB16_0::BN;
+  }
+  return (long long)cdiv(m, bm) * cdiv(ldc, bn);
+}
+
+long long lp_theta_job_tiles(int mode, int variant, int ng, int rk) {
+  if (variant < 0) variant = mode == 1 ? 2 : 0;
+  int bm, bn;
+  if (mode == 1) {
+    bm = variant == 1 ? LpF32_1::BM : variant == 2 ? LpF32_2::BM : LpF32_0::BM;
+    bn = variant == 1 ? LpF32_1::BN : variant == 2 ? LpF32_2::BN : LpF32_0::BN;
+  } else {
+    bm = variant == 1 ? LpTB16_1::BM : variant == 2 ? LpTB16_2::BM : LpTB16_0::BM;
+    bn = variant == 1 ? LpTB16_1::BN : variant == 2 ? LpTB16_2::BN : LpTB16_0::BN;
+  }
+  return (long long)cdiv(ng, bn) * cdiv(rk, bm);
+}
+
+int launch_zct_lp_grouped(int mode, int variant, const LpGemmJob* jobs, int njobs, long long total,
+                          int grid, hipStream_t st) {
+  if (variant < 0) variant = mode == 1 ? 2 : 0;
+#define ISDF_ZG(TL)                                                                             \
+  hipLaunchKernelGGL((k_zct_lp_grouped<TL>), dim3(grid), dim3(TL::NT), 0, st, jobs, njobs, total)
+  if (mode == 1) {
+    if (variant == 1) ISDF_ZG(LpF32_1);
+    else if (variant == 2) ISDF_ZG(LpF32_2);
+    else ISDF_ZG(LpF32_0);
+  } else {
+    if (variant == 1) ISDF_ZG(LpZB16_1);
+    else if (variant == 2) ISDF_ZG(LpZB16_2);
+    else ISDF_ZG(LpZB16_0);
+  }
+#undef ISDF_ZG
+  ISDF_HIP_LAUNCHED("k_zct_lp_grouped");
+  return 0;
+}
+
+int launch_theta_lp_grouped(int mode, int variant, const LpThetaJob* jobs, int njobs,
+                            long long total, int grid, hipStream_t st) {
+  if (variant < 0) variant = mode == 1 ? 2 : 0;
+#define ISDF_TG(TL)                                                                             \
+  hipLaunchKernelGGL((k_theta_lp_grouped<TL>), dim3(grid), dim3(TL::NT), 0, st, jobs, njobs,    \
+                     total)
+  if (mode == 1) {
+    if (variant == 1) ISDF_TG(LpF32_1);
+    else if (variant == 2) ISDF_TG(LpF32_2);
+    else ISDF_TG(LpF32_0);
+  } else {
+    if (variant == 1) ISDF_TG(LpTB16_1);
+    else if (variant == 2) ISDF_TG(LpTB16_2);
+    else ISDF_TG(LpTB16_0);
+  }
+#undef ISDF_TG
+  ISDF_HIP_LAUNCHED("k_theta_lp_grouped");
+  return 0;
+}
+
+int launch_cct_lp_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st) {
+  hipLaunchKernelGGL(k_cct_lp_batched, dim3(cdiv((long long)max_n * max_n, 256), 1, nb),
+                     dim3(256), 0, st, jobs);
+  ISDF_HIP_LAUNCHED("k_cct_lp_batched");
+  return launch_trace_eps_batched(jobs, nb, max_n, st);
+}
+
+int launch_inverse_batched(int mode, const CholJob* jobs, int nb, int max_n, int max_ldz,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_identity_batched, dim3(cdiv((long long)max_n * max_n, 256), 1, nb),
+                     dim3(256), 0, st, jobs);
+  ISDF_HIP_LAUNCHED("k_identity_batched");
+  if (int rc = launch_potrs_identity_batched(jobs, nb, max_n, st)) return rc;
+  const dim3 grid(cdiv((long long)max_n * max_ldz, 256), 1, nb);
+  if (mode == 2)
+    hipLaunchKernelGGL(k_pack_w_batched<true>, grid, dim3(256), 0, st, jobs);
+  else
+    hipLaunchKernelGGL(k_pack_w_batched<false>, grid, dim3(256), 0, st, jobs);
+  ISDF_HIP_LAUNCHED("k_pack_w_batched");
+  return 0;
+}
+
+int launch_clear_info_batched(const CholJob* jobs, int nb, hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_info_batched, dim3(cdiv(nb, 64)), dim3(64), 0, st, jobs, nb);
+  ISDF_HIP_LAUNCHED("k_clear_info_batched");
   return 0;
 }
 

@@ -486,6 +486,15 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
 
 }  // namespace
 
+int solve_single_locked(const isdf_problem* p, const double* d_psi, double* d_theta, int* h_piv,
+                        hipStream_t st, isdf_report* rep, const isdf_debug* dbg) {
+  return solve_locked(p, d_psi, d_theta, h_piv, st, rep, dbg);
+}
+int problem_rank(const isdf_problem* p) { return irank(p); }
+int validate_problem(const isdf_problem* p, long long* ng_out) { return validate(p, ng_out); }
+std::mutex& solve_mutex() { return g_mutex; }
+void release_batch_plan();
+
 int set_error(int code, const std::string& msg) {
   std::lock_guard<std::mutex> lk(g_err_mutex);
   g_last_error = msg;
@@ -502,9 +511,9 @@ int isdf_hip_solve(const isdf_problem* prob, const double* d_psi, double* d_thet
   isdf_hip::set_error(0, "");
   const int rc =
       isdf_hip::solve_locked(prob, d_psi, d_theta, h_piv, (hipStream_t)stream, rep, dbg);
-  if (rc != ISDF_OK) {
-    // A failure may leave work queued on either stream; drain both before the
-    // workspace is reused.
+  if (rc == ISDF_EHIP) {
+    // A failed HIP call may leave work queued on either stream; drain both
+    // before the workspace is reused.
     (void)hipStreamSynchronize((hipStream_t)stream);
     int dev = 0;
     if (hipGetDevice(&dev) == hipSuccess && (int)isdf_hip::g_aux.size() > dev &&
@@ -530,6 +539,7 @@ const char* isdf_hip_version(void) { return "libisdf_hip 0.1 (gfx950)"; }
 
 int isdf_hip_release_workspace(void) {
   std::lock_guard<std::mutex> lk(isdf_hip::g_mutex);
+  isdf_hip::release_batch_plan();
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return ISDF_EHIP;
   if ((int)isdf_hip::g_ws.size() > dev && isdf_hip::g_ws[dev].ptr) {

@@ -41,13 +41,16 @@ def test_struct_layout_matches_c_header(tmp_path):
     prog.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "isdf_hip.h"\n'
                     'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(isdf_problem),'
                     ' sizeof(isdf_report), sizeof(isdf_debug), sizeof(isdf_check),'
-                    ' offsetof(isdf_problem, ld_psi), offsetof(isdf_report, ms_total));return 0;}\n')
+                    ' offsetof(isdf_problem, ld_psi), offsetof(isdf_report, ms_total));'
+                    'printf("%zu %zu\\n", sizeof(isdf_batch_report),'
+                    ' offsetof(isdf_batch_report, theta_tile_ms));return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)])
     got = [int(x) for x in subprocess.check_output([str(exe)]).split()]
     want = [ctypes.sizeof(_lib.IsdfProblem), ctypes.sizeof(_lib.IsdfReport),
             ctypes.sizeof(_lib.IsdfDebug), ctypes.sizeof(_lib.IsdfCheck),
-            _lib.IsdfProblem.ld_psi.offset, _lib.IsdfReport.ms_total.offset]
+            _lib.IsdfProblem.ld_psi.offset, _lib.IsdfReport.ms_total.offset,
+            ctypes.sizeof(_lib.IsdfBatchReport), _lib.IsdfBatchReport.theta_tile_ms.offset]
     assert got == want
 
 
@@ -106,3 +109,18 @@ def test_check_validator_dual_pair_sets():
     cou = oracle.coulomb_error(psi, (3.0, 3.5, 2.0), n, 5, 4, 3, 2, 5, theta, piv)
     assert c["fit_error"] == pytest.approx(fit, rel=1e-12)
     assert c["coulomb_error"] == pytest.approx(cou, rel=1e-9)
+
+
+@pytest.mark.parametrize("prec,msg", [
+    (isdf.ISDF_PREC_FP64, "mixed forms"),
+    (isdf.ISDF_PREC_FP32 | isdf.ISDF_PREC_REFINE, "REFINE is not supported"),
+])
+def test_batched_rejects_before_device_work(prec, msg):
+    dom = isdf.Domain((10.0,) * 3, (16, 16, 16))
+    inst = [dict(d_psi=1, domain=dom, nocc=8, nstate=16, nv1=8, nc1=8, nv2=8, nc2=8,
+                 mu_points=64, d_theta=1, piv=np.zeros(4096, np.int32), max_iter=5)]
+    with pytest.raises(ValueError, match=msg):
+        isdf.solve_batched(inst, precision=prec, stream=0)
+    inst[0]["nv2"] = 4  # differing pair sets
+    with pytest.raises(ValueError, match="pair sets|mixed forms"):
+        isdf.solve_batched(inst, precision=isdf.ISDF_PREC_FP32, stream=0)

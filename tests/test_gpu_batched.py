@@ -1,0 +1,104 @@
+"""The batched solve (isdf_hip_solve_batched) through the C ABI.
+
+Each instance of a batch must come out exactly as the single solve of the same
+precision gives it: the same K-means (piv bit for bit), the same kernels for
+the Cholesky chain, and grouped contractions whose per-element sums run in the
+same k order as the single launches.  The captured hipGraph is replayed
+several times, with eager work and synchronisations between the replays, and
+every replay must reproduce the first call bit for bit.
+"""
+import numpy as np
+import pytest
+
+import libisdf_amd as isdf
+import oracle
+from libisdf_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def make_batch(seed, count, partial=None):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.choice([12, 14, 16, 20]))
+        grid = (n, n + 2 * (i % 2), n)
+        L = tuple(0.32 * g for g in grid)
+        nstate = int(rng.choice([8, 12, 16, 20]))
+        nocc = nstate // 2
+        ng = grid[0] * grid[1] * grid[2]
+        psi = synth.cholesky_qr2(rng.standard_normal((nstate, ng)), L[0] * L[1] * L[2] / ng)
+        if partial is not None and i == partial:
+            psi[:, : 2 * grid[0] * grid[1]] = 0.0  # npt < Ng: the single-solve fallback
+        out.append(dict(psi=psi, grid=grid, L=L, nstate=nstate, nocc=nocc, mu=8 * nocc,
+                        iters=20))
+    return out
+
+
+def to_instances(batch):
+    inst = []
+    for b in batch:
+        ng = b["psi"].shape[1]
+        inst.append(dict(d_psi=torch.from_numpy(b["psi"]).cuda(),
+                         domain=isdf.Domain(b["L"], b["grid"]), nocc=b["nocc"],
+                         nstate=b["nstate"], nv1=b["nocc"], nc1=b["nocc"], nv2=b["nocc"],
+                         nc2=b["nocc"], mu_points=b["mu"],
+                         d_theta=torch.full((b["mu"], ng), float("nan"), dtype=torch.float64,
+                                            device="cuda"),
+                         piv=np.full(ng, -5, np.int32), max_iter=b["iters"]))
+    return inst
+
+
+def single(a, precision):
+    d_theta = torch.empty_like(a["d_theta"])
+    piv = np.zeros_like(a["piv"])
+    rep = isdf.solve(a["d_psi"], a["domain"], a["nocc"], a["nstate"], a["nv1"], a["nc1"],
+                     a["nv2"], a["nc2"], a["mu_points"], d_theta, piv, a["max_iter"],
+                     precision=precision)
+    torch.cuda.synchronize()
+    return d_theta.cpu().numpy(), piv, rep
+
+
+@pytest.mark.parametrize("precision", [isdf.ISDF_PREC_FP32, isdf.ISDF_PREC_BF16X3])
+def test_batched_matches_single_solves_and_replays(precision):
+    batch = make_batch(31, 6, partial=4)
+    inst = to_instances(batch)
+    reps, brep = isdf.solve_batched(inst, precision=precision)
+    torch.cuda.synchronize()
+    assert brep["graph_replayed"] == 0
+    first = [a["d_theta"].cpu().numpy() for a in inst]
+    for i, a in enumerate(inst):
+        th, piv, rep = single(a, precision)
+        np.testing.assert_array_equal(a["piv"], piv)
+        assert reps[i]["rk_kmeans"] == rep["rk_kmeans"]
+        assert reps[i]["kmeans_iters"] == rep["kmeans_iters"]
+        assert np.isfinite(first[i]).all()
+        err = np.linalg.norm(first[i] - th) / np.linalg.norm(th)
+        assert err < 1e-12, (i, err)
+    # The partial-density instance went through the single solve.
+    assert reps[4]["npt"] < inst[4]["domain"].NumGridTotal()
+    for rep_i in range(3):
+        for a in inst:  # eager work between replays
+            a["d_theta"].fill_(0.0)
+            a["piv"][:] = -1
+        torch.cuda.synchronize()
+        reps2, brep2 = isdf.solve_batched(inst, precision=precision)
+        torch.cuda.synchronize()
+        assert brep2["graph_replayed"] == 1
+        for i, a in enumerate(inst):
+            np.testing.assert_array_equal(a["d_theta"].cpu().numpy().view(np.uint64),
+                                          first[i].view(np.uint64))
+            assert reps2[i]["kmeans_iters"] == reps[i]["kmeans_iters"]
+
+
+def test_batched_pivots_match_oracle():
+    batch = make_batch(7, 3)
+    inst = to_instances(batch)
+    isdf.solve_batched(inst, precision=isdf.ISDF_PREC_BF16X3)
+    torch.cuda.synchronize()
+    for b, a in zip(batch, inst):
+        w = oracle.weights(b["psi"], b["nocc"], b["nocc"], b["nocc"], b["nocc"], b["nocc"])
+        km = oracle.kmeans(w, b["mu"], b["iters"], b["L"], b["grid"])
+        np.testing.assert_array_equal(a["piv"], km["piv"])
