@@ -4,14 +4,15 @@
 //
 // Schedule of one batch (all of it captured once into a hipGraph and replayed
 // on later calls with the same shapes and buffers):
-//   1. per instance, on one of a few helper streams (concurrent branches of
-//      the graph): weights -> K-means (every iteration queued, no host polls)
-//      -> piv -> operand rows of Psi(mu) and psi -> piv copied to pinned host
-//      memory;
+//   1. weights -> K-means -> piv of every instance, each stage one launch over
+//      the whole batch (blockIdx.y = instance; every Lloyd iteration queued,
+//      converged instances return at once), then per instance the operand
+//      rows of Psi(mu) and psi and the copy of piv to pinned host memory;
 //   2. one persistent grouped launch of the pivot-row contractions (C C^T of
 //      every instance), then the batched Cholesky chain (every launch covers
 //      all instances: blockIdx.z = instance) up to each instance's W;
-//      beside it, one persistent grouped launch of every instance's Z C^T;
+//      beside it, on a helper stream, one persistent grouped launch of every
+//      instance's Z C^T;
 //   3. one persistent grouped launch of every instance's Theta = Zt W.
 // The first call for a batch shape runs the schedule eagerly and times every
 // tile variant of the two grouped contractions (the autotuner); the graph is
@@ -100,6 +101,8 @@ struct Plan {
   LpThetaJob* d_tjobs = nullptr;
   StatusSrc* d_src = nullptr;
   InstStatus* d_status = nullptr;
+  KmJob* d_km = nullptr;
+  std::vector<KmJob> h_km;
   std::vector<LpGemmJob> pjobs, zjobs;
   std::vector<LpThetaJob> tjobs;
   long long ptotal = 0, ztotal = 0, ttotal = 0;
@@ -188,7 +191,7 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
   const size_t off_jobs = off;
   off += al256(sizeof(CholJob) * nb) + 2 * al256(sizeof(LpGemmJob) * nb) +
          al256(sizeof(LpThetaJob) * nb) + al256(sizeof(StatusSrc) * nb) +
-         al256(sizeof(InstStatus) * nb);
+         al256(sizeof(InstStatus) * nb) + al256(sizeof(KmJob) * nb);
   hipError_t e = hipMalloc(&pl.dev, off);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -204,6 +207,7 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
   char* d = (char*)pl.dev;
   std::vector<CholJob> chol(nb);
   std::vector<StatusSrc> src(nb);
+  pl.h_km.assign(nb, KmJob{});
   for (int i = 0; i < nb; i++) {
     Inst& I = pl.inst[i];
     const int nJ = (I.rk + kNB - 1) / kNB + 1;
@@ -232,7 +236,23 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
     I.eps = (double*)(sc + 64);
     chol[i] = CholJob{I.P, I.Dinv, I.D128, I.Ft, I.Gt, I.W, I.Wf, I.Pz, I.info, I.eps, I.rk,
                       I.ldz};
-    src[i] = StatusSrc{I.kb.bcount + I.kb.nblk_flag, I.kb.scal, I.info, I.eps};
+    src[i] = StatusSrc{I.kb.scal + KS_NPT, I.kb.scal, I.info, I.eps};
+    kmeans_geometry(&I.kb, I.p.ngrid, I.p.length, I.ng);
+    const int s2 = I.p.nocc - I.p.nv2, K2 = I.p.nv2 + I.p.nc2;
+    KmJob& kj = pl.h_km[i];
+    kj.b = I.kb;
+    kj.psi = I.psi;
+    kj.ld = I.ld;
+    kj.lo = std::min(I.s1, s2);
+    kj.hi = std::max(I.s1 + I.K1, s2 + K2);
+    kj.s1lo = I.s1;
+    kj.s1hi = I.s1 + I.K1;
+    kj.s2lo = s2;
+    kj.s2hi = s2 + K2;
+    kj.rk = I.rk;
+    kj.max_iter = I.p.kmeans_max_iter;
+    kj.maxF = 1e-5 * I.ng;
+    kj.piv = I.piv_dev;
   }
   size_t o = off_jobs;
   pl.d_chol = (CholJob*)(d + o);
@@ -246,6 +266,9 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
   pl.d_src = (StatusSrc*)(d + o);
   o += al256(sizeof(StatusSrc) * nb);
   pl.d_status = (InstStatus*)(d + o);
+  o += al256(sizeof(InstStatus) * nb);
+  pl.d_km = (KmJob*)(d + o);
+  ISDF_HIP_TRY(hipMemcpy(pl.d_km, pl.h_km.data(), sizeof(KmJob) * nb, hipMemcpyHostToDevice));
   ISDF_HIP_TRY(hipMemcpy(pl.d_chol, chol.data(), sizeof(CholJob) * nb, hipMemcpyHostToDevice));
   ISDF_HIP_TRY(hipMemcpy(pl.d_src, src.data(), sizeof(StatusSrc) * nb, hipMemcpyHostToDevice));
   int cus = 0;
@@ -293,16 +316,8 @@ int set_jobs(Plan& pl, int zv, int tv) {
   return 0;
 }
 
-// Per-instance front: weights -> K-means -> piv -> operand rows.
+// Per-instance operand rows, after the batched K-means has written piv.
 int front(Plan& pl, Inst& I, hipStream_t h) {
-  const isdf_problem& p = I.p;
-  const int s2 = p.nocc - p.nv2, K2 = p.nv2 + p.nc2;
-  if (int rc = launch_weights(I.psi, I.ld, I.ng, I.s1, I.s1 + I.K1, s2, s2 + K2, I.kb.w, h))
-    return rc;
-  if (int rc = kmeans_prepare(&I.kb, p.ngrid, p.length, 1e-8, nullptr, h)) return rc;
-  if (int rc = kmeans_setup_points(&I.kb, p.ngrid, p.length, I.ng, I.rk, h)) return rc;
-  if (int rc = kmeans_iterate(&I.kb, I.rk, p.kmeans_max_iter, 1e-5, nullptr, h)) return rc;
-  if (int rc = kmeans_finish(&I.kb, I.piv_dev, h, true)) return rc;
   ISDF_HIP_TRY(hipMemcpyAsync(pl.piv_host + I.piv_host_off, I.piv_dev, sizeof(int) * (size_t)I.ng,
                               hipMemcpyDeviceToHost, h));
   if (int rc = launch_gather_rows(pl.mode, I.psi, I.ld, I.piv_dev, I.rk, I.s1, I.K1, I.kp, I.B, h))
@@ -343,6 +358,8 @@ struct Timer {
 int enqueue(Plan& pl, hipStream_t m, bool tune) {
   const int nb = pl.nb, nh = (int)pl.hs.size();
   hipEvent_t fork = pl.ev[0], zfork = pl.ev[nh + 1], zjoin = pl.ev[nh + 2];
+  // K-means of every instance, each stage one launch over the batch.
+  if (int rc = kmeans_batched(pl.d_km, pl.h_km.data(), nb, m)) return rc;
   ISDF_HIP_TRY(hipEventRecord(fork, m));
   for (int h = 0; h < nh; h++) ISDF_HIP_TRY(hipStreamWaitEvent(pl.hs[h], fork, 0));
   for (int i = 0; i < nb; i++)

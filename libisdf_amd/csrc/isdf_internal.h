@@ -77,6 +77,23 @@ enum {
   KS_COUNT = 8
 };
 
+// One instance of a batched K-means run (device array; the batched launches
+// take blockIdx.y as the instance).  Every grid point is a K-means point.
+struct KmJob {
+  KmeansBufs b;
+  const double* psi;
+  long long ld;
+  int lo, hi, s1lo, s1hi, s2lo, s2hi;  // state ranges of the weights
+  int rk, max_iter;
+  double maxF;                          // kmeans_tol * n
+  int* piv;                             // device pivot list (n entries)
+};
+
+// Weights -> K-means -> piv for every instance, each stage one launch over all
+// instances; no host synchronisation (every iteration is queued; converged
+// instances return at once).  h_jobs mirrors d_jobs on the host.
+int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t st);
+
 size_t kmeans_bytes(int n, const int ngrid[3], int rkmax, int max_iter);
 void kmeans_carve(KmeansBufs* b, void* base, int n, const int ngrid[3], int rkmax, int max_iter);
 
@@ -91,6 +108,8 @@ int kmeans_prepare(KmeansBufs* b, const int ngrid[3], const double length[3], do
                    int* npt_host, hipStream_t st);
 int kmeans_setup_points(KmeansBufs* b, const int ngrid[3], const double length[3], int npt,
                         int rk, hipStream_t st);
+// Host-side geometry of b (grid, spacing, point blocks) for npt points.
+void kmeans_geometry(KmeansBufs* b, const int ngrid[3], const double length[3], int npt);
 // The Lloyd loop; returns the number of iterations through *iters_host.  With
 // iters_host NULL all max_iter iterations are queued without host reads (the
 // kernels stop working once converged; the count stays in scal[KS_ITERS]).
