@@ -98,12 +98,13 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct(int ng, int rk, const doubl
 
 // trace summed in ascending order, eps = 1e-10 * trace / rk added to the
 // diagonal (getbasis_scalapack.cpp:398-428).  One block.
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_trace_eps(double* __restrict__ P, int rk,
                                                    double* __restrict__ eps_out,
                                                    const CholJob* jobs) {
   __shared__ double eps_s;
   __shared__ double diag[8192];
-  if (jobs) {
+  if constexpr (BATCH) {
     const CholJob& J = jobs[blockIdx.z];
     P = J.P;
     rk = J.n;
@@ -127,17 +128,27 @@ __global__ __launch_bounds__(256) void k_trace_eps(double* __restrict__ P, int r
 // Blocked right-looking Cholesky, lower (the reference's pdpotrf 'L').
 // ---------------------------------------------------------------------------
 
+// Value of lane `src` (a compile-time constant after unrolling) in every lane,
+// through two v_readlane_b32: no LDS, no barrier.
+__device__ __forceinline__ double wave_bcast(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Factor the jb x jb diagonal block at J0 with one wave, write L_JJ back and
 // its inverse into Dinv[J] (64 x 64, column-major, zero padded).  Lane r keeps
-// row r of the block in registers (fully unrolled, static indices); the column
-// of each step and the finished factor pass through LDS as broadcasts.
+// row r of the block in registers (fully unrolled, static indices); column j
+// of each step reaches the other lanes by readlane broadcasts.  The inverse
+// is formed right-looking (column `lane` of inv(L_JJ)): the subtractions of one
+// step are independent, and each entry still accumulates its terms in
+// ascending t before the division by L_ii.
+template <bool BATCH>
 __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int n, int J0,
                                                    double* __restrict__ Dinv,
                                                    int* __restrict__ info, const CholJob* jobs) {
-  __shared__ double colj[kNB];
-  __shared__ double Lrow[kNB][kNB + 1];  // Lrow[i][t] = L[i][t]
-  __shared__ int fail_s;
-  if (jobs) {
+  if constexpr (BATCH) {
     const CholJob& J = jobs[blockIdx.z];
     A = J.P;
     n = J.n;
@@ -152,63 +163,57 @@ __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int n
   double v[kNB];  // v[k] = A[lane][k], k <= lane
 #pragma unroll
   for (int k = 0; k < kNB; k++) v[k] = (k < jb && lane < jb && lane >= k) ? Ab[(long long)k * n + lane] : 0.0;
-  if (lane == 0) fail_s = 0;
-  __syncthreads();
+  int fail = 0;
 #pragma unroll
   for (int j = 0; j < kNB; j++) {
-    if (j < jb) {
-      // Column j: lane j publishes its pivot, lanes below publish A[lane][j].
-      colj[lane] = v[j];
-      __syncthreads();
-      const double piv = colj[j];
+    if (j < jb && !fail) {
+      const double piv = wave_bcast(v[j], j);
       if (!(piv > 0.0)) {
-        if (lane == 0 && fail_s == 0) fail_s = j + 1;
-      }
-      const double d = sqrt(piv);
-      double lij = 0.0;
-      if (lane == j) v[j] = d;
-      if (lane > j && lane < jb) {
-        lij = v[j] / d;
-        v[j] = lij;
-      }
-      __syncthreads();
-      colj[lane] = lij;
-      __syncthreads();
+        fail = j + 1;
+      } else {
+        const double d = sqrt(piv);
+        double lij = 0.0;
+        if (lane == j) v[j] = d;
+        if (lane > j && lane < jb) {
+          lij = v[j] / d;
+          v[j] = lij;
+        }
 #pragma unroll
-      for (int k = j + 1; k < kNB; k++)
-        if (k <= lane) v[k] -= lij * colj[k];
-      __syncthreads();
+        for (int k = j + 1; k < kNB; k++) {
+          const double lkj = wave_bcast(lij, k);
+          if (k <= lane) v[k] -= lij * lkj;
+        }
+      }
     }
   }
-  if (fail_s) {
-    if (lane == 0) atomicCAS(info, 0, J0 + fail_s);
+  if (fail) {
+    if (lane == 0) atomicCAS(info, 0, J0 + fail);
     return;
   }
   double* Aw = A + (long long)J0 * n + J0;
 #pragma unroll
-  for (int k = 0; k < kNB; k++) {
+  for (int k = 0; k < kNB; k++)
     if (k < jb && lane < jb && lane >= k) Aw[(long long)k * n + lane] = v[k];
-    Lrow[lane][k] = (k <= lane && lane < jb) ? v[k] : 0.0;
-  }
-  __syncthreads();
-  // Column `lane` of inv(L_JJ) by forward substitution, in registers.
+  // Column c = lane of inv(L_JJ): x = e_c, then for t ascending x_t /= L_tt and
+  // x_i -= L_it x_t (i > t).  L_it is lane i's v[t].
   const int c = lane;
   double x[kNB];
 #pragma unroll
-  for (int i = 0; i < kNB; i++) {
-    double s = 0.0;
+  for (int i = 0; i < kNB; i++) x[i] = (i == c) ? 1.0 : 0.0;
 #pragma unroll
-    for (int t = 0; t < i; t++) s += Lrow[i][t] * x[t];  // x[t] = 0 for t < c
-    double val = 0.0;
-    if (c < jb && i < jb) {
-      if (i == c) val = 1.0 / Lrow[i][i];
-      else if (i > c) val = -s / Lrow[i][i];
+  for (int t = 0; t < kNB; t++) {
+    if (t < jb) {
+      x[t] = x[t] / wave_bcast(v[t], t);
+#pragma unroll
+      for (int i = t + 1; i < kNB; i++) {
+        const double lit = wave_bcast(v[t], i);
+        x[i] -= lit * x[t];
+      }
     }
-    x[i] = val;
   }
   double* D = Dinv + (long long)(J0 / kNB) * kNB * kNB;
 #pragma unroll
-  for (int i = 0; i < kNB; i++) D[(long long)c * kNB + i] = x[i];
+  for (int i = 0; i < kNB; i++) D[(long long)c * kNB + i] = (c < jb && i < jb) ? x[i] : 0.0;
 }
 
 typedef MfmaTile<128, 64, 16, 2, 2> PanelTile;
@@ -217,12 +222,13 @@ typedef MfmaTile<128, 128, 16, 2, 2> SyrkTile;
 // L_IJ = A_IJ inv(L_JJ)^T for the rows below the diagonal block (MFMA).
 // Each workgroup owns 128 rows, so the in-place overwrite is safe once its
 // GEMM has read them.
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_potrf_panel(double* __restrict__ A, int n, int J0,
                                                      const double* __restrict__ Dinv,
                                                      const int* __restrict__ info,
                                                      const CholJob* jobs) {
   __shared__ double lds[PanelTile::LDS_DOUBLES];
-  if (jobs) {
+  if constexpr (BATCH) {
     const CholJob& J = jobs[blockIdx.z];
     A = J.P;
     n = J.n;
@@ -256,11 +262,12 @@ __global__ __launch_bounds__(256) void k_potrf_panel(double* __restrict__ A, int
 }
 
 // Trailing update A_IK -= L_IJ L_KJ^T on the lower 128 x 128 tiles (MFMA).
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_potrf_syrk(double* __restrict__ A, int n, int J0,
                                                     const int* __restrict__ info,
                                                     const CholJob* jobs) {
   __shared__ double lds[SyrkTile::LDS_DOUBLES];
-  if (jobs) {
+  if constexpr (BATCH) {
     const CholJob& J = jobs[blockIdx.z];
     A = J.P;
     n = J.n;
@@ -309,6 +316,7 @@ __global__ __launch_bounds__(256) void k_potrf_syrk(double* __restrict__ A, int 
 constexpr int kSB = 128;
 
 // Dinv128[J] = inv(L_JJ) for the 128-blocks: [[Da, 0], [-Db L_ba Da, Db]].
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_dinv128(const double* __restrict__ L, int n,
                                                  const double* __restrict__ Dinv64,
                                                  double* __restrict__ Dinv128,
@@ -317,7 +325,7 @@ __global__ __launch_bounds__(256) void k_dinv128(const double* __restrict__ L, i
   __shared__ double db[kNB][kNB];
   __shared__ double lt[kNB][kNB];   // L_ba Da, [col][row]
   const int J = blockIdx.x, J0 = J * kSB;
-  if (jobs) {
+  if constexpr (BATCH) {
     const CholJob& jb = jobs[blockIdx.z];
     L = jb.P;
     n = jb.n;
@@ -371,6 +379,7 @@ __global__ __launch_bounds__(256) void k_dinv128(const double* __restrict__ L, i
 typedef MfmaTile<128, 128, 16, 2, 2, false> FgTile;
 
 // One workgroup per block pair (P, Q), Q <= P: F_PQ and G_PQ.
+template <bool BATCH>
 __global__ __launch_bounds__(256, 1) void k_make_fg(const double* __restrict__ L, int n,
                                                     const double* __restrict__ Dinv128,
                                                     double* __restrict__ Ft,
@@ -379,7 +388,7 @@ __global__ __launch_bounds__(256, 1) void k_make_fg(const double* __restrict__ L
   __shared__ double lds[FgTile::LDS_DOUBLES];
   const int P = blockIdx.x, Q = blockIdx.y;
   if (Q > P) return;
-  if (jobs) {
+  if constexpr (BATCH) {
     const CholJob& J = jobs[blockIdx.z];
     L = J.P;
     n = J.n;
@@ -435,10 +444,11 @@ __global__ __launch_bounds__(256, 1) void k_make_fg(const double* __restrict__ L
 }
 
 // The in-place solve.  A workgroup owns Tile::BM rows; Tile::BN == kSB.
-// unit_lower: T is the identity (ng == n), so row i of the forward result
-// (column i of L^{-1}) vanishes left of column i: block columns before the
-// strip's own are zero and stay so, and the forward GEMMs start there.
-template <class Tile>
+// unit_lower: T is the identity (ng == n) and only the forward pass runs, so
+// T becomes L^{-T} (upper triangular).  Row i of it (column i of L^{-1})
+// vanishes left of column i: block columns before the strip's own are zero
+// and stay so, and the forward GEMMs start there.
+template <class Tile, bool BATCH>
 __global__ __launch_bounds__(Tile::NT, 1) void k_potrs(double* __restrict__ T, long long ng,
                                                        long long ldt, int n,
                                                        const double* __restrict__ Ft,
@@ -448,9 +458,9 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_potrs(double* __restrict__ T, l
   static_assert(kSB % Tile::BM == 0 || Tile::BM % kSB == 0, "strip / block alignment");
   __shared__ double lds[Tile::LDS_DOUBLES];
   const long long m0 = (long long)blockIdx.x * Tile::BM;
-  if (jobs) {  // T = W of the instance, which holds the identity
+  if constexpr (BATCH) {  // T = Gt of the instance, which holds the identity
     const CholJob& J = jobs[blockIdx.z];
-    T = J.W;
+    T = J.Gt;
     n = J.n;
     ng = ldt = J.n;
     Ft = J.Ft;
@@ -461,7 +471,8 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_potrs(double* __restrict__ T, l
   double* Tm = T + m0;
   const int nJ = (n + kSB - 1) / kSB;
   const int Js = unit_lower ? (int)(m0 / kSB) : 0;
-  for (int pass = 0; pass < 2; pass++) {
+  const int npass = unit_lower ? 1 : 2;
+  for (int pass = 0; pass < npass; pass++) {
     for (int s = pass == 0 ? Js : 0; s < nJ; s++) {
       const int J = pass == 0 ? s : nJ - 1 - s;
       const int J0 = J * kSB;
@@ -481,6 +492,40 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_potrs(double* __restrict__ T, l
       __syncthreads();
     }
   }
+}
+
+typedef MfmaTile<128, 128, 16, 2, 2> XxtTile;
+
+// W = X X^T for the upper-triangular X = L^{-T} (n x n, column-major), so
+// W = L^{-T} L^{-1} = (L L^T)^{-1}.  One workgroup per lower tile (ti >= tj);
+// X(i, k) = 0 for k < i, so the sum starts at the tile's first row; the tile
+// is written at (i, j) and mirrored to (j, i).
+template <bool BATCH>
+__global__ __launch_bounds__(XxtTile::NT, 1) void k_xxt(const double* __restrict__ X, int n,
+                                                        double* __restrict__ W,
+                                                        const CholJob* jobs) {
+  __shared__ double lds[XxtTile::LDS_DOUBLES];
+  if constexpr (BATCH) {
+    const CholJob& J = jobs[blockIdx.z];
+    X = J.Gt;
+    W = J.W;
+    n = J.n;
+  }
+  const int ti = blockIdx.x, tj = blockIdx.y;
+  if (tj > ti) return;
+  const int i0 = ti * XxtTile::BM, j0 = tj * XxtTile::BN;
+  if (i0 >= n) return;
+  const int mlim = min(XxtTile::BM, n - i0), nlim = min(XxtTile::BN, n - j0);
+  typename XxtTile::Acc acc;
+  XxtTile::zero(acc);
+  XxtTile::run(X + (long long)i0 * n + i0, n, X + (long long)i0 * n + j0, n, n - i0, mlim, nlim,
+               lds, acc);
+  XxtTile::epilogue(acc, [&](int m, int c, double v) {
+    if (m < mlim && c < nlim) {
+      W[(long long)(j0 + c) * n + i0 + m] = v;
+      W[(long long)(i0 + m) * n + j0 + c] = v;
+    }
+  });
 }
 
 // Tile variants of the solve (chosen per shape, see launch_potrs).
@@ -532,7 +577,7 @@ int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, in
 }
 
 int launch_trace_eps(double* P, int rk, double* eps_dev, hipStream_t st) {
-  hipLaunchKernelGGL(k_trace_eps, dim3(1), dim3(256), 0, st, P, rk, eps_dev,
+  hipLaunchKernelGGL(k_trace_eps<false>, dim3(1), dim3(256), 0, st, P, rk, eps_dev,
                      (const CholJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_trace_eps");
   return 0;
@@ -543,16 +588,16 @@ int launch_potrf(double* P, int rk, double* Dinv, int* info_dev, hipStream_t st)
   for (int J = 0; J < nJ; J++) {
     const int J0 = J * kNB;
     const int jb = std::min(kNB, rk - J0);
-    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, st, P, rk, J0, Dinv, info_dev,
+    hipLaunchKernelGGL(k_potrf_diag<false>, dim3(1), dim3(64), 0, st, P, rk, J0, Dinv, info_dev,
                        (const CholJob*)nullptr);
     ISDF_HIP_LAUNCHED("k_potrf_diag");
     const int rest = rk - J0 - jb;
     if (rest > 0) {
-      hipLaunchKernelGGL(k_potrf_panel, dim3(cdiv(rest, PanelTile::BM)), dim3(PanelTile::NT), 0,
+      hipLaunchKernelGGL(k_potrf_panel<false>, dim3(cdiv(rest, PanelTile::BM)), dim3(PanelTile::NT), 0,
                          st, P, rk, J0, Dinv, info_dev, (const CholJob*)nullptr);
       ISDF_HIP_LAUNCHED("k_potrf_panel");
       const unsigned t = cdiv(rest, SyrkTile::BM);
-      hipLaunchKernelGGL(k_potrf_syrk, dim3(t, t), dim3(SyrkTile::NT), 0, st, P, rk, J0, info_dev,
+      hipLaunchKernelGGL(k_potrf_syrk<false>, dim3(t, t), dim3(SyrkTile::NT), 0, st, P, rk, J0, info_dev,
                          (const CholJob*)nullptr);
       ISDF_HIP_LAUNCHED("k_potrf_syrk");
     }
@@ -563,10 +608,10 @@ int launch_potrf(double* P, int rk, double* Dinv, int* info_dev, hipStream_t st)
 int launch_make_fg(const double* L, int rk, const double* Dinv, double* Dinv128, double* Ft,
                    double* Gt, hipStream_t st) {
   const unsigned nJ = cdiv(rk, kSB);
-  hipLaunchKernelGGL(k_dinv128, dim3(nJ), dim3(256), 0, st, L, rk, Dinv, Dinv128,
+  hipLaunchKernelGGL(k_dinv128<false>, dim3(nJ), dim3(256), 0, st, L, rk, Dinv, Dinv128,
                      (const CholJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_dinv128");
-  hipLaunchKernelGGL(k_make_fg, dim3(nJ, nJ), dim3(256), 0, st, L, rk, Dinv128, Ft, Gt,
+  hipLaunchKernelGGL(k_make_fg<false>, dim3(nJ, nJ), dim3(256), 0, st, L, rk, Dinv128, Ft, Gt,
                      (const CholJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_make_fg");
   return 0;
@@ -581,7 +626,7 @@ int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* F
   switch (variant) {
 #define ISDF_POTRS_CASE(V)                                                                      \
   case V:                                                                                       \
-    hipLaunchKernelGGL((k_potrs<PotrsTile##V>), dim3(cdiv(ng, PotrsTile##V::BM)),               \
+    hipLaunchKernelGGL((k_potrs<PotrsTile##V, false>), dim3(cdiv(ng, PotrsTile##V::BM)),               \
                        dim3(PotrsTile##V::NT), 0, st, T, ng, ldt, rk, Ft, Gt, (int)unit_lower,  \
                        (const CholJob*)nullptr);                                                \
     break;
@@ -601,7 +646,7 @@ int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* F
 
 int launch_trace_eps_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st) {
   (void)max_n;
-  hipLaunchKernelGGL(k_trace_eps, dim3(1, 1, nb), dim3(256), 0, st, (double*)nullptr, 0,
+  hipLaunchKernelGGL(k_trace_eps<true>, dim3(1, 1, nb), dim3(256), 0, st, (double*)nullptr, 0,
                      (double*)nullptr, jobs);
   ISDF_HIP_LAUNCHED("k_trace_eps (batched)");
   return 0;
@@ -611,17 +656,17 @@ int launch_potrf_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st)
   const int nJ = (max_n + kNB - 1) / kNB;
   for (int J = 0; J < nJ; J++) {
     const int J0 = J * kNB;
-    hipLaunchKernelGGL(k_potrf_diag, dim3(1, 1, nb), dim3(64), 0, st, (double*)nullptr, 0, J0,
+    hipLaunchKernelGGL(k_potrf_diag<true>, dim3(1, 1, nb), dim3(64), 0, st, (double*)nullptr, 0, J0,
                        (double*)nullptr, (int*)nullptr, jobs);
     ISDF_HIP_LAUNCHED("k_potrf_diag (batched)");
     const int rest = max_n - J0 - kNB;
     if (rest > 0) {
-      hipLaunchKernelGGL(k_potrf_panel, dim3(cdiv(rest, PanelTile::BM), 1, nb),
+      hipLaunchKernelGGL(k_potrf_panel<true>, dim3(cdiv(rest, PanelTile::BM), 1, nb),
                          dim3(PanelTile::NT), 0, st, (double*)nullptr, 0, J0,
                          (const double*)nullptr, (const int*)nullptr, jobs);
       ISDF_HIP_LAUNCHED("k_potrf_panel (batched)");
       const unsigned t = cdiv(rest, SyrkTile::BM);
-      hipLaunchKernelGGL(k_potrf_syrk, dim3(t, t, nb), dim3(SyrkTile::NT), 0, st,
+      hipLaunchKernelGGL(k_potrf_syrk<true>, dim3(t, t, nb), dim3(SyrkTile::NT), 0, st,
                          (double*)nullptr, 0, J0, (const int*)nullptr, jobs);
       ISDF_HIP_LAUNCHED("k_potrf_syrk (batched)");
     }
@@ -631,20 +676,36 @@ int launch_potrf_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st)
 
 int launch_make_fg_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st) {
   const unsigned nJ = cdiv(max_n, kSB);
-  hipLaunchKernelGGL(k_dinv128, dim3(nJ, 1, nb), dim3(256), 0, st, (const double*)nullptr, 0,
+  hipLaunchKernelGGL(k_dinv128<true>, dim3(nJ, 1, nb), dim3(256), 0, st, (const double*)nullptr, 0,
                      (const double*)nullptr, (double*)nullptr, jobs);
   ISDF_HIP_LAUNCHED("k_dinv128 (batched)");
-  hipLaunchKernelGGL(k_make_fg, dim3(nJ, nJ, nb), dim3(256), 0, st, (const double*)nullptr, 0,
+  hipLaunchKernelGGL(k_make_fg<true>, dim3(nJ, nJ, nb), dim3(256), 0, st, (const double*)nullptr, 0,
                      (const double*)nullptr, (double*)nullptr, (double*)nullptr, jobs);
   ISDF_HIP_LAUNCHED("k_make_fg (batched)");
   return 0;
 }
 
 int launch_potrs_identity_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st) {
-  hipLaunchKernelGGL((k_potrs<PotrsTile5>), dim3(cdiv(max_n, PotrsTile5::BM), 1, nb),
+  hipLaunchKernelGGL((k_potrs<PotrsTile5, true>), dim3(cdiv(max_n, PotrsTile5::BM), 1, nb),
                      dim3(PotrsTile5::NT), 0, st, (double*)nullptr, 0LL, 0LL, 0,
                      (const double*)nullptr, (const double*)nullptr, 1, jobs);
   ISDF_HIP_LAUNCHED("k_potrs (batched identity)");
+  return 0;
+}
+
+int launch_xxt(const double* X, int n, double* W, hipStream_t st) {
+  const unsigned t = cdiv(n, XxtTile::BM);
+  hipLaunchKernelGGL(k_xxt<false>, dim3(t, t), dim3(XxtTile::NT), 0, st, X, n, W,
+                     (const CholJob*)nullptr);
+  ISDF_HIP_LAUNCHED("k_xxt");
+  return 0;
+}
+
+int launch_xxt_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st) {
+  const unsigned t = cdiv(max_n, XxtTile::BM);
+  hipLaunchKernelGGL(k_xxt<true>, dim3(t, t, nb), dim3(XxtTile::NT), 0, st,
+                     (const double*)nullptr, 0, (double*)nullptr, jobs);
+  ISDF_HIP_LAUNCHED("k_xxt (batched)");
   return 0;
 }
 

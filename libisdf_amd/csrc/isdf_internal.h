@@ -136,7 +136,8 @@ int launch_make_fg(const double* L, int rk, const double* Dinv, double* Dinv128,
                    double* Gt, hipStream_t st);
 int potrs_variants();
 // T (ng x rk, ld ldt) <- T (L L^T)^{-1} in place.  unit_lower: T is the rk x rk
-// identity, whose forward result is upper triangular (skips the zero blocks).
+// identity and only the forward pass runs: T <- L^{-T} (upper triangular; the
+// zero blocks are skipped).
 int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* Ft,
                  const double* Gt, int variant, hipStream_t st, bool unit_lower = false);
 
@@ -205,8 +206,11 @@ int launch_clear_info_batched(const CholJob* jobs, int nb, hipStream_t st);
 int launch_trace_eps_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st);
 int launch_potrf_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st);
 int launch_make_fg_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st);
-// W = I (L L^T)^{-1} per instance.
+// Gt <- L^{-T} per instance (Gt holds the identity; the forward pass only).
 int launch_potrs_identity_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st);
+// W = X X^T for upper-triangular X = L^{-T}: W = (L L^T)^{-1}.
+int launch_xxt(const double* X, int n, double* W, hipStream_t st);
+int launch_xxt_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Mixed-precision contractions (lowprec.hip).  mode 1 = FP32 MFMA, mode 2 =
@@ -230,9 +234,10 @@ int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void*
 // P(mu, nu) = Zt[piv[mu]][nu] (column-major FP64; piv NULL: row mu) and the eps shift.
 int launch_cct_lp(const float* Zt, int ldz, const int* piv, int rk, double* P, double* eps_dev,
                   hipStream_t st);
-// W = (L L^T)^{-1} (FP64, column-major rk x rk) from the solve operands, and
-// operand rows nu of W(k, nu), k < ldw (zero for k >= rk), in the mode's format.
-int launch_inverse(int mode, int rk, const double* Ft, const double* Gt, double* W, int ldw,
+// W = (L L^T)^{-1} (FP64, column-major rk x rk) from the forward-solve operand
+// (Gt is overwritten: it holds L^{-T} on the way), and operand rows nu of
+// W(k, nu), k < ldw (zero for k >= rk), in the mode's format.
+int launch_inverse(int mode, int rk, const double* Ft, double* Gt, double* W, int ldw,
                    void* Wf, hipStream_t st);
 // theta(g, nu) = (or +=) sum_k Wf[nu][k] Zt[g][k], k < ldk.
 int launch_theta_lp(int mode, int ng, int rk, int ldk, const void* Wf, const float* Zt,

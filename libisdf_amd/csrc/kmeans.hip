@@ -95,11 +95,12 @@ __device__ int block_excl_scan(int v, int* total) {
 // ---------------------------------------------------------------------------
 // Weights, src/cpu/getpoints_Kmeans_MPI.cpp:37-45.
 // ---------------------------------------------------------------------------
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_weights(const double* __restrict__ psi, long long ld,
                                                  int ng, int lo, int hi, int s1lo, int s1hi,
                                                  int s2lo, int s2hi, double* __restrict__ w,
                                                  const KmJob* jobs) {
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     psi = J.psi;
     ld = J.ld;
@@ -130,9 +131,10 @@ __global__ __launch_bounds__(256) void k_weights(const double* __restrict__ psi,
 
 // findMax (numvec_impl.hpp:259-266) as an atomic max over IEEE bit patterns:
 // all weights are >= +0, where the bit order is the numeric order.
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_maxw(const double* __restrict__ w, int n,
                                               unsigned long long* maxbits, const KmJob* jobs) {
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     w = J.b.w;
     n = J.b.n;
@@ -153,10 +155,11 @@ __global__ __launch_bounds__(256) void k_maxw(const double* __restrict__ w, int 
 }
 
 // flags[i] = w[i] > df_tol * maxW (utility.cpp:650-659).
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_thresh(const double* __restrict__ w, int n,
                                                 const unsigned long long* maxbits, double df_tol,
                                                 uint8_t* __restrict__ flags, const KmJob* jobs) {
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     w = J.b.w;
     n = J.b.n;
@@ -171,10 +174,11 @@ __global__ __launch_bounds__(256) void k_thresh(const double* __restrict__ w, in
 }
 
 // Ordered compaction: count, scan, scatter of indices whose flag == want.
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_flag_count(const uint8_t* __restrict__ flags, int n,
                                                     int want, int* __restrict__ bcount,
                                                     const KmJob* jobs) {
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     if ((int)blockIdx.x >= J.b.nblk_flag) return;
     flags = J.b.flags;
@@ -193,9 +197,10 @@ __global__ __launch_bounds__(256) void k_flag_count(const uint8_t* __restrict__ 
 }
 
 // In-place exclusive scan of a[0..m) by one block; a[m] receives the total.
+template <bool BATCH>
 __global__ __launch_bounds__(1024) void k_scan_single(int* a, int m, const KmJob* jobs) {
   __shared__ int carry;
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     a = J.b.bcount;
     m = J.b.nblk_flag;
@@ -216,11 +221,12 @@ __global__ __launch_bounds__(1024) void k_scan_single(int* a, int m, const KmJob
 }
 
 // base_dev, when given, replaces base (an offset still on the device).
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_flag_scatter(const uint8_t* __restrict__ flags, int n,
                                                       int want, const int* __restrict__ bexcl,
                                                       int base, int* __restrict__ out,
                                                       const int* base_dev, const KmJob* jobs) {
-  if (jobs) {  // the tail of piv after the distinct centers
+  if constexpr (BATCH) {  // the tail of piv after the distinct centers
     const KmJob& J = jobs[blockIdx.y];
     if ((int)blockIdx.x >= J.b.nblk_flag) return;
     flags = J.b.flags;
@@ -247,13 +253,14 @@ __global__ __launch_bounds__(256) void k_flag_scatter(const uint8_t* __restrict_
 // Local points, utility.cpp:707-715.  With one rank nptLocal = n; entries of
 // `index` past npt read the zero-initialised tail of the reference's vector,
 // i.e. grid point 0 (utility.cpp:648, :659, :708).
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_setup_points(int n, int npt, const int* __restrict__ index,
                                                       const double* __restrict__ w, int n0, int n1,
                                                       double dx0, double dx1, double dx2,
                                                       int* __restrict__ gidx, double* __restrict__ X,
                                                       double* __restrict__ Y, double* __restrict__ Z,
                                                       double* __restrict__ WL, const KmJob* jobs) {
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     n = npt = J.b.n;
     w = J.b.w;
@@ -281,12 +288,13 @@ __global__ __launch_bounds__(256) void k_setup_points(int n, int npt, const int*
 }
 
 // Initial centroids = the first rk points above the threshold (utility.cpp:750-762).
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_init_centroids(int rk, int n, int npt,
                                                         const int* __restrict__ index, int n0,
                                                         int n1, double dx0, double dx1, double dx2,
                                                         double* cx, double* cy, double* cz,
                                                         int* scal, const KmJob* jobs) {
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     rk = J.rk;
     n = npt = J.b.n;
@@ -315,9 +323,10 @@ __global__ __launch_bounds__(256) void k_init_centroids(int rk, int n, int npt,
 // bricks numbered x fastest), slot s -> grid point or -1 off the grid.  When
 // some grid points are below the density threshold the blocks are plain runs
 // of 256 local point indices instead.
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_build_blocks(int n, int grid_mode, int n0, int n1, int n2,
                                                      int* __restrict__ porder, const KmJob* jobs) {
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     if ((int)blockIdx.x >= J.b.nblk_pts) return;
     n = J.b.n;
@@ -370,6 +379,7 @@ constexpr int kCandCap = 256;
 // the reference's first minimum over all centroids.
 // In grid mode the workgroup also folds its points into the bounding boxes
 // (grid indices) of their clusters, which the centroid sums walk.
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_assign_culled(
     const int* __restrict__ porder, const double* __restrict__ X, const double* __restrict__ Y,
     const double* __restrict__ Z, const double* __restrict__ cx, const double* __restrict__ cy,
@@ -377,7 +387,7 @@ __global__ __launch_bounds__(256) void k_assign_culled(
     int* __restrict__ bbox, int rkmax, int grid_mode, int n0, int n1,
     int* __restrict__ changed_blk, const KmJob* jobs) {
   __shared__ int cand[kCandCap];
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     if ((int)blockIdx.x >= J.b.nblk_pts) return;
     porder = J.b.porder;
@@ -560,8 +570,9 @@ __global__ __launch_bounds__(256) void k_assign_culled(
   }
 }
 
+template <bool BATCH>
 __global__ void k_reset_boxes(int* bbox, int rkmax, const KmJob* jobs) {
-  if (jobs) {
+  if constexpr (BATCH) {
     bbox = jobs[blockIdx.y].b.bbox;
     rkmax = jobs[blockIdx.y].b.rkmax;
   }
@@ -576,13 +587,14 @@ __global__ void k_reset_boxes(int* bbox, int rkmax, const KmJob* jobs) {
 // position order through a block scan, and their products w*x (one rounding
 // each, as in the reference) land in LDS; threads 0..3 then carry the four
 // sequential sums over the round's members in that order.
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_centroid_sums_box(
     int n0, int n1, const int* __restrict__ label, const double* __restrict__ WL,
     const double* __restrict__ X, const double* __restrict__ Y, const double* __restrict__ Z,
     const int* __restrict__ bbox, int rkmax, double* __restrict__ sw, double* __restrict__ sx,
     double* __restrict__ sy, double* __restrict__ sz, const int* scal, const KmJob* jobs) {
   __shared__ double buf[4][1024];
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     n0 = J.b.ngrid[0];
     n1 = J.b.ngrid[1];
@@ -671,6 +683,7 @@ __global__ __launch_bounds__(256) void k_centroid_sums_box(
 // coordinate, and so does any point at the same distance: one wave scans that
 // window (plus one cell for rounding) and keeps the lexicographic minimum of
 // (distance, grid index), i.e. the first minimum in grid order.
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_final_window(int n0, int n1, int n2, double dx0,
                                                       double dx1, double dx2,
                                                       const double* __restrict__ X,
@@ -681,7 +694,7 @@ __global__ __launch_bounds__(256) void k_final_window(int n0, int n1, int n2, do
                                                       const double* __restrict__ cz,
                                                       int* __restrict__ imin, const int* scal,
                                                       const KmJob* jobs) {
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     n0 = J.b.ngrid[0];
     n1 = J.b.ngrid[1];
@@ -875,6 +888,7 @@ __global__ __launch_bounds__(256) void k_centroid_sums(const int* __restrict__ s
 
 // Division and empty-cluster compaction, utility.cpp:831-854 (swap-with-last,
 // re-testing the slot that received the last cluster).  One block.
+template <bool BATCH>
 __global__ __launch_bounds__(1024) void k_update(const double* __restrict__ sw,
                                                  const double* __restrict__ sx,
                                                  const double* __restrict__ sy,
@@ -884,7 +898,7 @@ __global__ __launch_bounds__(1024) void k_update(const double* __restrict__ sw,
                                                  int* scal, int* __restrict__ bbox, int rkmax,
                                                  const KmJob* jobs) {
   __shared__ int nempty, rknew;
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     sw = J.b.sw;
     sx = J.b.sx;
@@ -950,10 +964,11 @@ __global__ __launch_bounds__(1024) void k_update(const double* __restrict__ sw,
 
 // Loop control, utility.cpp:813 and :906-921: the changed-label count is the
 // sum of the assignment's per-block counts.
+template <bool BATCH>
 __global__ __launch_bounds__(1024) void k_control(int s, int max_iter, double maxF, int nblk,
                                                   const int* __restrict__ changed_blk, int* trace,
                                                   int* scal, const KmJob* jobs) {
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     max_iter = J.max_iter;
     maxF = J.maxF;
@@ -1024,11 +1039,12 @@ __global__ __launch_bounds__(256) void k_final_reduce(int nchunk, const double* 
 }
 
 // unique(), utility.cpp:617-629: bitonic sort in LDS, then drop repeats.
+template <bool BATCH>
 __global__ __launch_bounds__(1024) void k_unique(const int* __restrict__ imin,
                                                  int* __restrict__ centers, int* scal,
                                                  const KmJob* jobs) {
   extern __shared__ int s[];
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     imin = J.b.imin;
     centers = J.b.centers;
@@ -1071,11 +1087,12 @@ __global__ __launch_bounds__(1024) void k_unique(const int* __restrict__ imin,
   if (threadIdx.x == 0) scal[KS_RKU] = carry;
 }
 
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_mark_centers(const int* __restrict__ centers,
                                                       uint8_t* __restrict__ flags,
                                                       int* __restrict__ piv, const int* scal,
                                                       const KmJob* jobs) {
-  if (jobs) {
+  if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     centers = J.b.centers;
     flags = J.b.flags;
@@ -1132,12 +1149,12 @@ __global__ void k_km_batch_clear_flags(const KmJob* __restrict__ jobs) {
 int compact_flags(KmeansBufs* b, int want, int base, int* out, hipStream_t st,
                   const int* base_dev = nullptr) {
   const int nblk = b->nblk_flag;
-  hipLaunchKernelGGL(k_flag_count, dim3(nblk), dim3(256), 0, st, b->flags, b->n, want, b->bcount, (const KmJob*)nullptr);
+  hipLaunchKernelGGL(k_flag_count<false>, dim3(nblk), dim3(256), 0, st, b->flags, b->n, want, b->bcount, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_flag_count");
-  hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, b->bcount, nblk, (const KmJob*)nullptr);
+  hipLaunchKernelGGL(k_scan_single<false>, dim3(1), dim3(1024), 0, st, b->bcount, nblk, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_scan_single");
   if (out) {
-    hipLaunchKernelGGL(k_flag_scatter, dim3(nblk), dim3(256), 0, st, b->flags, b->n, want,
+    hipLaunchKernelGGL(k_flag_scatter<false>, dim3(nblk), dim3(256), 0, st, b->flags, b->n, want,
                        b->bcount, base, out, base_dev, (const KmJob*)nullptr);
     ISDF_HIP_LAUNCHED("k_flag_scatter");
   }
@@ -1212,7 +1229,7 @@ void kmeans_carve(KmeansBufs* b, void* basep, int n, const int ngrid[3], int rkm
 int launch_weights(const double* psi, long long ld, int ng, int s1lo, int s1hi, int s2lo,
                    int s2hi, double* w, hipStream_t st) {
   int lo = std::min(s1lo, s2lo), hi = std::max(s1hi, s2hi);
-  hipLaunchKernelGGL(k_weights, dim3(cdiv(ng, 256)), dim3(256), 0, st, psi, ld, ng, lo, hi, s1lo,
+  hipLaunchKernelGGL(k_weights<false>, dim3(cdiv(ng, 256)), dim3(256), 0, st, psi, ld, ng, lo, hi, s1lo,
                      s1hi, s2lo, s2hi, w, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_weights");
   return 0;
@@ -1226,11 +1243,11 @@ int kmeans_prepare(KmeansBufs* b, const int ngrid[3], const double length[3], do
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, b->scal, (long long)KS_COUNT, 0);
   ISDF_HIP_LAUNCHED("k_fill_i32");
   unsigned long long* maxbits = (unsigned long long*)(b->scal + KS_MAXW);
-  hipLaunchKernelGGL(k_maxw, dim3(std::min(cdiv(n, 256), 2048u)), dim3(256), 0, st, b->w, n,
+  hipLaunchKernelGGL(k_maxw<false>, dim3(std::min(cdiv(n, 256), 2048u)), dim3(256), 0, st, b->w, n,
                      maxbits, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_maxw");
   if (df_tol > 1e-16) {
-    hipLaunchKernelGGL(k_thresh, dim3(cdiv(n, 256)), dim3(256), 0, st, b->w, n, maxbits, df_tol,
+    hipLaunchKernelGGL(k_thresh<false>, dim3(cdiv(n, 256)), dim3(256), 0, st, b->w, n, maxbits, df_tol,
                        b->flags, (const KmJob*)nullptr);
   } else {
     hipLaunchKernelGGL(k_fill_u8, dim3(std::min(cdiv(n, 256), 4096u)), dim3(256), 0, st,
@@ -1247,7 +1264,7 @@ int kmeans_prepare(KmeansBufs* b, const int ngrid[3], const double length[3], do
   ISDF_HIP_TRY(hipStreamSynchronize(st));
   *npt_host = npt;
   if (npt < n) {
-    hipLaunchKernelGGL(k_flag_scatter, dim3(b->nblk_flag), dim3(256), 0, st, b->flags, n, 1,
+    hipLaunchKernelGGL(k_flag_scatter<false>, dim3(b->nblk_flag), dim3(256), 0, st, b->flags, n, 1,
                        b->bcount, 0, b->index, (const int*)nullptr, (const KmJob*)nullptr);
     ISDF_HIP_LAUNCHED("k_flag_scatter");
   }
@@ -1273,19 +1290,19 @@ int kmeans_setup_points(KmeansBufs* b, const int ngrid[3], const double length[3
   const int n = b->n;
   double dx0 = length[0] / ngrid[0], dx1 = length[1] / ngrid[1], dx2 = length[2] / ngrid[2];
   kmeans_geometry(b, ngrid, length, npt);
-  hipLaunchKernelGGL(k_setup_points, dim3(cdiv(n, 256)), dim3(256), 0, st, n, npt, b->index, b->w,
+  hipLaunchKernelGGL(k_setup_points<false>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, npt, b->index, b->w,
                      ngrid[0], ngrid[1], dx0, dx1, dx2, b->gidx, b->X, b->Y, b->Z, b->WL, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_setup_points");
-  hipLaunchKernelGGL(k_init_centroids, dim3(cdiv(rk, 256)), dim3(256), 0, st, rk, n, npt, b->index,
+  hipLaunchKernelGGL(k_init_centroids<false>, dim3(cdiv(rk, 256)), dim3(256), 0, st, rk, n, npt, b->index,
                      ngrid[0], ngrid[1], dx0, dx1, dx2, b->cx, b->cy, b->cz, b->scal, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_init_centroids");
-  hipLaunchKernelGGL(k_build_blocks, dim3(b->nblk_pts), dim3(256), 0, st, n, b->grid_mode ? 1 : 0,
+  hipLaunchKernelGGL(k_build_blocks<false>, dim3(b->nblk_pts), dim3(256), 0, st, n, b->grid_mode ? 1 : 0,
                      ngrid[0], ngrid[1], ngrid[2], b->porder, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_build_blocks");
-  hipLaunchKernelGGL(k_reset_boxes, dim3(cdiv(6 * (long long)b->rkmax, 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_reset_boxes<false>, dim3(cdiv(6 * (long long)b->rkmax, 256)), dim3(256), 0, st,
                      b->bbox, b->rkmax, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_reset_boxes");
-  hipLaunchKernelGGL(k_assign_culled, dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X, b->Y,
+  hipLaunchKernelGGL(k_assign_culled<false>, dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X, b->Y,
                      b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 0, b->bbox, b->rkmax,
                      b->grid_mode ? 1 : 0, ngrid[0], ngrid[1], b->changed_blk, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_assign_culled");
@@ -1310,7 +1327,7 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
   const int check_every = 4;
   for (int s = 0; s < max_iter; s++) {
     if (b->grid_mode) {
-      hipLaunchKernelGGL(k_centroid_sums_box, dim3(rk), dim3(256), 0, st, b->ngrid[0],
+      hipLaunchKernelGGL(k_centroid_sums_box<false>, dim3(rk), dim3(256), 0, st, b->ngrid[0],
                          b->ngrid[1], b->label, b->WL, b->X, b->Y, b->Z, b->bbox, b->rkmax, b->sw,
                          b->sx, b->sy, b->sz, b->scal, (const KmJob*)nullptr);
     } else {
@@ -1326,13 +1343,13 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
                          b->base, b->cnt_pts, b->WL, b->X, b->Y, b->Z, b->sw, b->sx, b->sy, b->sz,
                          b->scal);
     }
-    hipLaunchKernelGGL(k_update, dim3(1), dim3(1024), 0, st, b->sw, b->sx, b->sy, b->sz, b->cx,
+    hipLaunchKernelGGL(k_update<false>, dim3(1), dim3(1024), 0, st, b->sw, b->sx, b->sy, b->sz, b->cx,
                        b->cy, b->cz, b->map, b->scal, b->grid_mode ? b->bbox : (int*)nullptr,
                        b->rkmax, (const KmJob*)nullptr);
-    hipLaunchKernelGGL(k_assign_culled, dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X,
+    hipLaunchKernelGGL(k_assign_culled<false>, dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X,
                        b->Y, b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 1, b->bbox, b->rkmax,
                        b->grid_mode ? 1 : 0, b->ngrid[0], b->ngrid[1], b->changed_blk, (const KmJob*)nullptr);
-    hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, st, s, max_iter, maxF, b->nblk_pts,
+    hipLaunchKernelGGL(k_control<false>, dim3(1), dim3(1024), 0, st, s, max_iter, maxF, b->nblk_pts,
                        b->changed_blk, b->trace, b->scal, (const KmJob*)nullptr);
     ISDF_HIP_LAUNCHED("k-means iteration");
     if (poll && (s + 1) % check_every == 0 && s + 1 < max_iter) {
@@ -1355,7 +1372,7 @@ int kmeans_finish(KmeansBufs* b, int* piv_dev, hipStream_t st, bool nosync) {
   const int n = b->n;
   const int rkmax = b->rkmax;
   if (b->grid_mode) {
-    hipLaunchKernelGGL(k_final_window, dim3(cdiv(rkmax, 4)), dim3(256), 0, st, b->ngrid[0],
+    hipLaunchKernelGGL(k_final_window<false>, dim3(cdiv(rkmax, 4)), dim3(256), 0, st, b->ngrid[0],
                        b->ngrid[1], b->ngrid[2], b->dx[0], b->dx[1], b->dx[2], b->X, b->Y, b->Z,
                        b->cx, b->cy, b->cz, b->imin, b->scal, (const KmJob*)nullptr);
     ISDF_HIP_LAUNCHED("k_final_window");
@@ -1370,13 +1387,13 @@ int kmeans_finish(KmeansBufs* b, int* piv_dev, hipStream_t st, bool nosync) {
   }
   int P = 1;
   while (P < rkmax) P <<= 1;
-  hipLaunchKernelGGL(k_unique, dim3(1), dim3(1024), sizeof(int) * (size_t)P, st, b->imin,
+  hipLaunchKernelGGL(k_unique<false>, dim3(1), dim3(1024), sizeof(int) * (size_t)P, st, b->imin,
                      b->centers, b->scal, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_unique");
   // piv = [sorted unique centers] ++ [all other grid indices ascending] (utility.cpp:973-983).
   hipLaunchKernelGGL(k_fill_u8, dim3(std::min(cdiv(n, 256), 4096u)), dim3(256), 0, st, b->flags,
                      (long long)n, (uint8_t)0);
-  hipLaunchKernelGGL(k_mark_centers, dim3(cdiv(rkmax, 256)), dim3(256), 0, st, b->centers,
+  hipLaunchKernelGGL(k_mark_centers<false>, dim3(cdiv(rkmax, 256)), dim3(256), 0, st, b->centers,
                      b->flags, piv_dev, b->scal, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_mark_centers");
   if (nosync) return compact_flags(b, 0, 0, piv_dev, st, b->scal + KS_RKU);
@@ -1400,31 +1417,31 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
   const KmJob* J = d_jobs;
   const unsigned gn = cdiv(maxn, 256);
   // Weights (getpoints_Kmeans_MPI.cpp:33-45).
-  hipLaunchKernelGGL(k_weights, dim3(gn, nb), dim3(256), 0, st, (const double*)nullptr, 0LL, 0, 0,
+  hipLaunchKernelGGL(k_weights<true>, dim3(gn, nb), dim3(256), 0, st, (const double*)nullptr, 0LL, 0, 0,
                      0, 0, 0, 0, 0, (double*)nullptr, J);
   ISDF_HIP_LAUNCHED("k_weights (batched)");
   // Threshold and npt (utility.cpp:646-662); npt stays in bcount[nblk_flag].
   hipLaunchKernelGGL(k_km_batch_init, dim3(1, nb), dim3(64), 0, st, J);
-  hipLaunchKernelGGL(k_maxw, dim3(std::min(gn, 2048u), nb), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_maxw<true>, dim3(std::min(gn, 2048u), nb), dim3(256), 0, st,
                      (const double*)nullptr, 0, (unsigned long long*)nullptr, J);
-  hipLaunchKernelGGL(k_thresh, dim3(gn, nb), dim3(256), 0, st, (const double*)nullptr, 0,
+  hipLaunchKernelGGL(k_thresh<true>, dim3(gn, nb), dim3(256), 0, st, (const double*)nullptr, 0,
                      (const unsigned long long*)nullptr, 1e-8, (uint8_t*)nullptr, J);
-  hipLaunchKernelGGL(k_flag_count, dim3(maxflag, nb), dim3(256), 0, st, (const uint8_t*)nullptr, 0,
+  hipLaunchKernelGGL(k_flag_count<true>, dim3(maxflag, nb), dim3(256), 0, st, (const uint8_t*)nullptr, 0,
                      1, (int*)nullptr, J);
-  hipLaunchKernelGGL(k_scan_single, dim3(1, nb), dim3(1024), 0, st, (int*)nullptr, 0, J);
+  hipLaunchKernelGGL(k_scan_single<true>, dim3(1, nb), dim3(1024), 0, st, (int*)nullptr, 0, J);
   ISDF_HIP_LAUNCHED("threshold (batched)");
   // Points, initial centroids, brick order, first assignment (utility.cpp:682-812).
-  hipLaunchKernelGGL(k_setup_points, dim3(gn, nb), dim3(256), 0, st, 0, 0, (const int*)nullptr,
+  hipLaunchKernelGGL(k_setup_points<true>, dim3(gn, nb), dim3(256), 0, st, 0, 0, (const int*)nullptr,
                      (const double*)nullptr, 0, 0, 0.0, 0.0, 0.0, (int*)nullptr, (double*)nullptr,
                      (double*)nullptr, (double*)nullptr, (double*)nullptr, J);
-  hipLaunchKernelGGL(k_init_centroids, dim3(cdiv(maxrk, 256), nb), dim3(256), 0, st, 0, 0, 0,
+  hipLaunchKernelGGL(k_init_centroids<true>, dim3(cdiv(maxrk, 256), nb), dim3(256), 0, st, 0, 0, 0,
                      (const int*)nullptr, 0, 0, 0.0, 0.0, 0.0, (double*)nullptr, (double*)nullptr,
                      (double*)nullptr, (int*)nullptr, J);
-  hipLaunchKernelGGL(k_build_blocks, dim3(maxpts, nb), dim3(256), 0, st, 0, 1, 0, 0, 0,
+  hipLaunchKernelGGL(k_build_blocks<true>, dim3(maxpts, nb), dim3(256), 0, st, 0, 1, 0, 0, 0,
                      (int*)nullptr, J);
-  hipLaunchKernelGGL(k_reset_boxes, dim3(cdiv(6LL * maxrk, 256), nb), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_reset_boxes<true>, dim3(cdiv(6LL * maxrk, 256), nb), dim3(256), 0, st,
                      (int*)nullptr, 0, J);
-  hipLaunchKernelGGL(k_assign_culled, dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
+  hipLaunchKernelGGL(k_assign_culled<true>, dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
                      (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                      (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                      (int*)nullptr, (int*)nullptr, 0, (int*)nullptr, 0, 1, 0, 0, (int*)nullptr, J);
@@ -1432,40 +1449,40 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
   ISDF_HIP_LAUNCHED("k-means setup (batched)");
   // Lloyd iterations (utility.cpp:812-922); converged instances return at once.
   for (int s = 0; s < maxit; s++) {
-    hipLaunchKernelGGL(k_centroid_sums_box, dim3(maxrk, nb), dim3(256), 0, st, 0, 0,
+    hipLaunchKernelGGL(k_centroid_sums_box<true>, dim3(maxrk, nb), dim3(256), 0, st, 0, 0,
                        (const int*)nullptr, (const double*)nullptr, (const double*)nullptr,
                        (const double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0,
                        (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr,
                        (const int*)nullptr, J);
-    hipLaunchKernelGGL(k_update, dim3(1, nb), dim3(1024), 0, st, (const double*)nullptr,
+    hipLaunchKernelGGL(k_update<true>, dim3(1, nb), dim3(1024), 0, st, (const double*)nullptr,
                        (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                        (double*)nullptr, (double*)nullptr, (double*)nullptr, (int*)nullptr,
                        (int*)nullptr, (int*)nullptr, 0, J);
-    hipLaunchKernelGGL(k_assign_culled, dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
+    hipLaunchKernelGGL(k_assign_culled<true>, dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
                        (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                        (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                        (int*)nullptr, (int*)nullptr, 1, (int*)nullptr, 0, 1, 0, 0, (int*)nullptr,
                        J);
-    hipLaunchKernelGGL(k_control, dim3(1, nb), dim3(1024), 0, st, s, 0, 0.0, 0,
+    hipLaunchKernelGGL(k_control<true>, dim3(1, nb), dim3(1024), 0, st, s, 0, 0.0, 0,
                        (const int*)nullptr, (int*)nullptr, (int*)nullptr, J);
     ISDF_HIP_LAUNCHED("k-means iteration (batched)");
   }
   // Final nearest points, unique(), piv (utility.cpp:926-983).
-  hipLaunchKernelGGL(k_final_window, dim3(cdiv(maxrk, 4), nb), dim3(256), 0, st, 0, 0, 0, 0.0,
+  hipLaunchKernelGGL(k_final_window<true>, dim3(cdiv(maxrk, 4), nb), dim3(256), 0, st, 0, 0, 0, 0.0,
                      0.0, 0.0, (const double*)nullptr, (const double*)nullptr,
                      (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                      (const double*)nullptr, (int*)nullptr, (const int*)nullptr, J);
   int P = 1;
   while (P < maxrk) P <<= 1;
-  hipLaunchKernelGGL(k_unique, dim3(1, nb), dim3(1024), sizeof(int) * (size_t)P, st,
+  hipLaunchKernelGGL(k_unique<true>, dim3(1, nb), dim3(1024), sizeof(int) * (size_t)P, st,
                      (const int*)nullptr, (int*)nullptr, (int*)nullptr, J);
   hipLaunchKernelGGL(k_km_batch_clear_flags, dim3(std::min(gn, 1024u), nb), dim3(256), 0, st, J);
-  hipLaunchKernelGGL(k_mark_centers, dim3(cdiv(maxrk, 256), nb), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_mark_centers<true>, dim3(cdiv(maxrk, 256), nb), dim3(256), 0, st,
                      (const int*)nullptr, (uint8_t*)nullptr, (int*)nullptr, (const int*)nullptr, J);
-  hipLaunchKernelGGL(k_flag_count, dim3(maxflag, nb), dim3(256), 0, st, (const uint8_t*)nullptr, 0,
+  hipLaunchKernelGGL(k_flag_count<true>, dim3(maxflag, nb), dim3(256), 0, st, (const uint8_t*)nullptr, 0,
                      0, (int*)nullptr, J);
-  hipLaunchKernelGGL(k_scan_single, dim3(1, nb), dim3(1024), 0, st, (int*)nullptr, 0, J);
-  hipLaunchKernelGGL(k_flag_scatter, dim3(maxflag, nb), dim3(256), 0, st, (const uint8_t*)nullptr,
+  hipLaunchKernelGGL(k_scan_single<true>, dim3(1, nb), dim3(1024), 0, st, (int*)nullptr, 0, J);
+  hipLaunchKernelGGL(k_flag_scatter<true>, dim3(maxflag, nb), dim3(256), 0, st, (const uint8_t*)nullptr,
                      0, 0, (const int*)nullptr, 0, (int*)nullptr, (const int*)nullptr, J);
   ISDF_HIP_LAUNCHED("k-means finish (batched)");
   return 0;

@@ -547,7 +547,7 @@ __global__ __launch_bounds__(256) void k_identity_batched(const CholJob* __restr
   const CholJob J = jobs[blockIdx.z];
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)J.n * J.n) return;
-  J.W[t] = (t % J.n == t / J.n) ? 1.0 : 0.0;
+  J.Gt[t] = (t % J.n == t / J.n) ? 1.0 : 0.0;
 }
 
 template <bool SPLIT>
@@ -677,13 +677,15 @@ int launch_cct_lp(const float* Zt, int ldz, const int* piv, int rk, double* P, d
   return launch_trace_eps(P, rk, eps_dev, st);
 }
 
-int launch_inverse(int mode, int rk, const double* Ft, const double* Gt, double* W, int ldw,
+int launch_inverse(int mode, int rk, const double* Ft, double* Gt, double* W, int ldw,
                    void* Wf, hipStream_t st) {
-  hipLaunchKernelGGL(k_identity, dim3(cdiv((long long)rk * rk, 256)), dim3(256), 0, st, W, rk);
+  // X = I L^{-T} in Gt (the backward-solve operand is not needed by the mixed
+  // forms): the forward pass of the blocked solve with its 16-row tile (rk/16
+  // workgroups), skipping the blocks the identity leaves zero; then W = X X^T.
+  hipLaunchKernelGGL(k_identity, dim3(cdiv((long long)rk * rk, 256)), dim3(256), 0, st, Gt, rk);
   ISDF_HIP_LAUNCHED("k_identity");
-  // W = I (L L^T)^{-1}: the blocked in-place solve with its 16-row tile (rk/16
-  // workgroups), skipping the blocks the identity leaves zero.
-  if (int rc = launch_potrs(W, rk, rk, rk, Ft, Gt, 5, st, true)) return rc;
+  if (int rc = launch_potrs(Gt, rk, rk, rk, Ft, Gt, 5, st, true)) return rc;
+  if (int rc = launch_xxt(Gt, rk, W, st)) return rc;
   const dim3 grid(cdiv((long long)rk * ldw, 256));
   if (mode == 2)
     hipLaunchKernelGGL(k_pack_w<true>, grid, dim3(256), 0, st, W, rk, ldw, Wf);
@@ -808,6 +810,7 @@ int launch_inverse_batched(int mode, const CholJob* jobs, int nb, int max_n, int
                      dim3(256), 0, st, jobs);
   ISDF_HIP_LAUNCHED("k_identity_batched");
   if (int rc = launch_potrs_identity_batched(jobs, nb, max_n, st)) return rc;
+  if (int rc = launch_xxt_batched(jobs, nb, max_n, st)) return rc;
   const dim3 grid(cdiv((long long)max_n * max_ldz, 256), 1, nb);
   if (mode == 2)
     hipLaunchKernelGGL(k_pack_w_batched<true>, grid, dim3(256), 0, st, jobs);
