@@ -96,6 +96,71 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct(int ng, int rk, const doubl
   }
 }
 
+// Persistent form of k_zct (one product): one workgroup per CU walks the
+// tiles t = blockIdx.x, blockIdx.x + gridDim.x, ...  The first K chunk of its
+// next tile is fetched from HBM while the current tile's last chunk runs and
+// its epilogue stores, so the CU's matrix cores wait only for the epilogue.
+// gridDim.x is a multiple of 8, so tile t stays on XCD t % 8 and the column
+// tiles of one row strip share that XCD's L2 as in the plain launch.
+template <class Tile>
+__global__ __launch_bounds__(Tile::NT, 1) void k_zct_persistent(
+    int ng, int rk, const double* __restrict__ A, long long lda, int K,
+    const double* __restrict__ B, int ldb, double* __restrict__ C, long long ldc) {
+  __shared__ double lds[Tile::LDS_DOUBLES];
+  const int nrows = (ng + Tile::BM - 1) / Tile::BM;
+  const int ncolt = (rk + Tile::BN - 1) / Tile::BN;
+  const int total = nrows * ncolt;
+  const int nch = (K + Tile::KB - 1) / Tile::KB;
+  int t = blockIdx.x;
+  if (t >= total || nch == 0) return;
+  auto coords = [&](int tt, long long* m0, int* n0, int* mlim, int* nlim) {
+    int rs, ct;
+    xcd_tile(tt, ncolt, nrows, &rs, &ct);
+    *m0 = (long long)rs * Tile::BM;
+    *n0 = ct * Tile::BN;
+    *mlim = (int)std::min<long long>(Tile::BM, ng - *m0);
+    *nlim = std::min(Tile::BN, rk - *n0);
+  };
+  double ra[Tile::APT], rb[Tile::BPT];
+  long long m0;
+  int n0, mlim, nlim;
+  coords(t, &m0, &n0, &mlim, &nlim);
+  Tile::load(A + m0, lda, B + n0, ldb, K, 0, mlim, nlim, ra, rb);
+  Tile::store(lds, 0, ra, rb);
+  __syncthreads();
+  while (true) {
+    const int tn = t + gridDim.x;
+    const bool has_next = tn < total;
+    long long m0n = 0;
+    int n0n = 0, mlimn = 0, nlimn = 0;
+    if (has_next) coords(tn, &m0n, &n0n, &mlimn, &nlimn);
+    typename Tile::Acc acc;
+    Tile::zero(acc);
+    for (int c = 0; c < nch; c++) {
+      const bool more = c + 1 < nch;
+      if (more)
+        Tile::load(A + m0, lda, B + n0, ldb, K, (c + 1) * Tile::KB, mlim, nlim, ra, rb);
+      else if (has_next)
+        Tile::load(A + m0n, lda, B + n0n, ldb, K, 0, mlimn, nlimn, ra, rb);
+      Tile::compute(lds, c & 1, acc);
+      if (more) Tile::store(lds, (c + 1) & 1, ra, rb);
+      __syncthreads();
+    }
+    Tile::epilogue(acc, [&](int m, int n, double v) {
+      if (m < mlim && n < nlim) C[(long long)(n0 + n) * ldc + m0 + m] = v * v;
+    });
+    if (!has_next) break;
+    // Every wave is past the last chunk's barrier: buffer 0 is free.
+    Tile::store(lds, 0, ra, rb);
+    __syncthreads();
+    t = tn;
+    m0 = m0n;
+    n0 = n0n;
+    mlim = mlimn;
+    nlim = nlimn;
+  }
+}
+
 // trace summed in ascending order, eps = 1e-10 * trace / rk added to the
 // diagonal (getbasis_scalapack.cpp:398-428).  One block.
 template <bool BATCH>
@@ -546,7 +611,20 @@ int launch_gather_psimu(const double* psi, long long ld, int nstate, const int* 
   return 0;
 }
 
-int zct_variants() { return 3; }
+int zct_variants() { return 4; }
+
+int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      v = 256;
+    cus[dev] = v;
+  }
+  return cus[dev];
+}
 
 int zct_pick_variant(long long ng, int requested) {
   return requested >= 0 ? requested : (ng >= 256LL * 256 ? 0 : 2);
@@ -566,7 +644,12 @@ int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, in
   if (!dual) {
     if (variant == 0) ISDF_ZCT_LAUNCH(ZctTile0, false);
     else if (variant == 1) ISDF_ZCT_LAUNCH(ZctTile1, false);
-    else ISDF_ZCT_LAUNCH(ZctTile2, false);
+    else if (variant == 3) {
+      const long long total = (long long)cdiv(ng, ZctTile0::BM) * cdiv(rk, ZctTile0::BN);
+      const int grid = (int)std::min<long long>(total, std::max(8, device_cus() / 8 * 8));
+      hipLaunchKernelGGL(k_zct_persistent<ZctTile0>, dim3(grid), dim3(ZctTile0::NT), 0, st, ng,
+                         rk, A1, ld, K1, B1, rk, C, ldc);
+    } else ISDF_ZCT_LAUNCH(ZctTile2, false);
   } else {
     if (variant == 2) ISDF_ZCT_LAUNCH(ZctTileDual1, true);
     else ISDF_ZCT_LAUNCH(ZctTileDual0, true);

@@ -658,13 +658,15 @@ __global__ __launch_bounds__(256) void k_centroid_sums_box(
       }
       __syncthreads();
       if (t < 4) {
+        // Eight terms fetched ahead of the adds, which stay in sequence.
         const double* qv = buf[t];
         int e = 0;
-        for (; e + 3 < tot; e += 4) {
-          acc = acc + qv[e];
-          acc = acc + qv[e + 1];
-          acc = acc + qv[e + 2];
-          acc = acc + qv[e + 3];
+        for (; e + 7 < tot; e += 8) {
+          double v[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) v[u] = qv[e + u];
+#pragma unroll
+          for (int u = 0; u < 8; u++) acc = acc + v[u];
         }
         for (; e < tot; e++) acc = acc + qv[e];
       }

@@ -34,7 +34,12 @@ struct MfmaTile {
   static constexpr int TN = BN / WGN / 16;
   static constexpr int APT = BM * KB / NT;  // A elements each thread stages per chunk
   static constexpr int BPT = BN * KB / NT;
-  static constexpr int LDS_DOUBLES = 2 * KB * (BM + BN);
+  // LDS row strides (doubles), 16 mod 32.  The four lane groups of a wave read
+  // four k rows at once; a stride of 128 bytes past a multiple of 256 puts
+  // them on disjoint banks instead of the same ones.
+  static constexpr int AS = BM % 32 == 0 ? BM + 16 : BM;
+  static constexpr int BS = BN % 32 == 0 ? BN + 16 : BN;
+  static constexpr int LDS_DOUBLES = 2 * KB * (AS + BS);
   static_assert(TM >= 1 && TN >= 1, "tile too small for the wave grid");
   static_assert(BM * KB % NT == 0 && BN * KB % NT == 0, "chunk must split evenly");
   static_assert(KB % 4 == 0, "KB must be a multiple of 4");
@@ -78,25 +83,28 @@ struct MfmaTile {
 
   __device__ static void store(double* lds, int buf, const double (&ra)[APT],
                                const double (&rb)[BPT]) {
-    double* As = lds + buf * KB * (BM + BN);
-    double* Bs = As + KB * BM;
+    double* As = lds + buf * KB * (AS + BS);
+    double* Bs = As + KB * AS;
     const int t = threadIdx.x;
 #pragma unroll
-    for (int q = 0; q < APT; q++) As[t + q * NT] = ra[q];  // As[kk][m]
+    for (int q = 0; q < APT; q++) {
+      const int e = t + q * NT;
+      As[(e / BM) * AS + e % BM] = ra[q];  // As[kk][m]
+    }
 #pragma unroll
     for (int q = 0; q < BPT; q++) {
       const int e = t + q * NT;
       if constexpr (B_KMAJOR) {
-        Bs[e] = rb[q];  // Bs[kk][n]
+        Bs[(e / BN) * BS + e % BN] = rb[q];  // Bs[kk][n]
       } else {
-        Bs[(e % KB) * BN + e / KB] = rb[q];
+        Bs[(e % KB) * BS + e / KB] = rb[q];
       }
     }
   }
 
   __device__ static void compute(const double* lds, int buf, Acc& acc) {
-    const double* As = lds + buf * KB * (BM + BN);
-    const double* Bs = As + KB * BM;
+    const double* As = lds + buf * KB * (AS + BS);
+    const double* Bs = As + KB * AS;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave % WGM, wn = wave / WGM;
     const int lr = lane & 15, lk = lane >> 4;
@@ -106,9 +114,9 @@ struct MfmaTile {
     for (int kk = 0; kk < KB; kk += 4) {
       double a[TM], b[TN];
 #pragma unroll
-      for (int i = 0; i < TM; i++) a[i] = Aw[(kk + lk) * BM + 16 * i];
+      for (int i = 0; i < TM; i++) a[i] = Aw[(kk + lk) * AS + 16 * i];
 #pragma unroll
-      for (int j = 0; j < TN; j++) b[j] = Bw[(kk + lk) * BN + 16 * j];
+      for (int j = 0; j < TN; j++) b[j] = Bw[(kk + lk) * BS + 16 * j];
 #pragma unroll
       for (int i = 0; i < TM; i++)
 #pragma unroll
