@@ -366,7 +366,11 @@ __device__ double block_min256(double v, double* red) {
   return r;
 }
 
-constexpr int kCandCap = 256;
+// Candidate-list capacity of the assignment: 256 once the centroids have
+// spread; 1024 for the first iterations, when the initial centroids (the first
+// rk grid points, one corner of the grid) leave hundreds of candidates per
+// brick.  A brick with more falls back to testing every centroid.
+constexpr int kCandCap = 256, kCandCapWide = 1024;
 
 // Fused pdist2 + findMin(Dim=1) (utility.cpp:598-615, :535-562) with
 // conservative culling.  A workgroup takes one block of up to 256 points,
@@ -379,14 +383,14 @@ constexpr int kCandCap = 256;
 // the reference's first minimum over all centroids.
 // In grid mode the workgroup also folds its points into the bounding boxes
 // (grid indices) of their clusters, which the centroid sums walk.
-template <bool BATCH>
+template <bool BATCH, int CAP>
 __global__ __launch_bounds__(256) void k_assign_culled(
     const int* __restrict__ porder, const double* __restrict__ X, const double* __restrict__ Y,
     const double* __restrict__ Z, const double* __restrict__ cx, const double* __restrict__ cy,
     const double* __restrict__ cz, int* __restrict__ label, int* scal, int count_changes,
     int* __restrict__ bbox, int rkmax, int grid_mode, int n0, int n1,
     int* __restrict__ changed_blk, const KmJob* jobs) {
-  __shared__ int cand[kCandCap];
+  __shared__ int cand[CAP];
   if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     if ((int)blockIdx.x >= J.b.nblk_pts) return;
@@ -406,8 +410,8 @@ __global__ __launch_bounds__(256) void k_assign_culled(
     n1 = J.b.ngrid[1];
     changed_blk = J.b.changed_blk;
   }
-  __shared__ double ccx[kCandCap], ccy[kCandCap], ccz[kCandCap];
-  __shared__ int cbox[6][kCandCap];
+  __shared__ double ccx[CAP], ccy[CAP], ccz[CAP];
+  __shared__ int cbox[6][CAP];
   __shared__ double red[8];
   __shared__ double rbox[6][4];
   if (count_changes && scal[KS_ACTIVE] == 0) return;
@@ -468,7 +472,7 @@ __global__ __launch_bounds__(256) void k_assign_culled(
     int pos = block_excl_scan(mycnt, &nc);
     for (int c = c0; c < c1; c++) {
       if (!((keepmask >> (c - c0)) & 1ull)) continue;
-      if (pos < kCandCap) {
+      if (pos < CAP) {
         cand[pos] = c;
         ccx[pos] = cx[c];
         ccy[pos] = cy[c];
@@ -477,7 +481,7 @@ __global__ __launch_bounds__(256) void k_assign_culled(
       pos++;
     }
   }
-  const bool culled = nc <= kCandCap;
+  const bool culled = nc <= CAP;
   if (culled && grid_mode) {
     for (int k = t; k < nc; k += 256) {
       cbox[0][k] = 0x7fffffff;
@@ -1304,7 +1308,8 @@ int kmeans_setup_points(KmeansBufs* b, const int ngrid[3], const double length[3
   hipLaunchKernelGGL(k_reset_boxes<false>, dim3(cdiv(6 * (long long)b->rkmax, 256)), dim3(256), 0, st,
                      b->bbox, b->rkmax, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_reset_boxes");
-  hipLaunchKernelGGL(k_assign_culled<false>, dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X, b->Y,
+  hipLaunchKernelGGL((k_assign_culled<false, kCandCapWide>), dim3(b->nblk_pts), dim3(256), 0, st,
+                     b->porder, b->X, b->Y,
                      b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 0, b->bbox, b->rkmax,
                      b->grid_mode ? 1 : 0, ngrid[0], ngrid[1], b->changed_blk, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_assign_culled");
@@ -1348,9 +1353,14 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
     hipLaunchKernelGGL(k_update<false>, dim3(1), dim3(1024), 0, st, b->sw, b->sx, b->sy, b->sz, b->cx,
                        b->cy, b->cz, b->map, b->scal, b->grid_mode ? b->bbox : (int*)nullptr,
                        b->rkmax, (const KmJob*)nullptr);
-    hipLaunchKernelGGL(k_assign_culled<false>, dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X,
-                       b->Y, b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 1, b->bbox, b->rkmax,
-                       b->grid_mode ? 1 : 0, b->ngrid[0], b->ngrid[1], b->changed_blk, (const KmJob*)nullptr);
+    if (s < 3)
+      hipLaunchKernelGGL((k_assign_culled<false, kCandCapWide>), dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X,
+                         b->Y, b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 1, b->bbox, b->rkmax,
+                         b->grid_mode ? 1 : 0, b->ngrid[0], b->ngrid[1], b->changed_blk, (const KmJob*)nullptr);
+    else
+      hipLaunchKernelGGL((k_assign_culled<false, kCandCap>), dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X,
+                         b->Y, b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 1, b->bbox, b->rkmax,
+                         b->grid_mode ? 1 : 0, b->ngrid[0], b->ngrid[1], b->changed_blk, (const KmJob*)nullptr);
     hipLaunchKernelGGL(k_control<false>, dim3(1), dim3(1024), 0, st, s, max_iter, maxF, b->nblk_pts,
                        b->changed_blk, b->trace, b->scal, (const KmJob*)nullptr);
     ISDF_HIP_LAUNCHED("k-means iteration");
@@ -1443,7 +1453,8 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
                      (int*)nullptr, J);
   hipLaunchKernelGGL(k_reset_boxes<true>, dim3(cdiv(6LL * maxrk, 256), nb), dim3(256), 0, st,
                      (int*)nullptr, 0, J);
-  hipLaunchKernelGGL(k_assign_culled<true>, dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
+  hipLaunchKernelGGL((k_assign_culled<true, kCandCapWide>), dim3(maxpts, nb), dim3(256), 0, st,
+                     (const int*)nullptr,
                      (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                      (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                      (int*)nullptr, (int*)nullptr, 0, (int*)nullptr, 0, 1, 0, 0, (int*)nullptr, J);
@@ -1460,11 +1471,18 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
                        (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                        (double*)nullptr, (double*)nullptr, (double*)nullptr, (int*)nullptr,
                        (int*)nullptr, (int*)nullptr, 0, J);
-    hipLaunchKernelGGL(k_assign_culled<true>, dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
-                       (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
-                       (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
-                       (int*)nullptr, (int*)nullptr, 1, (int*)nullptr, 0, 1, 0, 0, (int*)nullptr,
-                       J);
+    if (s < 3)
+      hipLaunchKernelGGL((k_assign_culled<true, kCandCapWide>), dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
+                         (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
+                         (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
+                         (int*)nullptr, (int*)nullptr, 1, (int*)nullptr, 0, 1, 0, 0, (int*)nullptr,
+                         J);
+    else
+      hipLaunchKernelGGL((k_assign_culled<true, kCandCap>), dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
+                         (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
+                         (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
+                         (int*)nullptr, (int*)nullptr, 1, (int*)nullptr, 0, 1, 0, 0, (int*)nullptr,
+                         J);
     hipLaunchKernelGGL(k_control<true>, dim3(1, nb), dim3(1024), 0, st, s, 0, 0.0, 0,
                        (const int*)nullptr, (int*)nullptr, (int*)nullptr, J);
     ISDF_HIP_LAUNCHED("k-means iteration (batched)");

@@ -303,11 +303,35 @@ __global__ __launch_bounds__(256) void k_pack_rows(const double* __restrict__ ps
     tile[kk][gg] = (g < ng && k < K) ? (float)psi[(long long)(c0 + k) * ld + g] : 0.f;
   }
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int gg = (t >> 5) + 8 * q, kk = t & 31;
+  if constexpr (SPLIT) {
+    // Thread t splits 8 consecutive k of row t/4 and writes the three 16-byte
+    // plane pieces of the row's 192-byte chunk.
+    const int gg = t >> 2, k8 = (t & 3) * 8;
     const long long g = g0 + gg;
-    if (g < ng) put_operand<SPLIT>(out, kp, g, k0 + kk, tile[kk][gg]);
+    if (g < ng) {
+      bf8 h, m, l;
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const float v = tile[k8 + u][gg];
+        const __bf16 a = (__bf16)v;
+        const float r1 = v - (float)a;
+        const __bf16 b = (__bf16)r1;
+        h[u] = a;
+        m[u] = b;
+        l[u] = (__bf16)(r1 - (float)b);
+      }
+      __bf16* o = (__bf16*)out + g * 3 * kp + (k0 / 32) * 96 + k8;
+      *(bf8*)o = h;
+      *(bf8*)(o + 32) = m;
+      *(bf8*)(o + 64) = l;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int gg = (t >> 5) + 8 * q, kk = t & 31;
+      const long long g = g0 + gg;
+      if (g < ng) put_operand<SPLIT>(out, kp, g, k0 + kk, tile[kk][gg]);
+    }
   }
 }
 
