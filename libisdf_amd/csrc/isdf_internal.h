@@ -236,9 +236,14 @@ int launch_cct_lp(const float* Zt, int ldz, const int* piv, int rk, double* P, d
                   hipStream_t st);
 // W = (L L^T)^{-1} (FP64, column-major rk x rk) from the forward-solve operand
 // (Gt is overwritten: it holds L^{-T} on the way), and operand rows nu of
-// W(k, nu), k < ldw (zero for k >= rk), in the mode's format.
-int launch_inverse(int mode, int rk, const double* Ft, double* Gt, double* W, int ldw,
+// W(k, nu), k < ldw (zero for k >= rk), in operand format wmode (1 FP32, 2
+// pre-split bf16 planes).
+int launch_inverse(int wmode, int rk, const double* Ft, double* Gt, double* W, int ldw,
                    void* Wf, hipStream_t st);
+// Theta tile variant (-1: the default of the mode) and the W operand format it
+// reads (the wmode of launch_inverse): BF16X3 variant 3 reads FP32 W.
+int lp_theta_variant(int mode, int variant);
+int lp_theta_w_mode(int mode, int variant);
 // theta(g, nu) = (or +=) sum_k Wf[nu][k] Zt[g][k], k < ldk.
 int launch_theta_lp(int mode, int ng, int rk, int ldk, const void* Wf, const float* Zt,
                     double* theta, long long ldt, bool add, int variant, hipStream_t st);

@@ -40,6 +40,9 @@ namespace {
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
@@ -66,8 +69,9 @@ __device__ __forceinline__ void split3(f4 x, bf4& h, bf4& m, bf4& l) {
 //     plane p of X(r, k) at ((const __bf16*)X)[r*ld + (k/32)*96 + p*32 + k%32]
 //     (ld = 3 K), copied to the planes unchanged: no conversion work in the loop.
 // LDS planes: 64-byte rows of four 16-byte chunks, chunk c of row r stored at
-// chunk c ^ ((r >> 2) & 3), so the 16 rows a lane group reads at one k chunk
-// cover all 64 banks once.
+// chunk c ^ ((r >> 2) & 3).  (Measured on gfx950, the fragment reads of this
+// layout take two LDS cycles per ds_read_b128 instead of one; LpDmaTile's
+// 128-byte rows are conflict-free.)
 template <int ROWS, int NT, bool BF16, bool PRE>
 struct LpOperand {
   static_assert(BF16 || !PRE, "pre-split operands are bf16");
@@ -91,9 +95,11 @@ struct LpOperand {
     for (int q = 0; q < PT; q++) {
       const int e = t + q * NT;
       if constexpr (PRE) {
-        const int r = e / 12, pc = e % 12;
+        // Plane-major: 16 consecutive lanes store four whole 64-byte rows of
+        // one plane (all 64 banks once) rather than three planes of one row.
+        const int p = e / (ROWS * 4), r = (e >> 2) % ROWS, c16 = e & 3;
         reg[q] = r < lim ? *(const i4*)((const __bf16*)X + (long long)r * ld + (k0 / 32) * 96 +
-                                        8 * pc)
+                                        32 * p + 8 * c16)
                          : i4{0, 0, 0, 0};
       } else {
         const int r = e >> 3, k = k0 + 4 * (e & 7);
@@ -109,7 +115,7 @@ struct LpOperand {
     for (int q = 0; q < PT; q++) {
       const int e = t + q * NT;
       if constexpr (PRE) {
-        const int r = e / 12, pc = e % 12, p = pc >> 2, c16 = pc & 3;
+        const int p = e / (ROWS * 4), r = (e >> 2) % ROWS, c16 = e & 3;
         __bf16* P = (__bf16*)base + p * ROWS * RS16;
         *(i4*)(P + r * RS16 + 8 * (c16 ^ ((r >> 2) & 3))) = reg[q];
       } else if constexpr (BF16) {
@@ -238,6 +244,179 @@ struct LpTile {
       }
       __syncthreads();
     }
+  }
+
+  __device__ static int row(int i, int r) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    return (wave % WGM) * (TM * 16) + 16 * i + 4 * (lane >> 4) + r;
+  }
+  __device__ static int col(int j) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    return (wave / WGM) * (TN * 16) + 16 * j + (lane & 15);
+  }
+};
+
+// BF16X3 tile with both operands FP32 in HBM, moved HBM -> LDS by
+// direct-to-LDS loads (global_load_lds_dwordx4, 16 bytes per lane, lane L of
+// a wave landing at byte 16 L of a 1 KB LDS block), into a ring of STAGES
+// LDS buffers (STAGES-1 k chunks in flight while one is multiplied).  With no
+// staging registers a 256 x 256 tile fits in 256 VGPRs (two waves per SIMD),
+// which halves the W traffic per flop of the 256 x 128 LpTile; measured on the
+// 96^3 workload, a third stage gains nothing (the loads are not the limit).
+// Each operand row of a chunk (32 FP32) is one 128-byte LDS row
+// whose 16-byte piece c sits at position c ^ sw(r); the fragment reads (16
+// lanes on 16 consecutive rows, one ds_read_b128 each) measured free of bank
+// conflicts in that layout on gfx950.  Fragments are split into their bf16
+// parts h/m/l (the arithmetic of split3) right after the read, so the products
+// and their order match LpTile's BF16X3 mode bit for bit.
+//   Operand rows are FP32 with row stride ld (a multiple of 4), K a multiple of
+//   32; rows >= mlim / nlim are read as row mlim-1 / nlim-1 (their outputs are
+//   not stored), so every load stays inside the operands.
+template <int BM_, int BN_, int WGM_, int WGN_, int STAGES_ = 2>
+struct LpDmaTile {
+  static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_, STAGES = STAGES_;
+  static constexpr bool BF16 = true, A_PRE = false, B_PRE = false;
+  static constexpr int NW = WGM * WGN, NT = 64 * NW;
+  static constexpr int TM = BM / WGM / 16, TN = BN / WGN / 16;
+  static constexpr int KB = 32, ROWB = 128;
+  static_assert(STAGES >= 2, "stages");
+  static constexpr int A_BYTES = BM * ROWB, STAGE_BYTES = (BM + BN) * ROWB;
+  static constexpr int LDS_BYTES = STAGES * STAGE_BYTES;
+  static constexpr int A_PW = BM / 8 / NW, B_PW = BN / 8 / NW;  // 1 KB blocks per wave
+  static constexpr int NDMA = A_PW + B_PW;                      // loads per wave per chunk
+  static_assert(A_PW * 8 * NW == BM && B_PW * 8 * NW == BN, "DMA blocks per wave");
+  static_assert(LDS_BYTES <= 160 * 1024, "LDS");
+  static_assert(NDMA * 2 <= 63 && STAGES <= 4, "vmcnt range");
+  typedef f4 Acc[TM][TN];
+  typedef __attribute__((address_space(3))) void* LdsPtr;
+
+  __device__ static int sw(int r) { return ((r & 3) << 1) | ((r >> 2) & 1); }
+
+  __device__ static void zero(Acc& acc) {
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+      for (int j = 0; j < TN; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  }
+
+  // Chunk at k0 of both operands -> stage buffer `stage` (NDMA loads per wave).
+  __device__ static void issue(const float* __restrict__ A, long long lda,
+                               const float* __restrict__ B, long long ldb, int k0, int mlim,
+                               int nlim, char* stage) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, rl = lane >> 3;
+    const int c4 = 4 * ((lane & 7) ^ sw(rl));
+#pragma unroll
+    for (int j = 0; j < A_PW; j++) {
+      const int blk = wave + NW * j, r = std::min(blk * 8 + rl, mlim - 1);
+      __builtin_amdgcn_global_load_lds((const void*)(A + (long long)r * lda + k0 + c4),
+                                       (LdsPtr)(stage + blk * 1024), 16, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < B_PW; j++) {
+      const int blk = wave + NW * j, r = std::min(blk * 8 + rl, nlim - 1);
+      __builtin_amdgcn_global_load_lds((const void*)(B + (long long)r * ldb + k0 + c4),
+                                       (LdsPtr)(stage + A_BYTES + blk * 1024), 16, 0, 0);
+    }
+  }
+
+  // This wave's loads except the newest N have landed, its LDS reads are
+  // done, and so are every other wave's (the barrier).  The asm is also a
+  // compiler memory barrier, so no LDS access moves across it.
+  template <int N>
+  __device__ static void wait_barrier() {
+    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
+  }
+
+  // split3 on a pair: one two-source bf16 conversion per part, the parts
+  // widened back to FP32 by shift / mask of the packed pair.
+  __device__ static void split2(float x0, float x1, unsigned& h, unsigned& m, unsigned& l) {
+    h = __builtin_bit_cast(unsigned, __builtin_convertvector(f2{x0, x1}, bf2));
+    const float r0 = x0 - __uint_as_float(h << 16), r1 = x1 - __uint_as_float(h & 0xffff0000u);
+    m = __builtin_bit_cast(unsigned, __builtin_convertvector(f2{r0, r1}, bf2));
+    l = __builtin_bit_cast(unsigned, __builtin_convertvector(
+                                         f2{r0 - __uint_as_float(m << 16),
+                                            r1 - __uint_as_float(m & 0xffff0000u)},
+                                         bf2));
+  }
+
+  __device__ static void split8(f4 x0, f4 x1, bf8& h, bf8& m, bf8& l) {
+    unsigned h0, h1, h2, h3, m0, m1, m2, m3, l0, l1, l2, l3;
+    split2(x0[0], x0[1], h0, m0, l0);
+    split2(x0[2], x0[3], h1, m1, l1);
+    split2(x1[0], x1[1], h2, m2, l2);
+    split2(x1[2], x1[3], h3, m3, l3);
+    const u4 hh = {h0, h1, h2, h3}, mm = {m0, m1, m2, m3}, ll = {l0, l1, l2, l3};
+    h = __builtin_bit_cast(bf8, hh);
+    m = __builtin_bit_cast(bf8, mm);
+    l = __builtin_bit_cast(bf8, ll);
+  }
+
+  __device__ static void compute(const char* stage, Acc& acc) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave % WGM, wn = wave / WGM;
+    const int lr = lane & 15, lk = lane >> 4, s = sw(lr);
+    const int p0 = 16 * ((2 * lk) ^ s), p1 = 16 * ((2 * lk + 1) ^ s);
+    const char* Aw = stage + (wm * TM * 16 + lr) * ROWB;
+    const char* Bw = stage + A_BYTES + (wn * TN * 16 + lr) * ROWB;
+    bf8 a[3][TM];
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+      split8(*(const f4*)(Aw + 16 * ROWB * i + p0), *(const f4*)(Aw + 16 * ROWB * i + p1),
+             a[0][i], a[1][i], a[2][i]);
+    // One B fragment at a time (its three parts live only for its column of
+    // accumulators); each accumulator still sees the six products in LpTile's
+    // order, so the result is the same bit for bit.
+    constexpr int PA[6] = {0, 2, 1, 0, 1, 0};
+    constexpr int PB[6] = {2, 0, 1, 1, 0, 0};
+#pragma unroll
+    for (int j = 0; j < TN; j++) {
+      bf8 b[3];
+      split8(*(const f4*)(Bw + 16 * ROWB * j + p0), *(const f4*)(Bw + 16 * ROWB * j + p1), b[0],
+             b[1], b[2]);
+#pragma unroll
+      for (int p = 0; p < 6; p++)
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[PA[p]][i], b[PB[p]], acc[i][j],
+                                                              0, 0, 0);
+    }
+  }
+
+  // acc += A[0:mlim, 0:K] B[0:nlim, 0:K]^T.  Ends with a workgroup barrier.
+  __device__ static void run(const void* __restrict__ Av, long long lda,
+                             const void* __restrict__ Bv, long long ldb, int K, int mlim,
+                             int nlim, char* lds, Acc& acc) {
+    const float* A = (const float*)Av;
+    const float* B = (const float*)Bv;
+    const int nch = K / KB;
+    if (nch == 0) return;
+    // Chunks 0 .. STAGES-2 in flight before the loop; iteration c waits for
+    // chunk c, then refills the stage freed by chunk c-1 with chunk c+STAGES-1.
+    constexpr int AHEAD = STAGES - 1;
+#pragma unroll
+    for (int c = 0; c < AHEAD; c++)
+      if (c < nch) issue(A, lda, B, ldb, c * KB, mlim, nlim, lds + c * STAGE_BYTES);
+    int cur = 0, nxt = AHEAD;  // stage of chunk c, stage of chunk c + AHEAD
+    for (int c = 0; c < nch; c++) {
+      // Loads issued after chunk c's: chunks c+1 .. min(c+AHEAD-1, nch-1).
+      const int later = std::min(AHEAD - 1, nch - 1 - c);
+      if constexpr (AHEAD >= 3) {
+        if (later >= 2) wait_barrier<2 * NDMA>();
+        else if (later == 1) wait_barrier<NDMA>();
+        else wait_barrier<0>();
+      } else if constexpr (AHEAD == 2) {
+        if (later >= 1) wait_barrier<NDMA>();
+        else wait_barrier<0>();
+      } else {
+        wait_barrier<0>();
+      }
+      if (c + AHEAD < nch) issue(A, lda, B, ldb, (c + AHEAD) * KB, mlim, nlim, lds + nxt * STAGE_BYTES);
+      compute(lds + cur * STAGE_BYTES, acc);
+      cur = cur == STAGES - 1 ? 0 : cur + 1;
+      nxt = nxt == STAGES - 1 ? 0 : nxt + 1;
+    }
+    wait_barrier<0>();
   }
 
   __device__ static int row(int i, int r) {
@@ -629,6 +808,7 @@ typedef LpTile<128, 128, 2, 2, true, true, true> LpZB16_2;  // 4 waves
 typedef LpTile<256, 128, 4, 2, true, true, false> LpTB16_0;
 typedef LpTile<128, 256, 2, 4, true, true, false> LpTB16_1;
 typedef LpTile<128, 128, 2, 2, true, true, false> LpTB16_2;
+typedef LpDmaTile<256, 256, 4, 2> LpTB16D_3;  // FP32 W and Zt, 8 waves of 64 x 128
 
 }  // namespace
 
@@ -701,7 +881,7 @@ int launch_cct_lp(const float* Zt, int ldz, const int* piv, int rk, double* P, d
   return launch_trace_eps(P, rk, eps_dev, st);
 }
 
-int launch_inverse(int mode, int rk, const double* Ft, double* Gt, double* W, int ldw,
+int launch_inverse(int wmode, int rk, const double* Ft, double* Gt, double* W, int ldw,
                    void* Wf, hipStream_t st) {
   // X = I L^{-T} in Gt (the backward-solve operand is not needed by the mixed
   // forms): the forward pass of the blocked solve with its 16-row tile (rk/16
@@ -711,7 +891,7 @@ int launch_inverse(int mode, int rk, const double* Ft, double* Gt, double* W, in
   if (int rc = launch_potrs(Gt, rk, rk, rk, Ft, Gt, 5, st, true)) return rc;
   if (int rc = launch_xxt(Gt, rk, W, st)) return rc;
   const dim3 grid(cdiv((long long)rk * ldw, 256));
-  if (mode == 2)
+  if (wmode == 2)
     hipLaunchKernelGGL(k_pack_w<true>, grid, dim3(256), 0, st, W, rk, ldw, Wf);
   else
     hipLaunchKernelGGL(k_pack_w<false>, grid, dim3(256), 0, st, W, rk, ldw, Wf);
@@ -719,10 +899,18 @@ int launch_inverse(int mode, int rk, const double* Ft, double* Gt, double* W, in
   return 0;
 }
 
+int lp_theta_variant(int mode, int variant) {
+  return variant >= 0 ? variant : mode == 1 ? 2 : 3;  // measured best on the 96^3 workload
+}
+
+int lp_theta_w_mode(int mode, int variant) {
+  return mode == 2 && lp_theta_variant(mode, variant) < 3 ? 2 : 1;
+}
+
 int launch_theta_lp(int mode, int ng, int rk, int ldk, const void* Wf, const float* Zt,
                     double* theta, long long ldt, bool add, int variant, hipStream_t st) {
-  if (variant < 0) variant = mode == 1 ? 2 : 0;  // measured best on the 96^3 workload
-  const long long ldwr = mode == 2 ? 3LL * ldk : ldk;
+  const long long ldwr = lp_theta_w_mode(mode, variant) == 2 ? 3LL * ldk : ldk;
+  variant = lp_theta_variant(mode, variant);
 #define ISDF_TLP(TL)                                                                            \
   do {                                                                                          \
     const unsigned nb = cdiv(ng, TL::BN) * cdiv(rk, TL::BM);                                    \
@@ -740,6 +928,7 @@ int launch_theta_lp(int mode, int ng, int rk, int ldk, const void* Wf, const flo
   } else {
     if (variant == 1) ISDF_TLP(LpTB16_1);
     else if (variant == 2) ISDF_TLP(LpTB16_2);
+    else if (variant == 3) ISDF_TLP(LpTB16D_3);
     else ISDF_TLP(LpTB16_0);
   }
 #undef ISDF_TLP

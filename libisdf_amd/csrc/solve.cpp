@@ -212,7 +212,9 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
   ISDF_HIP_TRY(hipMemsetAsync(info_dev, 0, sizeof(int), s2));
   if (int rc = launch_potrf(P, rk, Dinv, info_dev, s2)) return rc;
   if (int rc = launch_make_fg(P, rk, Dinv, Dinv128, Ft, Gt, s2)) return rc;
-  if (int rc = launch_inverse(mode, rk, Ft, Gt, m.W, ldz, m.Wf, s2)) return rc;
+  const int tv = env_variant("ISDF_LP_THETA_TILE");
+  if (int rc = launch_inverse(lp_theta_w_mode(mode, tv), rk, Ft, Gt, m.W, ldz, m.Wf, s2))
+    return rc;
   if (int rc = ev.rec(8, s2)) return rc;
   ISDF_HIP_TRY(hipEventRecord(aux->join, s2));
 
@@ -224,7 +226,6 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
   if (int rc = ev.rec(5, st)) return rc;
   ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->join, 0));
   if (int rc = ev.rec(9, st)) return rc;
-  const int tv = env_variant("ISDF_LP_THETA_TILE");
   if (int rc = launch_theta_lp(mode, ng, rk, ldz, m.Wf, m.Zt, d_theta, ng, false, tv, st))
     return rc;
   if (refine) {

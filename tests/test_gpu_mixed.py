@@ -117,6 +117,31 @@ def test_mixed_repeatable_bitwise(cfg1):
         np.testing.assert_array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
+@pytest.mark.parametrize("refine", [0, isdf.ISDF_PREC_REFINE])
+@pytest.mark.parametrize("case", ["cfg1", "ragged"])
+def test_bf16x3_theta_tiles_bitwise_equal(cfg1, monkeypatch, case, refine):
+    # Every BF16X3 Theta tile runs the same six products per k chunk in the same
+    # order on the same bf16 parts (pre-split or split in registers), so the
+    # tile choice, including the three-stage direct-to-LDS tile (variant 3,
+    # FP32 W), must not change a bit.  The ragged case has partial row and
+    # column tiles (rk = 100, Ng = 1080).
+    if case == "cfg1":
+        psi, args = cfg1[1], cfg1[2]
+    else:
+        grid, L = (10, 12, 9), (4.1, 5.0, 3.7)
+        ng = grid[0] * grid[1] * grid[2]
+        psi = synth.cholesky_qr2(np.random.default_rng(77).standard_normal((22, ng)),
+                                 L[0] * L[1] * L[2] / ng)
+        args = (L, grid, 11, 22, 9, 7, 5, 11, 100, 40)
+    out = {}
+    for v in (0, 1, 2, 3):
+        monkeypatch.setenv("ISDF_LP_THETA_TILE", str(v))
+        out[v] = run(psi, *args, isdf.ISDF_PREC_BF16X3 | refine)[0]
+    assert np.isfinite(out[0]).all()
+    for v in (1, 2, 3):
+        np.testing.assert_array_equal(out[v].view(np.uint64), out[0].view(np.uint64))
+
+
 def test_mixed_ill_conditioned_raises_not_spd(si8):
     # Si8 has kappa(C C^T) ~ 2e11 (16 states give 136 pair products for rk =
     # 192).  The FP32 contraction's rounding (~1e-7 relative) exceeds the eps
