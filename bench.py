@@ -33,9 +33,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_potrs launch from two rocprofv3 counter passes (FETCH_SIZE x 2.000,
 # WRITE_SIZE x 1.000, factors from a copy kernel of known size), per workload.
 TRAFFIC = {
-    "cfg2": {"bytes": 2.487e10,
-             "source": "profiles/k_potrs_k_zct_cfg2_fp64_hbm_bytes.md (20.49 GB read + "
-                       "4.383 GB written per launch)"},
+    "cfg2": {"bytes": 2.490e10,
+             "source": "profiles/k_potrs_k_zct_cfg2_fp64_hbm_bytes.md (20.56 GB read + "
+                       "4.341 GB written per launch)"},
 }
 
 
