@@ -111,11 +111,33 @@ struct Events {
 // Per-device helper stream (default priority) and fork/join events.  The
 // C C^T -> Cholesky -> solve-operand branch depends only on the pivots, not
 // on the Ng x rk Z C^T, so it runs on the helper stream beside that GEMM.
+//
+// Also pinned host staging for what a solve reads back (piv, the K-means
+// iteration count, the unique-center count, info, eps): copies into pinned
+// memory stay asynchronous, so the whole solve is queued before the host
+// waits once at the end (a copy into pageable memory would block the host and
+// leave the GPU idle until the rest of the solve is queued).
 struct Aux {
   hipStream_t s = nullptr;
   hipEvent_t fork = nullptr, join = nullptr, pack = nullptr;
+  int* hpiv = nullptr;       // pinned, hpiv_cap ints
+  size_t hpiv_cap = 0;
+  long long* hscal = nullptr;  // pinned: [0] iters, [1] rku, [2] info, [3] eps bits
 };
 std::vector<Aux> g_aux;
+
+int aux_pinned(Aux* a, size_t ng) {
+  if (!a->hscal)
+    ISDF_HIP_TRY(hipHostMalloc((void**)&a->hscal, 8 * sizeof(long long), hipHostMallocDefault));
+  if (a->hpiv_cap < ng) {
+    if (a->hpiv) ISDF_HIP_TRY(hipHostFree(a->hpiv));
+    a->hpiv = nullptr;
+    a->hpiv_cap = 0;
+    ISDF_HIP_TRY(hipHostMalloc((void**)&a->hpiv, sizeof(int) * ng, hipHostMallocDefault));
+    a->hpiv_cap = ng;
+  }
+  return 0;
+}
 
 int aux_stream(Aux** out) {
   int dev = 0;
@@ -359,17 +381,24 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   if (int rc = ev.rec(1, st)) return rc;
 
   // 2. K-means (utility.cpp:631-988).
+  // npt is read back (one wait: the degenerate case is handled on the host);
+  // after that nothing is read until the end of the solve.  All max_iter
+  // Lloyd iterations are queued; after convergence they return at once.
   int npt = 0, iters = 0, rku = rk;
+  if (int rc = aux_pinned(aux, (size_t)ng)) return rc;
   if (int rc = kmeans_prepare(&kb, p->ngrid, p->length, 1e-8, &npt, st)) return rc;
   bool degenerate = npt < rk;
   if (!degenerate) {
     if (int rc = kmeans_setup_points(&kb, p->ngrid, p->length, npt, rk, st)) return rc;
-    if (int rc = kmeans_iterate(&kb, rk, max_iter, 1e-5, &iters, st)) return rc;
+    if (int rc = kmeans_iterate(&kb, rk, max_iter, 1e-5, nullptr, st)) return rc;
     if (int rc = ev.rec(2, st)) return rc;
-    if (int rc = kmeans_finish(&kb, piv_dev, st)) return rc;
-    ISDF_HIP_TRY(hipMemcpyAsync(h_piv, piv_dev, sizeof(int) * (size_t)ng, hipMemcpyDeviceToHost,
-                                st));
-    ISDF_HIP_TRY(hipMemcpyAsync(&rku, kb.scal + KS_RKU, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (int rc = kmeans_finish(&kb, piv_dev, st, true)) return rc;
+    ISDF_HIP_TRY(hipMemcpyAsync(aux->hpiv, piv_dev, sizeof(int) * (size_t)ng,
+                                hipMemcpyDeviceToHost, st));
+    ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[0], kb.scal + KS_ITERS, sizeof(int),
+                                hipMemcpyDeviceToHost, st));
+    ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[1], kb.scal + KS_RKU, sizeof(int),
+                                hipMemcpyDeviceToHost, st));
   } else {
     // Fewer points than rk above the density threshold: the reference fills piv
     // from the threshold list and shuffles the tail (utility.cpp:667-681).
@@ -405,9 +434,14 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   if (dbg && dbg->d_labels && !degenerate)
     ISDF_HIP_TRY(hipMemcpyAsync(dbg->d_labels, kb.label, sizeof(int) * ng,
                                 hipMemcpyDeviceToDevice, st));
-  if (dbg && dbg->h_trace && max_iter > 0 && !degenerate)
-    ISDF_HIP_TRY(hipMemcpyAsync(dbg->h_trace, kb.trace, sizeof(int) * std::max(iters, 0),
+  if (dbg && dbg->h_trace && max_iter > 0 && !degenerate) {
+    // Debug output: the iteration count is needed now (one extra wait).
+    int it = 0;
+    ISDF_HIP_TRY(hipMemcpyAsync(&it, kb.scal + KS_ITERS, sizeof(int), hipMemcpyDeviceToHost, st));
+    ISDF_HIP_TRY(hipStreamSynchronize(st));
+    ISDF_HIP_TRY(hipMemcpyAsync(dbg->h_trace, kb.trace, sizeof(int) * std::max(it, 0),
                                 hipMemcpyDeviceToHost, st));
+  }
 
   if (mixed) {
     // Psi(mu) operand rows (main stream), then the two branches.
@@ -458,11 +492,17 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
     if (int rc = ev.rec(10, st)) return rc;
   }
 
-  int info = 0;
-  double eps = 0.0;
-  ISDF_HIP_TRY(hipMemcpyAsync(&info, info_dev, sizeof(int), hipMemcpyDeviceToHost, st));
-  ISDF_HIP_TRY(hipMemcpyAsync(&eps, eps_dev, sizeof(double), hipMemcpyDeviceToHost, st));
+  ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[2], info_dev, sizeof(int), hipMemcpyDeviceToHost, st));
+  ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[3], eps_dev, sizeof(double), hipMemcpyDeviceToHost, st));
   ISDF_HIP_TRY(hipStreamSynchronize(st));
+  const int info = (int)aux->hscal[2];
+  double eps = 0.0;
+  std::memcpy(&eps, &aux->hscal[3], sizeof(double));
+  if (!degenerate) {
+    std::memcpy(h_piv, aux->hpiv, sizeof(int) * (size_t)ng);
+    iters = (int)aux->hscal[0];
+    rku = (int)aux->hscal[1];
+  }
 
   if (rep) {
     rep->rk = rk;
@@ -547,6 +587,11 @@ int isdf_hip_release_workspace(void) {
     if (hipFree(isdf_hip::g_ws[dev].ptr) != hipSuccess) return ISDF_EHIP;
     isdf_hip::g_ws[dev].ptr = nullptr;
     isdf_hip::g_ws[dev].bytes = 0;
+  }
+  if ((int)isdf_hip::g_aux.size() > dev && isdf_hip::g_aux[dev].hpiv) {
+    if (hipHostFree(isdf_hip::g_aux[dev].hpiv) != hipSuccess) return ISDF_EHIP;
+    isdf_hip::g_aux[dev].hpiv = nullptr;
+    isdf_hip::g_aux[dev].hpiv_cap = 0;
   }
   return ISDF_OK;
 }
