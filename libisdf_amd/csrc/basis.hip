@@ -330,11 +330,14 @@ __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int n
   for (int i = 0; i < kNB; i++) D[(long long)c * kNB + i] = (c < jb && i < jb) ? x[i] : 0.0;
 }
 
-typedef MfmaTile<128, 64, 16, 2, 2> PanelTile;
-typedef MfmaTile<128, 128, 16, 2, 2> SyrkTile;
+// 64 x 64 tiles: the trailing update of one step is a rank-64 product, so
+// the work per tile is small and the step's latency is set by how many tiles
+// run side by side (128 x 128 tiles left most CUs idle).
+typedef MfmaTile<64, 64, 16, 2, 2> PanelTile;
+typedef MfmaTile<64, 64, 16, 2, 2> SyrkTile;
 
 // L_IJ = A_IJ inv(L_JJ)^T for the rows below the diagonal block (MFMA).
-// Each workgroup owns 128 rows, so the in-place overwrite is safe once its
+// Each workgroup owns PanelTile::BM rows, so the in-place overwrite is safe once its
 // GEMM has read them.
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_potrf_panel(double* __restrict__ A, int n, int J0,
