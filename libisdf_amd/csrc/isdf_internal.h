@@ -240,6 +240,12 @@ int launch_cct_lp(const float* Zt, int ldz, const int* piv, int rk, double* P, d
 // pre-split bf16 planes).
 int launch_inverse(int wmode, int rk, const double* Ft, double* Gt, double* W, int ldw,
                    void* Wf, hipStream_t st);
+// Z C^T tile variant after defaults and the dual-product overrides, and the
+// format of the operand rows it reads (1 FP32, 2 pre-split bf16 planes; the
+// mode argument of launch_pack_psi / launch_gather_rows): BF16X3 variants 3
+// and 4 read FP32 rows and split them in registers.
+int lp_zct_variant(int mode, int variant, bool dual);
+int lp_zct_op_mode(int mode, int variant, bool dual);
 // Theta tile variant (-1: the default of the mode) and the W operand format it
 // reads (the wmode of launch_inverse): BF16X3 variant 3 reads FP32 W.
 int lp_theta_variant(int mode, int variant);

@@ -571,7 +571,9 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lp(int ng, int rk, int ldz,
         if (m < mlim && n < wlim) {
           const float v1 = acc1[i][j][r];
           const float v2 = DUAL ? acc2[i][j][r] : v1;
-          Zm[(long long)m * ldz + n] = v1 * v2;
+          // Columns past rk are zero (a tile that clamps operand rows instead
+          // of zeroing them leaves other values in those accumulators).
+          Zm[(long long)m * ldz + n] = n < nlim ? v1 * v2 : 0.f;
         }
       }
   }
@@ -809,6 +811,8 @@ typedef LpTile<256, 128, 4, 2, true, true, false> LpTB16_0;
 typedef LpTile<128, 256, 2, 4, true, true, false> LpTB16_1;
 typedef LpTile<128, 128, 2, 2, true, true, false> LpTB16_2;
 typedef LpDmaTile<256, 256, 4, 2> LpTB16D_3;  // FP32 W and Zt, 8 waves of 64 x 128
+typedef LpDmaTile<256, 128, 4, 2> LpZB16D_3;  // FP32 psi rows, 8 waves of 64 x 64
+typedef LpDmaTile<256, 256, 4, 2> LpZB16D_4;  // 8 waves of 64 x 128 (one product only)
 
 }  // namespace
 
@@ -840,15 +844,26 @@ int launch_gather_rows(int mode, const double* psi, long long ld, const int* piv
   return 0;
 }
 
+int lp_zct_variant(int mode, int variant, bool dual) {
+  if (variant < 0) variant = mode == 1 ? 2 : 4;  // measured best on the 96^3 workloads
+  // Two accumulators: 256 x 256 (FP32 and direct-to-LDS BF16X3) and the 8-wave
+  // pre-split BF16X3 tiles spill; use the smaller tiles.
+  if (dual && mode == 1 && variant == 2) variant = 0;
+  if (dual && mode == 2 && variant < 2) variant = 2;
+  if (dual && mode == 2 && variant == 4) variant = 3;
+  return variant;
+}
+
+int lp_zct_op_mode(int mode, int variant, bool dual) {
+  return mode == 2 && lp_zct_variant(mode, variant, dual) < 3 ? 2 : 1;
+}
+
 int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void* A2, int kp1,
                   int kp2, const void* B1, const void* B2, bool dual, float* Zt, int variant,
                   hipStream_t st) {
-  if (variant < 0) variant = mode == 1 ? 2 : 0;  // measured best on the 96^3 workload
-  // Two accumulators: 256 x 256 (FP32) and the 8-wave BF16X3 tiles spill; use
-  // the smaller tiles.
-  if (dual && mode == 1 && variant == 2) variant = 0;
-  if (dual && mode == 2 && variant != 2) variant = 2;
-  const long long lda1 = mode == 2 ? 3LL * kp1 : kp1, lda2 = mode == 2 ? 3LL * kp2 : kp2;
+  variant = lp_zct_variant(mode, variant, dual);
+  const bool pre = mode == 2 && variant < 3;  // pre-split operands (else FP32 rows)
+  const long long lda1 = pre ? 3LL * kp1 : kp1, lda2 = pre ? 3LL * kp2 : kp2;
 #define ISDF_ZLP(TL)                                                                           \
   do {                                                                                         \
     const unsigned nb = cdiv(ng, TL::BM) * cdiv(ldz, TL::BN);                                  \
@@ -866,6 +881,8 @@ int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void*
   } else {
     if (variant == 1) ISDF_ZLP(LpZB16_1);
     else if (variant == 2) ISDF_ZLP(LpZB16_2);
+    else if (variant == 3) ISDF_ZLP(LpZB16D_3);
+    else if (variant == 4) ISDF_ZLP(LpZB16D_4);
     else ISDF_ZLP(LpZB16_0);
   }
 #undef ISDF_ZLP

@@ -220,8 +220,9 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
   ISDF_HIP_TRY(hipStreamWaitEvent(s2, aux->fork, 0));
   if (int rc = ev.rec(6, s2)) return rc;
   // C C^T: the contraction on the pivot rows alone (rk x rk x K).
+  const int zv = env_variant("ISDF_LP_ZCT_TILE");  // the format of the packed rows follows it
   if (int rc = launch_zct_lp(mode, rk, rk, ldz, m.B1, m.B2, kp1, dual ? kp2 : kp1, m.B1, m.B2,
-                             dual, m.Pz, -1, s2))
+                             dual, m.Pz, zv, s2))
     return rc;
   if (int rc = launch_cct_lp(m.Pz, ldz, nullptr, rk, P, eps_dev, s2)) return rc;
   if (dbg && dbg->d_cct)
@@ -243,7 +244,7 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
   // Main stream: Z C^T once the packed psi rows are in place.
   ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->pack, 0));
   if (int rc = launch_zct_lp(mode, ng, rk, ldz, m.A1, m.A2, kp1, dual ? kp2 : kp1, m.B1, m.B2,
-                             dual, m.Zt, env_variant("ISDF_LP_ZCT_TILE"), st))
+                             dual, m.Zt, zv, st))
     return rc;
   if (int rc = ev.rec(5, st)) return rc;
   ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->join, 0));
@@ -275,6 +276,8 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   const int nJ = (rk + kNB - 1) / kNB + 1;  // 64-blocks, plus one so 128-blocks pair up
   const int mode = p->precision & ~ISDF_PREC_REFINE;
   const bool mixed = mode != ISDF_PREC_FP64;
+  // Format of the packed psi / pivot rows: the one the Z C^T tile reads.
+  const int opm = mixed ? lp_zct_op_mode(mode, env_variant("ISDF_LP_ZCT_TILE"), dual) : 0;
   const bool refine = (p->precision & ISDF_PREC_REFINE) != 0;
   // Mixed forms: FP32 operand rows padded to multiples of 32 in k.
   auto pad32 = [](int v) { return (v + 31) / 32 * 32; };
@@ -364,9 +367,9 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
     // Operand rows of psi on the helper stream, beside weights and K-means.
     ISDF_HIP_TRY(hipEventRecord(aux->fork, st));
     ISDF_HIP_TRY(hipStreamWaitEvent(aux->s, aux->fork, 0));
-    if (int rc = launch_pack_psi(mode, d_psi, ld, ng, s1, K1, kp1, mb.A1, aux->s)) return rc;
+    if (int rc = launch_pack_psi(opm, d_psi, ld, ng, s1, K1, kp1, mb.A1, aux->s)) return rc;
     if (dual)
-      if (int rc = launch_pack_psi(mode, d_psi, ld, ng, s2, K2, kp2, mb.A2, aux->s)) return rc;
+      if (int rc = launch_pack_psi(opm, d_psi, ld, ng, s2, K2, kp2, mb.A2, aux->s)) return rc;
     ISDF_HIP_TRY(hipEventRecord(aux->pack, aux->s));
   }
 
@@ -445,10 +448,10 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
 
   if (mixed) {
     // Psi(mu) operand rows (main stream), then the two branches.
-    if (int rc = launch_gather_rows(mode, d_psi, ld, piv_dev, rk, s1, K1, kp1, mb.B1, st))
+    if (int rc = launch_gather_rows(opm, d_psi, ld, piv_dev, rk, s1, K1, kp1, mb.B1, st))
       return rc;
     if (dual)
-      if (int rc = launch_gather_rows(mode, d_psi, ld, piv_dev, rk, s2, K2, kp2, mb.B2, st))
+      if (int rc = launch_gather_rows(opm, d_psi, ld, piv_dev, rk, s2, K2, kp2, mb.B2, st))
         return rc;
     if (int rc = ev.rec(4, st)) return rc;
     if (int rc = basis_mixed(p, ng, rk, mode, refine, mb, kp1, kp2, ldz, P, Dinv, Dinv128, Ft,

@@ -142,6 +142,30 @@ def test_bf16x3_theta_tiles_bitwise_equal(cfg1, monkeypatch, case, refine):
         np.testing.assert_array_equal(out[v].view(np.uint64), out[0].view(np.uint64))
 
 
+@pytest.mark.parametrize("case", ["cfg1", "dual"])
+def test_bf16x3_zct_tiles_bitwise_equal(cfg1, monkeypatch, case):
+    # Z C^T tiles: pre-split operand rows (variants 0-2) or FP32 rows split in
+    # registers after direct-to-LDS loads (3: 256 x 128, 4: 256 x 256, which
+    # the dual product maps to 3).  The same six products per k chunk in the
+    # same order: Theta must not change a bit.  The dual case has two pair
+    # sets (two products in the epilogue), rk = 300 and Ng = 4032.
+    if case == "cfg1":
+        psi, args = cfg1[1], cfg1[2]
+    else:
+        grid, L = (16, 14, 18), (6.0, 5.5, 7.0)
+        ng = grid[0] * grid[1] * grid[2]
+        psi = synth.cholesky_qr2(np.random.default_rng(77).standard_normal((40, ng)),
+                                 L[0] * L[1] * L[2] / ng)
+        args = (L, grid, 20, 40, 20, 20, 8, 20, 300, 30)
+    out = {}
+    for v in (0, 1, 2, 3, 4):
+        monkeypatch.setenv("ISDF_LP_ZCT_TILE", str(v))
+        out[v] = run(psi, *args, isdf.ISDF_PREC_BF16X3)[0]
+    assert np.isfinite(out[0]).all()
+    for v in (1, 2, 3, 4):
+        np.testing.assert_array_equal(out[v].view(np.uint64), out[0].view(np.uint64))
+
+
 def test_mixed_ill_conditioned_raises_not_spd(si8):
     # Si8 has kappa(C C^T) ~ 2e11 (16 states give 136 pair products for rk =
     # 192).  The FP32 contraction's rounding (~1e-7 relative) exceeds the eps
