@@ -111,37 +111,42 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lds_epilogue(
   __shared__ double lds[Tile::LDS_DOUBLES > STAGE ? Tile::LDS_DOUBLES : STAGE];
   const int nrows = (ng + Tile::BM - 1) / Tile::BM;
   const int ncolt = (rk + Tile::BN - 1) / Tile::BN;
-  int rs, ct;
-  xcd_tile(blockIdx.x, ncolt, nrows, &rs, &ct);
-  if (rs >= nrows) return;
-  const long long m0 = (long long)rs * Tile::BM;
-  const int n0 = ct * Tile::BN;
-  const int mlim = (int)std::min<long long>(Tile::BM, ng - m0);
-  const int nlim = std::min(Tile::BN, rk - n0);
-  typename Tile::Acc acc, acc2;
-  Tile::zero(acc);
-  Tile::run(A1 + m0, lda, B1 + n0, ldb, K1, mlim, nlim, lds, acc);
-  if constexpr (DUAL) {
-    Tile::zero(acc2);
-    Tile::run(A2 + m0, lda, B2 + n0, ldb, K2, mlim, nlim, lds, acc2);
-  }
-  const int wave = threadIdx.x >> 6;
-  for (int h = 0; h < 2; h++) {
-    if (wave / Tile::WGM == h) {
-      if constexpr (DUAL)
-        Tile::epilogue2(acc, acc2, [&](int m, int n, double v1, double v2) {
-          lds[(n - h * HALF) * CS + m] = v1 * v2;
-        });
-      else
-        Tile::epilogue(acc, [&](int m, int n, double v) { lds[(n - h * HALF) * CS + m] = v * v; });
+  // One tile per workgroup, or, with a smaller grid (a multiple of 8, so tile
+  // tb stays on XCD tb % 8), each workgroup walks tiles tb, tb + gridDim.x, ...
+  for (int tb = blockIdx.x; tb < nrows * ncolt; tb += gridDim.x) {
+    int rs, ct;
+    xcd_tile(tb, ncolt, nrows, &rs, &ct);
+    if (rs >= nrows) continue;
+    const long long m0 = (long long)rs * Tile::BM;
+    const int n0 = ct * Tile::BN;
+    const int mlim = (int)std::min<long long>(Tile::BM, ng - m0);
+    const int nlim = std::min(Tile::BN, rk - n0);
+    typename Tile::Acc acc, acc2;
+    Tile::zero(acc);
+    Tile::run(A1 + m0, lda, B1 + n0, ldb, K1, mlim, nlim, lds, acc);
+    if constexpr (DUAL) {
+      Tile::zero(acc2);
+      Tile::run(A2 + m0, lda, B2 + n0, ldb, K2, mlim, nlim, lds, acc2);
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < HALF * Tile::BM; e += Tile::NT) {
-      const int c = e / Tile::BM, m = e % Tile::BM;
-      const int n = h * HALF + c;
-      if (m < mlim && n < nlim) C[(long long)(n0 + n) * ldc + m0 + m] = lds[c * CS + m];
+    const int wave = threadIdx.x >> 6;
+    for (int h = 0; h < 2; h++) {
+      if (wave / Tile::WGM == h) {
+        if constexpr (DUAL)
+          Tile::epilogue2(acc, acc2, [&](int m, int n, double v1, double v2) {
+            lds[(n - h * HALF) * CS + m] = v1 * v2;
+          });
+        else
+          Tile::epilogue(acc,
+                         [&](int m, int n, double v) { lds[(n - h * HALF) * CS + m] = v * v; });
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < HALF * Tile::BM; e += Tile::NT) {
+        const int c = e / Tile::BM, m = e % Tile::BM;
+        const int n = h * HALF + c;
+        if (m < mlim && n < nlim) C[(long long)(n0 + n) * ldc + m0 + m] = lds[c * CS + m];
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
@@ -701,9 +706,14 @@ int zct_pick_variant(long long ng, int requested) {
   return requested >= 0 ? requested : (ng >= 256LL * 256 ? 4 : 2);
 }
 
+int grid_leaving_cus(long long tiles, int reserve) {
+  if (reserve <= 0) return (int)tiles;
+  return (int)std::min<long long>(tiles, std::max(8, (device_cus() - reserve) / 8 * 8));
+}
+
 int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, int rk, int s1,
                int K1, int s2, int K2, bool dual, double* C, long long ldc, int variant,
-               hipStream_t st) {
+               hipStream_t st, int reserve_cus) {
   const double* A1 = psi + (long long)s1 * ld;
   const double* A2 = psi + (long long)s2 * ld;
   const double* B1 = psimuT + (long long)s1 * rk;
@@ -717,8 +727,10 @@ int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, in
     else if (variant == 1) ISDF_ZCT_LAUNCH(ZctTile1, false);
     else if (variant == 4) {
       hipLaunchKernelGGL((k_zct_lds_epilogue<ZctTile0, false>),
-                         dim3(cdiv(ng, ZctTile0::BM) * cdiv(rk, ZctTile0::BN)), dim3(ZctTile0::NT),
-                         0, st, ng, rk, A1, A2, ld, K1, K2, B1, B2, rk, C, ldc);
+                         dim3(grid_leaving_cus((long long)cdiv(ng, ZctTile0::BM) *
+                                                   cdiv(rk, ZctTile0::BN),
+                                               reserve_cus)),
+                         dim3(ZctTile0::NT), 0, st, ng, rk, A1, A2, ld, K1, K2, B1, B2, rk, C, ldc);
     } else if (variant == 3) {
       const long long total = (long long)cdiv(ng, ZctTile0::BM) * cdiv(rk, ZctTile0::BN);
       const int grid = (int)std::min<long long>(total, std::max(8, device_cus() / 8 * 8));
@@ -729,7 +741,9 @@ int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, in
     if (variant == 2) ISDF_ZCT_LAUNCH(ZctTileDual1, true);
     else if (variant == 4)
       hipLaunchKernelGGL((k_zct_lds_epilogue<ZctTileDual0, true>),
-                         dim3(cdiv(ng, ZctTileDual0::BM) * cdiv(rk, ZctTileDual0::BN)),
+                         dim3(grid_leaving_cus((long long)cdiv(ng, ZctTileDual0::BM) *
+                                                   cdiv(rk, ZctTileDual0::BN),
+                                               reserve_cus)),
                          dim3(ZctTileDual0::NT), 0, st, ng, rk, A1, A2, ld, K1, K2, B1, B2, rk, C,
                          ldc);
     else ISDF_ZCT_LAUNCH(ZctTileDual0, true);

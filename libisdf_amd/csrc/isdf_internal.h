@@ -128,9 +128,15 @@ int launch_gather_psimu(const double* psi, long long ld, int nstate, const int* 
 int zct_variants();
 // The tile variant launch_zct uses for ng rows when `requested` < 0.
 int zct_pick_variant(long long ng, int requested);
+// reserve_cus > 0: the default (LDS write-back) tile runs on a grid of
+// (CUs - reserve_cus) workgroups that walk the tiles, leaving CUs free for work
+// on another stream (the Cholesky branch beside the Ng-row product).
 int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, int rk, int s1,
                int K1, int s2, int K2, bool dual, double* C, long long ldc, int variant,
-               hipStream_t st);
+               hipStream_t st, int reserve_cus = 0);
+// Grid for `tiles` tiles leaving `reserve` CUs free (a multiple of 8; all
+// tiles when reserve <= 0).
+int grid_leaving_cus(long long tiles, int reserve);
 int launch_potrf(double* P, int rk, double* Dinv, int* info_dev, hipStream_t st);
 int launch_make_fg(const double* L, int rk, const double* Dinv, double* Dinv128, double* Ft,
                    double* Gt, hipStream_t st);
@@ -230,7 +236,7 @@ int launch_gather_rows(int mode, const double* psi, long long ld, const int* piv
 // Zt[g*ldz + nu] = P1(g,nu) P2(g,nu) (FP32), zero for rk <= nu < ldz.
 int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void* A2, int kp1,
                   int kp2, const void* B1, const void* B2, bool dual, float* Zt, int variant,
-                  hipStream_t st);
+                  hipStream_t st, int reserve_cus = 0);
 // P(mu, nu) = Zt[piv[mu]][nu] (column-major FP64; piv NULL: row mu) and the eps shift.
 int launch_cct_lp(const float* Zt, int ldz, const int* piv, int rk, double* P, double* eps_dev,
                   hipStream_t st);

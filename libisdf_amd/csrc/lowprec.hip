@@ -541,9 +541,12 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lp(int ng, int rk, int ldz,
   __shared__ __attribute__((aligned(16))) char lds[Tile::LDS_BYTES];
   const int nstrip = (ng + Tile::BM - 1) / Tile::BM;
   const int ntile = (ldz + Tile::BN - 1) / Tile::BN;
+  // One tile per workgroup, or, with a smaller grid (a multiple of 8), each
+  // workgroup walks tiles tb, tb + gridDim.x, ... on the same XCD.
+  for (int tb = blockIdx.x; tb < nstrip * ntile; tb += gridDim.x) {
   int s, ct;
-  xcd_map(blockIdx.x, ntile, nstrip, &s, &ct);
-  if (s >= nstrip) return;
+  xcd_map(tb, ntile, nstrip, &s, &ct);
+  if (s >= nstrip) continue;
   const long long m0 = (long long)s * Tile::BM;
   const int n0 = ct * Tile::BN;
   const int mlim = (int)std::min<long long>(Tile::BM, ng - m0);
@@ -576,6 +579,7 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lp(int ng, int rk, int ldz,
           Zm[(long long)m * ldz + n] = n < nlim ? v1 * v2 : 0.f;
         }
       }
+  }
   }
 }
 
@@ -860,13 +864,14 @@ int lp_zct_op_mode(int mode, int variant, bool dual) {
 
 int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void* A2, int kp1,
                   int kp2, const void* B1, const void* B2, bool dual, float* Zt, int variant,
-                  hipStream_t st) {
+                  hipStream_t st, int reserve_cus) {
   variant = lp_zct_variant(mode, variant, dual);
   const bool pre = mode == 2 && variant < 3;  // pre-split operands (else FP32 rows)
   const long long lda1 = pre ? 3LL * kp1 : kp1, lda2 = pre ? 3LL * kp2 : kp2;
 #define ISDF_ZLP(TL)                                                                           \
   do {                                                                                         \
-    const unsigned nb = cdiv(ng, TL::BM) * cdiv(ldz, TL::BN);                                  \
+    const unsigned nb =                                                                        \
+        grid_leaving_cus((long long)cdiv(ng, TL::BM) * cdiv(ldz, TL::BN), reserve_cus);        \
     if (dual)                                                                                  \
       hipLaunchKernelGGL((k_zct_lp<TL, true>), dim3(nb), dim3(TL::NT), 0, st, ng, rk, ldz, A1, \
                          A2, kp1, kp2, lda1, lda2, B1, B2, Zt);                                \

@@ -198,6 +198,24 @@ int env_variant(const char* name) {
   return v ? std::atoi(v) : -1;
 }
 
+// CUs the Ng-row Z C^T leaves free for the concurrent Cholesky branch (its
+// workgroups then walk the tiles).  Without them every small Cholesky kernel
+// waits for CUs held by the GEMM.  That matters only when the branch (a chain
+// of rk/64 latency-bound steps, ~0.11 ms each; the mixed forms add the
+// inverse) is about as long as the GEMM (2 Ng rk K flops at ~45 TFLOP/s FP64,
+// ~140 nominal mixed); when the GEMM is much longer, reserved CUs only slow
+// it.  Measured: 64^3 FP64 18.15 -> 17.39 ms per solve with 48 CUs free,
+// 96^3 BF16X3 56.6 -> 54.6 ms with 64; 96^3 FP64 and the dual 96^3 BF16X3
+// case got slower with any reserve; 16-24 made the branch's first kernel wait
+// for the whole GEMM.
+int reserve_cus(bool mixed, long long ng, int rk, int ktot) {
+  if (const char* v = std::getenv("ISDF_ZCT_RESERVE_CUS")) return std::atoi(v);
+  const double gemm_ms = 2.0 * (double)ng * rk * ktot / (mixed ? 140e9 : 45e9);
+  const double chol_ms = (rk / 64.0) * 0.11 * (mixed ? 1.6 : 1.0);
+  if (gemm_ms > 2.0 * chol_ms) return 0;
+  return mixed ? 64 : 48;
+}
+
 // Basis construction of the mixed forms.  Helper stream: the operand rows of
 // psi (queued at the start of the solve, beside K-means), then, from the
 // pivots, C C^T + eps from a small low-precision contraction of the pivot rows
@@ -244,7 +262,8 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
   // Main stream: Z C^T once the packed psi rows are in place.
   ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->pack, 0));
   if (int rc = launch_zct_lp(mode, ng, rk, ldz, m.A1, m.A2, kp1, dual ? kp2 : kp1, m.B1, m.B2,
-                             dual, m.Zt, zv, st))
+                             dual, m.Zt, zv, st,
+                             reserve_cus(true, ng, rk, kp1 + (dual ? kp2 : 0))))
     return rc;
   if (int rc = ev.rec(5, st)) return rc;
   ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->join, 0));
@@ -483,7 +502,8 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
     if (int rc = ev.rec(8, s2s)) return rc;
     ISDF_HIP_TRY(hipEventRecord(aux->join, s2s));
     // Main stream: Z C^T into d_theta (getbasis_scalapack.cpp:297-369).
-    if (int rc = launch_zct(d_psi, ld, ng, psimuT, rk, s1, K1, s2, K2, dual, d_theta, ng, zv, st))
+    if (int rc = launch_zct(d_psi, ld, ng, psimuT, rk, s1, K1, s2, K2, dual, d_theta, ng, zv, st,
+                            reserve_cus(false, ng64, rk, K1 + (dual ? K2 : 0))))
       return rc;
     if (int rc = ev.rec(5, st)) return rc;
     ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->join, 0));
