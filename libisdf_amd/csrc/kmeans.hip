@@ -597,7 +597,13 @@ __global__ __launch_bounds__(256) void k_centroid_sums_box(
     const double* __restrict__ X, const double* __restrict__ Y, const double* __restrict__ Z,
     const int* __restrict__ bbox, int rkmax, double* __restrict__ sw, double* __restrict__ sx,
     double* __restrict__ sy, double* __restrict__ sz, const int* scal, const KmJob* jobs) {
-  __shared__ double buf[4][1024];
+  // Box positions per pass.  The batched launch has many more clusters than
+  // CUs: 512 (2 per thread) keeps the member buffer at 16 KB, so eight
+  // workgroups fit a CU (the sequential sums wait on memory and on the
+  // previous add; more resident workgroups hide it): 24 uneven instances
+  // 1468 -> 1506 solves/s.  One instance (rk up to a few per CU) keeps 1024.
+  constexpr int CH = BATCH ? 512 : 1024, PT = CH / 256;
+  __shared__ double buf[4][CH];
   if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     n0 = J.b.ngrid[0];
@@ -627,13 +633,13 @@ __global__ __launch_bounds__(256) void k_centroid_sums_box(
     const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
     const long long plane = (long long)bw * bh;
     const long long total = plane * (z1 - z0 + 1);
-    for (long long j0 = 0; j0 < total; j0 += 1024) {
-      int g[4];
-      bool mem[4];
+    for (long long j0 = 0; j0 < total; j0 += CH) {
+      int g[PT];
+      bool mem[PT];
       int cnt = 0;
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const long long j = j0 + 4 * t + q;
+      for (int q = 0; q < PT; q++) {
+        const long long j = j0 + PT * t + q;
         mem[q] = false;
         g[q] = 0;
         if (j < total) {
@@ -649,7 +655,7 @@ __global__ __launch_bounds__(256) void k_centroid_sums_box(
       int tot;
       int pos = block_excl_scan(cnt, &tot);
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
+      for (int q = 0; q < PT; q++) {
         if (mem[q]) {
           const int gg = g[q];
           const double wv = WL[gg];
