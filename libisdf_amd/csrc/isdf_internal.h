@@ -234,9 +234,11 @@ int launch_pack_psi(int mode, const double* psi, long long ld, int ng, int c0, i
 int launch_gather_rows(int mode, const double* psi, long long ld, const int* piv, int rk, int c0,
                        int K, int kp, void* out, hipStream_t st);
 // Zt[g*ldz + nu] = P1(g,nu) P2(g,nu) (FP32), zero for rk <= nu < ldz.
+// zsplit: Zt written pre-split (bf16 parts, row stride 3 ldz; BF16X3 variants
+// 3 and 4 only), for Theta variant 4.
 int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void* A2, int kp1,
                   int kp2, const void* B1, const void* B2, bool dual, float* Zt, int variant,
-                  hipStream_t st, int reserve_cus = 0);
+                  hipStream_t st, int reserve_cus = 0, bool zsplit = false);
 // P(mu, nu) = Zt[piv[mu]][nu] (column-major FP64; piv NULL: row mu) and the eps shift.
 int launch_cct_lp(const float* Zt, int ldz, const int* piv, int rk, double* P, double* eps_dev,
                   hipStream_t st);
@@ -256,6 +258,8 @@ int lp_zct_op_mode(int mode, int variant, bool dual);
 // reads (the wmode of launch_inverse): BF16X3 variant 3 reads FP32 W.
 int lp_theta_variant(int mode, int variant);
 int lp_theta_w_mode(int mode, int variant);
+// Whether the Theta variant reads Zt pre-split (BF16X3 variant 4).
+bool lp_theta_zt_split(int mode, int variant);
 // theta(g, nu) = (or +=) sum_k Wf[nu][k] Zt[g][k], k < ldk.
 int launch_theta_lp(int mode, int ng, int rk, int ldk, const void* Wf, const float* Zt,
                     double* theta, long long ldt, bool add, int variant, hipStream_t st);

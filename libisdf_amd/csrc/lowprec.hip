@@ -272,19 +272,30 @@ struct LpTile {
 //   Operand rows are FP32 with row stride ld (a multiple of 4), K a multiple of
 //   32; rows >= mlim / nlim are read as row mlim-1 / nlim-1 (their outputs are
 //   not stored), so every load stays inside the operands.
-template <int BM_, int BN_, int WGM_, int WGN_, int STAGES_ = 2>
+//
+// B_SPLIT: the B operand comes pre-split from HBM (the three bf16 parts
+// interleaved per 32-wide k chunk, row stride 3 K, as LpOperand's pre-split
+// rows) and needs no conversion.  In LDS, row r's h and m parts share one
+// 128-byte row (16-byte piece q at q ^ (r & 7)), and the l parts of rows 2R
+// and 2R+1 share row R of a second region (piece 4 (r & 1) + c at that
+// ^ (R & 7)); both measured conflict-free for the fragment reads.
+template <int BM_, int BN_, int WGM_, int WGN_, int STAGES_ = 2, bool B_SPLIT_ = false>
 struct LpDmaTile {
   static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_, STAGES = STAGES_;
-  static constexpr bool BF16 = true, A_PRE = false, B_PRE = false;
+  static constexpr bool BF16 = true, A_PRE = false, B_PRE = B_SPLIT_, B_SPLIT = B_SPLIT_;
   static constexpr int NW = WGM * WGN, NT = 64 * NW;
   static constexpr int TM = BM / WGM / 16, TN = BN / WGN / 16;
   static constexpr int KB = 32, ROWB = 128;
   static_assert(STAGES >= 2, "stages");
-  static constexpr int A_BYTES = BM * ROWB, STAGE_BYTES = (BM + BN) * ROWB;
+  static constexpr int A_BYTES = BM * ROWB;
+  static constexpr int B_HM_BYTES = BN * ROWB;  // FP32 rows, or the h|m rows
+  static constexpr int STAGE_BYTES = A_BYTES + B_HM_BYTES + (B_SPLIT ? BN / 2 * ROWB : 0);
   static constexpr int LDS_BYTES = STAGES * STAGE_BYTES;
   static constexpr int A_PW = BM / 8 / NW, B_PW = BN / 8 / NW;  // 1 KB blocks per wave
-  static constexpr int NDMA = A_PW + B_PW;                      // loads per wave per chunk
+  static constexpr int BL_PW = B_SPLIT ? BN / 16 / NW : 0;      // l-part blocks per wave
+  static constexpr int NDMA = A_PW + B_PW + BL_PW;              // loads per wave per chunk
   static_assert(A_PW * 8 * NW == BM && B_PW * 8 * NW == BN, "DMA blocks per wave");
+  static_assert(!B_SPLIT || BL_PW * 16 * NW == BN, "l-part blocks per wave");
   static_assert(LDS_BYTES <= 160 * 1024, "LDS");
   static_assert(NDMA * 2 <= 63 && STAGES <= 4, "vmcnt range");
   typedef f4 Acc[TM][TN];
@@ -312,11 +323,33 @@ struct LpDmaTile {
       __builtin_amdgcn_global_load_lds((const void*)(A + (long long)r * lda + k0 + c4),
                                        (LdsPtr)(stage + blk * 1024), 16, 0, 0);
     }
+    if constexpr (!B_SPLIT) {
 #pragma unroll
-    for (int j = 0; j < B_PW; j++) {
-      const int blk = wave + NW * j, r = std::min(blk * 8 + rl, nlim - 1);
-      __builtin_amdgcn_global_load_lds((const void*)(B + (long long)r * ldb + k0 + c4),
-                                       (LdsPtr)(stage + A_BYTES + blk * 1024), 16, 0, 0);
+      for (int j = 0; j < B_PW; j++) {
+        const int blk = wave + NW * j, r = std::min(blk * 8 + rl, nlim - 1);
+        __builtin_amdgcn_global_load_lds((const void*)(B + (long long)r * ldb + k0 + c4),
+                                         (LdsPtr)(stage + A_BYTES + blk * 1024), 16, 0, 0);
+      }
+    } else {
+      // B is bf16 here (ldb in bf16 elements); chunk k0 of a row starts at
+      // element (k0 / 32) * 96: h at +0, m at +32, l at +64.
+      const __bf16* Bh = (const __bf16*)(const void*)B + (k0 / 32) * 96;
+#pragma unroll
+      for (int j = 0; j < B_PW; j++) {
+        const int blk = wave + NW * j, row = blk * 8 + rl;
+        const int q = (lane & 7) ^ (row & 7), r = std::min(row, nlim - 1);
+        __builtin_amdgcn_global_load_lds((const void*)(Bh + (long long)r * ldb + 8 * q),
+                                         (LdsPtr)(stage + A_BYTES + blk * 1024), 16, 0, 0);
+      }
+#pragma unroll
+      for (int j = 0; j < BL_PW; j++) {
+        const int blk = wave + NW * j, R = blk * 8 + rl;
+        const int q = (lane & 7) ^ (R & 7);
+        const int r = std::min(2 * R + (q >> 2), nlim - 1);
+        __builtin_amdgcn_global_load_lds(
+            (const void*)(Bh + (long long)r * ldb + 64 + 8 * (q & 3)),
+            (LdsPtr)(stage + A_BYTES + B_HM_BYTES + blk * 1024), 16, 0, 0);
+      }
     }
   }
 
@@ -369,11 +402,22 @@ struct LpDmaTile {
     // order, so the result is the same bit for bit.
     constexpr int PA[6] = {0, 2, 1, 0, 1, 0};
     constexpr int PB[6] = {2, 0, 1, 1, 0, 0};
+    // Pre-split B: row r = wn*TN*16 + 16 j + lr, so r & 7 = lr & 7, r & 1 =
+    // lr & 1 and (r >> 1) & 7 = (lr >> 1) & 7.
+    const int ph = 16 * (lk ^ (lr & 7)), pm = 16 * ((4 + lk) ^ (lr & 7));
+    const int pl = 16 * ((((lr & 1) << 2) + lk) ^ ((lr >> 1) & 7));
+    const char* Bl = stage + A_BYTES + B_HM_BYTES + ((wn * TN * 16 + lr) >> 1) * ROWB;
 #pragma unroll
     for (int j = 0; j < TN; j++) {
       bf8 b[3];
-      split8(*(const f4*)(Bw + 16 * ROWB * j + p0), *(const f4*)(Bw + 16 * ROWB * j + p1), b[0],
-             b[1], b[2]);
+      if constexpr (B_SPLIT) {
+        b[0] = *(const bf8*)(Bw + 16 * ROWB * j + ph);
+        b[1] = *(const bf8*)(Bw + 16 * ROWB * j + pm);
+        b[2] = *(const bf8*)(Bl + 8 * ROWB * j + pl);
+      } else {
+        split8(*(const f4*)(Bw + 16 * ROWB * j + p0), *(const f4*)(Bw + 16 * ROWB * j + p1),
+               b[0], b[1], b[2]);
+      }
 #pragma unroll
       for (int p = 0; p < 6; p++)
 #pragma unroll
@@ -530,7 +574,9 @@ __global__ __launch_bounds__(256) void k_gather_rows(const double* __restrict__ 
 // nu < ldz (columns rk..ldz come out zero, as the Theta GEMM's k padding).
 // A1/A2 (grid rows) and B1/B2 (pivot rows) in the Tile's operand formats, row
 // strides lda1/lda2 (elements of the format).
-template <class Tile, bool DUAL>
+// ZSPLIT: Zt is written pre-split (the three bf16 parts per 32-wide chunk,
+// row stride 3 ldz) for a Theta tile that reads it without conversion.
+template <class Tile, bool DUAL, bool ZSPLIT = false>
 __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lp(int ng, int rk, int ldz,
                                                         const void* __restrict__ A1,
                                                         const void* __restrict__ A2, int K1,
@@ -562,23 +608,71 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lp(int ng, int rk, int ldz,
     Tile::run((const char*)A2 + m0 * lda2 * ESA, lda2,
               (const char*)B2 + (long long)n0 * lda2 * ESB, lda2, K2, mlim, nlim, lds, acc2);
   }
-  float* Zm = Zt + m0 * ldz + n0;
+  if constexpr (ZSPLIT) {
+    // Pre-split Zt through LDS (the staging buffers are free after run()):
+    // each wave stages 32 of its columns at a time for its 16 TM rows, then
+    // every lane splits 8 consecutive columns of one row and stores the three
+    // parts as 16-byte pieces of the row's 192-byte chunk.
+    static_assert(Tile::NW * 16 * Tile::TM * 36 * 4 <= Tile::LDS_BYTES, "epilogue staging");
+    constexpr int WR = 16 * Tile::TM, WS = 36;  // wave rows, LDS row stride (floats)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float* Ws = (float*)(void*)lds + wave * WR * WS;
+    const int wr0 = (wave % Tile::WGM) * WR, wc0 = (wave / Tile::WGM) * (16 * Tile::TN);
 #pragma unroll
-  for (int j = 0; j < Tile::TN; j++) {
-    const int n = Tile::col(j);
+    for (int jp = 0; jp < Tile::TN; jp += 2) {
 #pragma unroll
-    for (int i = 0; i < Tile::TM; i++)
+      for (int jj = 0; jj < 2; jj++) {
+        const int n = Tile::col(jp + jj);
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int m = Tile::row(i, r);
-        if (m < mlim && n < wlim) {
-          const float v1 = acc1[i][j][r];
-          const float v2 = DUAL ? acc2[i][j][r] : v1;
-          // Columns past rk are zero (a tile that clamps operand rows instead
-          // of zeroing them leaves other values in those accumulators).
-          Zm[(long long)m * ldz + n] = n < nlim ? v1 * v2 : 0.f;
+        for (int i = 0; i < Tile::TM; i++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const float v1 = acc1[i][jp + jj][r];
+            const float v2 = DUAL ? acc2[i][jp + jj][r] : v1;
+            // Columns past rk are zero (the tile clamps operand rows).
+            Ws[(16 * i + 4 * (lane >> 4) + r) * WS + 16 * jj + (lane & 15)] =
+                n < nlim ? v1 * v2 : 0.f;
+          }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < WR * 4 / 64; q++) {
+        const int item = lane + 64 * q, lr = item >> 2, g8 = item & 3;
+        const int m = wr0 + lr, nc = wc0 + 16 * jp + 8 * g8;  // nc: tile column of the piece
+        if (m < mlim && nc < wlim) {
+          const f4 x0 = *(const f4*)(Ws + lr * WS + 8 * g8);
+          const f4 x1 = *(const f4*)(Ws + lr * WS + 8 * g8 + 4);
+          bf8 h, mm, l;
+          LpDmaTile<256, 128, 4, 2>::split8(x0, x1, h, mm, l);
+          const int nn = n0 + nc;
+          __bf16* o = (__bf16*)(void*)Zt + (m0 + m) * 3LL * ldz + (nn >> 5) * 96 + (nn & 31);
+          *(bf8*)o = h;
+          *(bf8*)(o + 32) = mm;
+          *(bf8*)(o + 64) = l;
         }
       }
+      __syncthreads();
+    }
+  } else {
+    float* Zm = Zt + m0 * ldz + n0;
+#pragma unroll
+    for (int j = 0; j < Tile::TN; j++) {
+      const int n = Tile::col(j);
+#pragma unroll
+      for (int i = 0; i < Tile::TM; i++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int m = Tile::row(i, r);
+          if (m < mlim && n < wlim) {
+            const float v1 = acc1[i][j][r];
+            const float v2 = DUAL ? acc2[i][j][r] : v1;
+            // Columns past rk are zero (a tile that clamps operand rows
+            // instead of zeroing them leaves other values in those
+            // accumulators).
+            Zm[(long long)m * ldz + n] = n < nlim ? v1 * v2 : 0.f;
+          }
+        }
+    }
   }
   }
 }
@@ -635,11 +729,12 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_theta_lp(int ng, int rk, int ld
   const int v0 = vt * Tile::BM;
   const int mlim = std::min(Tile::BM, rk - v0);
   const int nlim = (int)std::min<long long>(Tile::BN, ng - g0);
-  constexpr int ESA = Tile::A_PRE ? 2 : 4;
+  constexpr int ESA = Tile::A_PRE ? 2 : 4, ESB = Tile::B_PRE ? 2 : 4;
+  const long long ldbz = Tile::B_PRE ? 3LL * ldk : ldk;  // Zt row stride (elements)
   typename Tile::Acc acc;
   Tile::zero(acc);
-  Tile::run((const char*)Wf + (long long)v0 * ldwr * ESA, ldwr, Zt + g0 * ldk, ldk, ldk, mlim,
-            nlim, lds, acc);
+  Tile::run((const char*)Wf + (long long)v0 * ldwr * ESA, ldwr,
+            (const char*)Zt + g0 * ldbz * ESB, ldbz, ldk, mlim, nlim, lds, acc);
   double* Tt = theta + (long long)v0 * ldt + g0;
 #pragma unroll
   for (int i = 0; i < Tile::TM; i++)
@@ -815,6 +910,7 @@ typedef LpTile<256, 128, 4, 2, true, true, false> LpTB16_0;
 typedef LpTile<128, 256, 2, 4, true, true, false> LpTB16_1;
 typedef LpTile<128, 128, 2, 2, true, true, false> LpTB16_2;
 typedef LpDmaTile<256, 256, 4, 2> LpTB16D_3;  // FP32 W and Zt, 8 waves of 64 x 128
+typedef LpDmaTile<256, 256, 4, 2, 2, true> LpTB16D_4;  // FP32 W, pre-split Zt (160 KB LDS)
 typedef LpDmaTile<256, 128, 4, 2> LpZB16D_3;  // FP32 psi rows, 8 waves of 64 x 64
 typedef LpDmaTile<256, 256, 4, 2> LpZB16D_4;  // 8 waves of 64 x 128 (one product only)
 
@@ -864,7 +960,7 @@ int lp_zct_op_mode(int mode, int variant, bool dual) {
 
 int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void* A2, int kp1,
                   int kp2, const void* B1, const void* B2, bool dual, float* Zt, int variant,
-                  hipStream_t st, int reserve_cus) {
+                  hipStream_t st, int reserve_cus, bool zsplit) {
   variant = lp_zct_variant(mode, variant, dual);
   const bool pre = mode == 2 && variant < 3;  // pre-split operands (else FP32 rows)
   const long long lda1 = pre ? 3LL * kp1 : kp1, lda2 = pre ? 3LL * kp2 : kp2;
@@ -879,6 +975,19 @@ int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void*
       hipLaunchKernelGGL((k_zct_lp<TL, false>), dim3(nb), dim3(TL::NT), 0, st, ng, rk, ldz,    \
                          A1, A2, kp1, kp2, lda1, lda2, B1, B2, Zt);                            \
   } while (0)
+#define ISDF_ZLPS(TL)                                                                          \
+  do {                                                                                         \
+    const unsigned nb =                                                                        \
+        grid_leaving_cus((long long)cdiv(ng, TL::BM) * cdiv(ldz, TL::BN), reserve_cus);        \
+    if (dual)                                                                                  \
+      hipLaunchKernelGGL((k_zct_lp<TL, true, true>), dim3(nb), dim3(TL::NT), 0, st, ng, rk,    \
+                         ldz, A1, A2, kp1, kp2, lda1, lda2, B1, B2, Zt);                       \
+    else                                                                                       \
+      hipLaunchKernelGGL((k_zct_lp<TL, false, true>), dim3(nb), dim3(TL::NT), 0, st, ng, rk,   \
+                         ldz, A1, A2, kp1, kp2, lda1, lda2, B1, B2, Zt);                       \
+  } while (0)
+  if (zsplit && (mode != 2 || variant < 3))
+    return set_error(1, "pre-split Zt needs a BF16X3 direct-to-LDS Z C^T tile");
   if (mode == 1) {
     if (variant == 1) ISDF_ZLP(LpF32_1);
     else if (variant == 2) ISDF_ZLP(LpF32_2);
@@ -886,11 +995,14 @@ int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void*
   } else {
     if (variant == 1) ISDF_ZLP(LpZB16_1);
     else if (variant == 2) ISDF_ZLP(LpZB16_2);
+    else if (variant == 3 && zsplit) ISDF_ZLPS(LpZB16D_3);
     else if (variant == 3) ISDF_ZLP(LpZB16D_3);
+    else if (variant == 4 && zsplit) ISDF_ZLPS(LpZB16D_4);
     else if (variant == 4) ISDF_ZLP(LpZB16D_4);
     else ISDF_ZLP(LpZB16_0);
   }
 #undef ISDF_ZLP
+#undef ISDF_ZLPS
   ISDF_HIP_LAUNCHED("k_zct_lp");
   return 0;
 }
@@ -922,11 +1034,15 @@ int launch_inverse(int wmode, int rk, const double* Ft, double* Gt, double* W, i
 }
 
 int lp_theta_variant(int mode, int variant) {
-  return variant >= 0 ? variant : mode == 1 ? 2 : 3;  // measured best on the 96^3 workload
+  return variant >= 0 ? variant : mode == 1 ? 2 : 4;  // measured best on the 96^3 workloads
 }
 
 int lp_theta_w_mode(int mode, int variant) {
   return mode == 2 && lp_theta_variant(mode, variant) < 3 ? 2 : 1;
+}
+
+bool lp_theta_zt_split(int mode, int variant) {
+  return mode == 2 && lp_theta_variant(mode, variant) == 4;
 }
 
 int launch_theta_lp(int mode, int ng, int rk, int ldk, const void* Wf, const float* Zt,
@@ -951,6 +1067,7 @@ int launch_theta_lp(int mode, int ng, int rk, int ldk, const void* Wf, const flo
     if (variant == 1) ISDF_TLP(LpTB16_1);
     else if (variant == 2) ISDF_TLP(LpTB16_2);
     else if (variant == 3) ISDF_TLP(LpTB16D_3);
+    else if (variant == 4) ISDF_TLP(LpTB16D_4);
     else ISDF_TLP(LpTB16_0);
   }
 #undef ISDF_TLP

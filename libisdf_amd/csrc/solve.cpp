@@ -228,6 +228,18 @@ struct MixedBufs {
   double *W, *Pe;
 };
 
+// Theta tile variant of a mixed solve.  The pre-split-Zt tile needs the
+// Z C^T epilogue to write Zt split, which only the direct-to-LDS Z C^T tiles
+// do, and the correction step writes an FP32 residual into Zt: otherwise the
+// FP32-Zt tile (3) is used.
+int mixed_theta_variant(int mode, bool refine, bool dual) {
+  int tv = env_variant("ISDF_LP_THETA_TILE");
+  if (lp_theta_zt_split(mode, tv) &&
+      (refine || lp_zct_variant(mode, env_variant("ISDF_LP_ZCT_TILE"), dual) < 3))
+    tv = 3;
+  return tv;
+}
+
 int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, const MixedBufs& m,
                 int kp1, int kp2, int ldz, double* P, double* Dinv, double* Dinv128, double* Ft,
                 double* Gt, int* info_dev, double* eps_dev, double* d_theta,
@@ -253,7 +265,8 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
   ISDF_HIP_TRY(hipMemsetAsync(info_dev, 0, sizeof(int), s2));
   if (int rc = launch_potrf(P, rk, Dinv, info_dev, s2)) return rc;
   if (int rc = launch_make_fg(P, rk, Dinv, Dinv128, Ft, Gt, s2)) return rc;
-  const int tv = env_variant("ISDF_LP_THETA_TILE");
+  const int tv = mixed_theta_variant(mode, refine, dual);
+  const bool zsplit = lp_theta_zt_split(mode, tv);
   if (int rc = launch_inverse(lp_theta_w_mode(mode, tv), rk, Ft, Gt, m.W, ldz, m.Wf, s2))
     return rc;
   if (int rc = ev.rec(8, s2)) return rc;
@@ -263,7 +276,7 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
   ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->pack, 0));
   if (int rc = launch_zct_lp(mode, ng, rk, ldz, m.A1, m.A2, kp1, dual ? kp2 : kp1, m.B1, m.B2,
                              dual, m.Zt, zv, st,
-                             reserve_cus(true, ng, rk, kp1 + (dual ? kp2 : 0))))
+                             reserve_cus(true, ng, rk, kp1 + (dual ? kp2 : 0)), zsplit))
     return rc;
   if (int rc = ev.rec(5, st)) return rc;
   ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->join, 0));
@@ -336,8 +349,9 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
     off += align256(eb * (size_t)rk * kp1);
     off_B2 = off;
     off += align256(eb * (size_t)rk * kp2);
-    off_Zt = off;
-    off += align256(sizeof(float) * (size_t)ng * ldz);
+    off_Zt = off;  // FP32, or three bf16 parts per element for the pre-split Theta tile
+    off += align256((lp_theta_zt_split(mode, mixed_theta_variant(mode, refine, dual)) ? 6 : 4) *
+                    (size_t)ng * ldz);
     off_W = off;
     off += align256(sizeof(double) * (size_t)rk * rk);
     off_Wf = off;

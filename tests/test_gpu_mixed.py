@@ -122,9 +122,10 @@ def test_mixed_repeatable_bitwise(cfg1):
 def test_bf16x3_theta_tiles_bitwise_equal(cfg1, monkeypatch, case, refine):
     # Every BF16X3 Theta tile runs the same six products per k chunk in the same
     # order on the same bf16 parts (pre-split or split in registers), so the
-    # tile choice, including the three-stage direct-to-LDS tile (variant 3,
-    # FP32 W), must not change a bit.  The ragged case has partial row and
-    # column tiles (rk = 100, Ng = 1080).
+    # tile choice must not change a bit: 0-2 read pre-split W and FP32 Zt, 3
+    # (direct-to-LDS) FP32 W and Zt, 4 FP32 W and a Zt the Z C^T epilogue wrote
+    # pre-split (with the correction step it falls back to 3).  The ragged
+    # case has partial row and column tiles (rk = 100, Ng = 1080).
     if case == "cfg1":
         psi, args = cfg1[1], cfg1[2]
     else:
@@ -134,11 +135,11 @@ def test_bf16x3_theta_tiles_bitwise_equal(cfg1, monkeypatch, case, refine):
                                  L[0] * L[1] * L[2] / ng)
         args = (L, grid, 11, 22, 9, 7, 5, 11, 100, 40)
     out = {}
-    for v in (0, 1, 2, 3):
+    for v in (0, 1, 2, 3, 4):
         monkeypatch.setenv("ISDF_LP_THETA_TILE", str(v))
         out[v] = run(psi, *args, isdf.ISDF_PREC_BF16X3 | refine)[0]
     assert np.isfinite(out[0]).all()
-    for v in (1, 2, 3):
+    for v in (1, 2, 3, 4):
         np.testing.assert_array_equal(out[v].view(np.uint64), out[0].view(np.uint64))
 
 
