@@ -100,7 +100,7 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct(int ng, int rk, const doubl
 // (the columns of one wave column) is formed (P1 P1 or P1 P2) into LDS
 // column-major, then written back as contiguous column segments (a wave stores
 // 512 consecutive bytes) instead of the accumulator layout's 32-byte pieces.
-template <class Tile, bool DUAL>
+template <class Tile, bool DUAL, bool PERSIST = false>
 __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lds_epilogue(
     int ng, int rk, const double* __restrict__ A1, const double* __restrict__ A2,
     long long lda, int K1, int K2, const double* __restrict__ B1,
@@ -111,8 +111,10 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lds_epilogue(
   __shared__ double lds[Tile::LDS_DOUBLES > STAGE ? Tile::LDS_DOUBLES : STAGE];
   const int nrows = (ng + Tile::BM - 1) / Tile::BM;
   const int ncolt = (rk + Tile::BN - 1) / Tile::BN;
-  // One tile per workgroup, or, with a smaller grid (a multiple of 8, so tile
-  // tb stays on XCD tb % 8), each workgroup walks tiles tb, tb + gridDim.x, ...
+  // One tile per workgroup, or (PERSIST: a smaller grid, a multiple of 8, so
+  // tile tb stays on XCD tb % 8) each workgroup walks tiles tb, tb +
+  // gridDim.x, ...  The one-tile form stays a separate instantiation: the
+  // loop costs registers (spills) that the plain launch does not need.
   for (int tb = blockIdx.x; tb < nrows * ncolt; tb += gridDim.x) {
     int rs, ct;
     xcd_tile(tb, ncolt, nrows, &rs, &ct);
@@ -147,6 +149,7 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lds_epilogue(
       }
       __syncthreads();
     }
+    if constexpr (!PERSIST) break;
   }
 }
 
@@ -726,11 +729,15 @@ int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, in
     if (variant == 0) ISDF_ZCT_LAUNCH(ZctTile0, false);
     else if (variant == 1) ISDF_ZCT_LAUNCH(ZctTile1, false);
     else if (variant == 4) {
-      hipLaunchKernelGGL((k_zct_lds_epilogue<ZctTile0, false>),
-                         dim3(grid_leaving_cus((long long)cdiv(ng, ZctTile0::BM) *
-                                                   cdiv(rk, ZctTile0::BN),
-                                               reserve_cus)),
-                         dim3(ZctTile0::NT), 0, st, ng, rk, A1, A2, ld, K1, K2, B1, B2, rk, C, ldc);
+      const long long tiles = (long long)cdiv(ng, ZctTile0::BM) * cdiv(rk, ZctTile0::BN);
+      const int grid = grid_leaving_cus(tiles, reserve_cus);
+      if (grid < tiles)
+        hipLaunchKernelGGL((k_zct_lds_epilogue<ZctTile0, false, true>), dim3(grid),
+                           dim3(ZctTile0::NT), 0, st, ng, rk, A1, A2, ld, K1, K2, B1, B2, rk, C,
+                           ldc);
+      else
+        hipLaunchKernelGGL((k_zct_lds_epilogue<ZctTile0, false>), dim3(grid), dim3(ZctTile0::NT),
+                           0, st, ng, rk, A1, A2, ld, K1, K2, B1, B2, rk, C, ldc);
     } else if (variant == 3) {
       const long long total = (long long)cdiv(ng, ZctTile0::BM) * cdiv(rk, ZctTile0::BN);
       const int grid = (int)std::min<long long>(total, std::max(8, device_cus() / 8 * 8));
@@ -739,13 +746,18 @@ int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, in
     } else ISDF_ZCT_LAUNCH(ZctTile2, false);
   } else {
     if (variant == 2) ISDF_ZCT_LAUNCH(ZctTileDual1, true);
-    else if (variant == 4)
-      hipLaunchKernelGGL((k_zct_lds_epilogue<ZctTileDual0, true>),
-                         dim3(grid_leaving_cus((long long)cdiv(ng, ZctTileDual0::BM) *
-                                                   cdiv(rk, ZctTileDual0::BN),
-                                               reserve_cus)),
-                         dim3(ZctTileDual0::NT), 0, st, ng, rk, A1, A2, ld, K1, K2, B1, B2, rk, C,
-                         ldc);
+    else if (variant == 4) {
+      const long long tiles = (long long)cdiv(ng, ZctTileDual0::BM) * cdiv(rk, ZctTileDual0::BN);
+      const int grid = grid_leaving_cus(tiles, reserve_cus);
+      if (grid < tiles)
+        hipLaunchKernelGGL((k_zct_lds_epilogue<ZctTileDual0, true, true>), dim3(grid),
+                           dim3(ZctTileDual0::NT), 0, st, ng, rk, A1, A2, ld, K1, K2, B1, B2, rk,
+                           C, ldc);
+      else
+        hipLaunchKernelGGL((k_zct_lds_epilogue<ZctTileDual0, true>), dim3(grid),
+                           dim3(ZctTileDual0::NT), 0, st, ng, rk, A1, A2, ld, K1, K2, B1, B2, rk,
+                           C, ldc);
+    }
     else ISDF_ZCT_LAUNCH(ZctTileDual0, true);
   }
 #undef ISDF_ZCT_LAUNCH

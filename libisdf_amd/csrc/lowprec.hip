@@ -576,7 +576,7 @@ __global__ __launch_bounds__(256) void k_gather_rows(const double* __restrict__ 
 // strides lda1/lda2 (elements of the format).
 // ZSPLIT: Zt is written pre-split (the three bf16 parts per 32-wide chunk,
 // row stride 3 ldz) for a Theta tile that reads it without conversion.
-template <class Tile, bool DUAL, bool ZSPLIT = false>
+template <class Tile, bool DUAL, bool ZSPLIT = false, bool PERSIST = false>
 __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lp(int ng, int rk, int ldz,
                                                         const void* __restrict__ A1,
                                                         const void* __restrict__ A2, int K1,
@@ -587,8 +587,9 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lp(int ng, int rk, int ldz,
   __shared__ __attribute__((aligned(16))) char lds[Tile::LDS_BYTES];
   const int nstrip = (ng + Tile::BM - 1) / Tile::BM;
   const int ntile = (ldz + Tile::BN - 1) / Tile::BN;
-  // One tile per workgroup, or, with a smaller grid (a multiple of 8), each
-  // workgroup walks tiles tb, tb + gridDim.x, ... on the same XCD.
+  // One tile per workgroup, or (PERSIST: a smaller grid, a multiple of 8)
+  // each workgroup walks tiles tb, tb + gridDim.x, ... on the same XCD; the
+  // one-tile form is its own instantiation (the loop costs registers).
   for (int tb = blockIdx.x; tb < nstrip * ntile; tb += gridDim.x) {
   int s, ct;
   xcd_map(tb, ntile, nstrip, &s, &ct);
@@ -674,6 +675,7 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct_lp(int ng, int rk, int ldz,
         }
     }
   }
+  if constexpr (!PERSIST) break;
   }
 }
 
@@ -964,28 +966,20 @@ int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void*
   variant = lp_zct_variant(mode, variant, dual);
   const bool pre = mode == 2 && variant < 3;  // pre-split operands (else FP32 rows)
   const long long lda1 = pre ? 3LL * kp1 : kp1, lda2 = pre ? 3LL * kp2 : kp2;
-#define ISDF_ZLP(TL)                                                                           \
+#define ISDF_ZLP4(TL, DU, ZS, PE)                                                              \
+  hipLaunchKernelGGL((k_zct_lp<TL, DU, ZS, PE>), dim3(nb), dim3(TL::NT), 0, st, ng, rk, ldz,   \
+                     A1, A2, kp1, kp2, lda1, lda2, B1, B2, Zt)
+#define ISDF_ZLPX(TL, ZS)                                                                      \
   do {                                                                                         \
-    const unsigned nb =                                                                        \
-        grid_leaving_cus((long long)cdiv(ng, TL::BM) * cdiv(ldz, TL::BN), reserve_cus);        \
-    if (dual)                                                                                  \
-      hipLaunchKernelGGL((k_zct_lp<TL, true>), dim3(nb), dim3(TL::NT), 0, st, ng, rk, ldz, A1, \
-                         A2, kp1, kp2, lda1, lda2, B1, B2, Zt);                                \
-    else                                                                                       \
-      hipLaunchKernelGGL((k_zct_lp<TL, false>), dim3(nb), dim3(TL::NT), 0, st, ng, rk, ldz,    \
-                         A1, A2, kp1, kp2, lda1, lda2, B1, B2, Zt);                            \
+    const long long tiles = (long long)cdiv(ng, TL::BM) * cdiv(ldz, TL::BN);                   \
+    const unsigned nb = grid_leaving_cus(tiles, reserve_cus);                                  \
+    if (dual && nb < tiles) ISDF_ZLP4(TL, true, ZS, true);                                     \
+    else if (dual) ISDF_ZLP4(TL, true, ZS, false);                                             \
+    else if (nb < tiles) ISDF_ZLP4(TL, false, ZS, true);                                       \
+    else ISDF_ZLP4(TL, false, ZS, false);                                                      \
   } while (0)
-#define ISDF_ZLPS(TL)                                                                          \
-  do {                                                                                         \
-    const unsigned nb =                                                                        \
-        grid_leaving_cus((long long)cdiv(ng, TL::BM) * cdiv(ldz, TL::BN), reserve_cus);        \
-    if (dual)                                                                                  \
-      hipLaunchKernelGGL((k_zct_lp<TL, true, true>), dim3(nb), dim3(TL::NT), 0, st, ng, rk,    \
-                         ldz, A1, A2, kp1, kp2, lda1, lda2, B1, B2, Zt);                       \
-    else                                                                                       \
-      hipLaunchKernelGGL((k_zct_lp<TL, false, true>), dim3(nb), dim3(TL::NT), 0, st, ng, rk,   \
-                         ldz, A1, A2, kp1, kp2, lda1, lda2, B1, B2, Zt);                       \
-  } while (0)
+#define ISDF_ZLP(TL) ISDF_ZLPX(TL, false)
+#define ISDF_ZLPS(TL) ISDF_ZLPX(TL, true)
   if (zsplit && (mode != 2 || variant < 3))
     return set_error(1, "pre-split Zt needs a BF16X3 direct-to-LDS Z C^T tile");
   if (mode == 1) {
@@ -1003,6 +997,8 @@ int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void*
   }
 #undef ISDF_ZLP
 #undef ISDF_ZLPS
+#undef ISDF_ZLPX
+#undef ISDF_ZLP4
   ISDF_HIP_LAUNCHED("k_zct_lp");
   return 0;
 }
