@@ -1,17 +1,20 @@
-// isdf.hpp — source-compatible drop-in for LibISDF's GPU entry point.
+// isdf.hpp — source-compatible drop-in for LibISDF's ISDF entry points.
 //
-// Replaces include/isdf.hpp of LibISDF for the GPU build (-DGPU):
+// Replaces include/isdf.hpp of LibISDF.  The GPU build (-DGPU) sees
 //   include/isdf.hpp:37  void isdf::isdffunc::ISDF(double *d_psirow, Domain domain,
 //                          int nocc, int nstate, int nv1, int nc1, int nv2, int nc2,
 //                          int mu_points, double *d_thetaCol, int *piv,
 //                          int KmeansMaxIter_ISDF, bool check=false);
-// together with the Domain type (include/common/domain.hpp:8-36), the isdfOFS
-// log stream and ErrorHandling (src/common/global.cpp:5-11).
+// and any other build the host-buffer interface of include/isdf.hpp:39, both
+// computed on the GPU by this library.  The header also provides the Domain
+// type (include/common/domain.hpp:8-36), the isdfOFS log stream and
+// ErrorHandling (src/common/global.cpp:5-11).
 //
 // Behaviour follows the reference at one rank: d_psirow is the caller's device
 // Ng x nstate column-major matrix, d_thetaCol the caller's device Ng x rk
 // matrix (column mu belongs to grid point piv[mu]), piv the caller's host
-// int[Ng].  The call blocks until the result is in place.  Errors are logged to
+// int[Ng]; the host interface takes the same matrices in host memory.  The
+// call blocks until the result is in place.  Errors are logged to
 // isdfOFS and thrown as std::runtime_error, as ErrorHandling does.
 //
 // Communicators: build with ISDF_WITH_MPI (and mpi.h on the include path) to
@@ -95,9 +98,27 @@ void ErrorHandling(const char* msg);
 
 namespace isdffunc {
 
+// As in the reference header, a -DGPU build sees the device-pointer ISDF and
+// any other build the host-pointer one; the library defines both
+// (ISDF_DECLARE_BOTH_INTERFACES).
+#if defined(GPU) || defined(ISDF_DECLARE_BOTH_INTERFACES)
 void ISDF(double* d_psirow, Domain domain, int nocc, int nstate, int nv1, int nc1, int nv2,
           int nc2, int mu_points, double* d_thetaCol, int* piv, int KmeansMaxIter_ISDF,
           bool check = false);
+#endif
+
+#if !defined(GPU) || defined(ISDF_DECLARE_BOTH_INTERFACES)
+// include/isdf.hpp:39 of the reference: host buffers (psirow Ng x nstate,
+// thetaCol Ng x rk, piv Ng; column-major, one rank), computed on the GPU.
+// s = "scalapack" or "MPI" (the same Theta at one rank); any other s only
+// zeroes piv, as the reference does.  s1 = "Kmeans"; "QRCP" is not built and
+// throws; anything else throws "Wrong select points method".  scalblocksize
+// (the ScaLAPACK block size) has no role at one rank and must be positive.
+void ISDF(double* psirow, Domain domain, int nocc, int nstate, int nv1, int nc1, int nv2,
+          int nc2, int mu_points, double* thetaCol, int* piv, int KmeansMaxIter_ISDF,
+          int scalblocksize = 64, std::string s = "scalapack", std::string s1 = "Kmeans",
+          bool check = false);
+#endif
 
 }  // namespace isdffunc
 }  // namespace isdf

@@ -104,6 +104,23 @@ typedef struct isdf_debug {
 int isdf_hip_solve(const isdf_problem* prob, const double* d_psi, double* d_theta, int* h_piv,
                    void* stream, isdf_report* rep, const isdf_debug* dbg);
 
+/* Full ISDF on host buffers: the CPU interface of LibISDF backed by the GPU.
+ * Replaces, at one rank, the non-GPU build's
+ *   include/isdf.hpp:39   void isdf::isdffunc::ISDF(double *psirow, Domain domain, ...,
+ *                             double *thetaCol, int *piv, int KmeansMaxIter_ISDF,
+ *                             int scalblocksize=64, std::string s="scalapack",
+ *                             std::string s1="Kmeans", bool check=false);
+ *   src/common/isdf.cpp:8-127 (its "scalapack" + "Kmeans" path)
+ * h_psi: host Ng x nstate doubles (ld_psi), read only; h_theta: host Ng x rk
+ * doubles (ld Ng), written; h_piv as in isdf_hip_solve.  The library copies
+ * Psi into a device buffer it keeps (grown as needed, freed by
+ * isdf_hip_release_workspace), runs the device solve of isdf_hip_solve with
+ * the same kernels (so Theta and piv equal that call's bit for bit), and
+ * copies Theta back.  Stage times in rep cover the device solve; the copies
+ * are outside them.  Synchronous. */
+int isdf_hip_solve_host(const isdf_problem* prob, const double* h_psi, double* h_theta,
+                        int* h_piv, void* stream, isdf_report* rep, const isdf_debug* dbg);
+
 /* rk the solve will use for this problem (the reference's rank rule). */
 int isdf_hip_rank(const isdf_problem* prob);
 

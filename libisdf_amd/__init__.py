@@ -18,7 +18,8 @@ from . import _lib
 from ._lib import (ISDF_EHIP, ISDF_EINVAL, ISDF_ENOMEM, ISDF_ENOTSPD, ISDF_OK,
                    ISDF_PREC_BF16X3, ISDF_PREC_FP32, ISDF_PREC_FP64, ISDF_PREC_REFINE)
 
-__all__ = ["Domain", "ISDF", "isdf_rank", "solve", "solve_batched", "check", "ISDF_PREC_FP64", "ISDF_PREC_FP32",
+__all__ = ["Domain", "ISDF", "ISDF_host", "isdf_rank", "solve", "solve_host", "solve_batched",
+           "check", "ISDF_PREC_FP64", "ISDF_PREC_FP32",
            "ISDF_PREC_BF16X3", "ISDF_PREC_REFINE"]
 
 
@@ -128,6 +129,44 @@ def solve(d_psi, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_theta, p
     return rep.as_dict()
 
 
+def solve_host(psi, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, theta, piv, max_iter,
+               precision=ISDF_PREC_FP64, debug=None, stream=0):
+    """isdf_hip_solve_host: the solve of solve() on host buffers.  psi: numpy
+    float64 (nstate, Ng) C-contiguous (the column-major Ng x nstate matrix);
+    theta: numpy float64 (rk, Ng) C-contiguous, written; piv as in solve().
+    debug: optional dict with 'trace' (numpy int32 array).  stream: a
+    hipStream_t as an int, 0 for the default stream."""
+    L = _lib.lib()
+    p = _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, max_iter, precision)
+    ng = domain.NumGridTotal()
+    rk = isdf_rank(nv1, nc1, nv2, nc2, mu_points)
+    for name, a, n in (("psi", psi, ng * max(nstate, 0)), ("theta", theta, ng * max(rk, 0))):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous
+                and a.size >= n):
+            raise ValueError("%s must be a contiguous numpy float64 array of %d entries"
+                             % (name, n))
+    if not (isinstance(piv, np.ndarray) and piv.dtype == np.int32 and piv.flags.c_contiguous
+            and piv.size >= ng):
+        raise ValueError("piv must be a contiguous numpy int32 array of Ng entries")
+    if theta.flags.writeable is False or piv.flags.writeable is False:
+        raise ValueError("theta and piv must be writeable")
+    rep = _lib.IsdfReport()
+    dbg = None
+    if debug and "trace" in debug:
+        dbg = _lib.IsdfDebug()
+        t = debug["trace"]
+        assert t.dtype == np.int32 and t.size >= max(max_iter, 1)
+        dbg.h_trace = t.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = L.isdf_hip_solve_host(ctypes.byref(p), psi.ctypes.data_as(dp), theta.ctypes.data_as(dp),
+                               piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                               ctypes.c_void_p(stream), ctypes.byref(rep),
+                               ctypes.byref(dbg) if dbg is not None else None)
+    if rc != ISDF_OK:
+        _raise(rc)
+    return rep.as_dict()
+
+
 def solve_batched(instances, precision=ISDF_PREC_FP32, stream=None):
     """Many independent instances in one call (isdf_hip_solve_batched; no
     counterpart in the reference).  instances: list of dicts with keys d_psi,
@@ -182,9 +221,45 @@ def check(psi, theta, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, 
 _validate = check
 
 
+def ISDF_host(psirow, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, thetaCol, piv,
+              KmeansMaxIter_ISDF, scalblocksize=64, s="scalapack", s1="Kmeans", check=False):
+    """Mirror of the CPU-interface isdf::isdffunc::ISDF (include/isdf.hpp:39,
+    src/common/isdf.cpp:8-127) at one rank, backed by the GPU solve.
+
+    psirow: host float64 numpy array, the column-major Ng x nstate matrix
+    ((nstate, Ng) C-contiguous).  thetaCol: host float64 numpy array (rk, Ng),
+    written.  piv: host int32 numpy array of Ng entries, zeroed and written.
+    s: "scalapack" or "MPI" (both give the same Theta at one rank; the
+    reference's two basis routes differ only in their distribution).  Any other
+    s does nothing after zeroing piv, as in the reference.  s1: "Kmeans";
+    "QRCP" raises (not built); anything else raises "Wrong select points
+    method" (isdf.cpp:73, :112).  scalblocksize is the ScaLAPACK block size of the
+    reference; it has no role in the one-rank solve and must be positive.
+    """
+    ng = domain.NumGridTotal()
+    if not (isinstance(piv, np.ndarray) and piv.dtype == np.int32 and piv.size >= ng):
+        raise ValueError("piv must be a numpy int32 array of Ng entries")
+    piv[:ng] = 0
+    if s not in ("scalapack", "MPI"):
+        return {}
+    if s1 == "QRCP":
+        raise RuntimeError("QRCP point selection is not built in libisdf_hip; use s1 = \"Kmeans\"")
+    if s1 != "Kmeans":
+        raise RuntimeError("Wrong select points method")
+    if scalblocksize <= 0:
+        raise ValueError("scalblocksize must be positive")
+    rep = solve_host(psirow, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, thetaCol, piv,
+                     KmeansMaxIter_ISDF)
+    if check:
+        rep.update(_validate(psirow, thetaCol, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2,
+                             mu_points))
+    return rep
+
+
 def ISDF(d_psirow, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_thetaCol, piv,
          KmeansMaxIter_ISDF, check=False, stream=None):
     """Drop-in mirror of isdf::isdffunc::ISDF (GPU, include/isdf.hpp:37).
+    (ISDF_host mirrors the CPU interface of include/isdf.hpp:39.)
 
     d_psirow: device float64 tensor holding the column-major Ng x nstate matrix
     (a (nstate, Ng) C-contiguous tensor).  d_thetaCol: device float64 tensor of

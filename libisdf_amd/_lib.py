@@ -15,6 +15,7 @@ ISDF_PREC_FP64, ISDF_PREC_FP32, ISDF_PREC_BF16X3, ISDF_PREC_REFINE = 0, 1, 2, 16
 # Every symbol the public header declares; tests check the library exports them.
 EXPORTS = (
     "isdf_hip_solve",
+    "isdf_hip_solve_host",
     "isdf_hip_rank",
     "isdf_hip_last_error",
     "isdf_hip_version",
@@ -117,6 +118,11 @@ def lib():
     L.isdf_hip_solve.argtypes = [ctypes.POINTER(IsdfProblem), ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.POINTER(ctypes.c_int), ctypes.c_void_p,
                                  ctypes.POINTER(IsdfReport), ctypes.POINTER(IsdfDebug)]
+    L.isdf_hip_solve_host.restype = ctypes.c_int
+    L.isdf_hip_solve_host.argtypes = [ctypes.POINTER(IsdfProblem), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_int), ctypes.c_void_p,
+                                      ctypes.POINTER(IsdfReport), ctypes.POINTER(IsdfDebug)]
     L.isdf_hip_rank.restype = ctypes.c_int
     L.isdf_hip_rank.argtypes = [ctypes.POINTER(IsdfProblem)]
     L.isdf_hip_last_error.restype = ctypes.c_char_p

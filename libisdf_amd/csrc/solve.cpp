@@ -32,12 +32,14 @@ struct Workspace {
   size_t bytes = 0;
 };
 std::vector<Workspace> g_ws;
+// Device copies of the caller's host Psi and Theta for isdf_hip_solve_host.
+std::vector<Workspace> g_stage;
 
-int workspace(size_t bytes, void** out) {
+int grow(std::vector<Workspace>& pool, size_t bytes, void** out) {
   int dev = 0;
   ISDF_HIP_TRY(hipGetDevice(&dev));
-  if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
-  Workspace& w = g_ws[dev];
+  if ((int)pool.size() <= dev) pool.resize(dev + 1);
+  Workspace& w = pool[dev];
   if (w.bytes < bytes) {
     if (w.ptr) ISDF_HIP_TRY(hipFree(w.ptr));
     w.ptr = nullptr;
@@ -53,6 +55,19 @@ int workspace(size_t bytes, void** out) {
   }
   *out = w.ptr;
   return 0;
+}
+
+int workspace(size_t bytes, void** out) { return grow(g_ws, bytes, out); }
+
+int free_pool(std::vector<Workspace>& pool) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return ISDF_EHIP;
+  if ((int)pool.size() > dev && pool[dev].ptr) {
+    if (hipFree(pool[dev].ptr) != hipSuccess) return ISDF_EHIP;
+    pool[dev].ptr = nullptr;
+    pool[dev].bytes = 0;
+  }
+  return ISDF_OK;
 }
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -562,6 +577,32 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   return ISDF_OK;
 }
 
+// isdf_hip_solve_host: the caller's host Psi goes to a cached device copy
+// (compact, ld Ng), the device solve runs as isdf_hip_solve does, and Theta
+// comes back into the caller's host buffer.
+int solve_host_locked(const isdf_problem* p, const double* h_psi, double* h_theta, int* h_piv,
+                      hipStream_t st, isdf_report* rep, const isdf_debug* dbg) {
+  long long ng = 0;
+  if (int rc = validate(p, &ng)) return rc;
+  if (!h_psi || !h_theta || !h_piv) return set_error(ISDF_EINVAL, "NULL buffer");
+  const int rk = irank(p);
+  const long long ld = p->ld_psi ? p->ld_psi : ng;
+  const size_t psi_bytes = align256(sizeof(double) * (size_t)ng * p->nstate);
+  const size_t theta_bytes = sizeof(double) * (size_t)ng * rk;
+  void* base = nullptr;
+  if (int rc = grow(g_stage, psi_bytes + theta_bytes, &base)) return rc;
+  double* d_psi = (double*)base;
+  double* d_theta = (double*)((char*)base + psi_bytes);
+  ISDF_HIP_TRY(hipMemcpy2DAsync(d_psi, sizeof(double) * ng, h_psi, sizeof(double) * ld,
+                                sizeof(double) * ng, p->nstate, hipMemcpyHostToDevice, st));
+  isdf_problem q = *p;
+  q.ld_psi = 0;
+  if (int rc = solve_locked(&q, d_psi, d_theta, h_piv, st, rep, dbg)) return rc;
+  ISDF_HIP_TRY(hipMemcpyAsync(h_theta, d_theta, theta_bytes, hipMemcpyDeviceToHost, st));
+  ISDF_HIP_TRY(hipStreamSynchronize(st));
+  return ISDF_OK;
+}
+
 }  // namespace
 
 int solve_single_locked(const isdf_problem* p, const double* d_psi, double* d_theta, int* h_piv,
@@ -601,6 +642,22 @@ int isdf_hip_solve(const isdf_problem* prob, const double* d_psi, double* d_thet
   return rc;
 }
 
+int isdf_hip_solve_host(const isdf_problem* prob, const double* h_psi, double* h_theta,
+                        int* h_piv, void* stream, isdf_report* rep, const isdf_debug* dbg) {
+  std::lock_guard<std::mutex> lk(isdf_hip::g_mutex);
+  isdf_hip::set_error(0, "");
+  const int rc =
+      isdf_hip::solve_host_locked(prob, h_psi, h_theta, h_piv, (hipStream_t)stream, rep, dbg);
+  if (rc == ISDF_EHIP) {
+    (void)hipStreamSynchronize((hipStream_t)stream);
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess && (int)isdf_hip::g_aux.size() > dev &&
+        isdf_hip::g_aux[dev].s)
+      (void)hipStreamSynchronize(isdf_hip::g_aux[dev].s);
+  }
+  return rc;
+}
+
 int isdf_hip_rank(const isdf_problem* prob) {
   if (!prob) return -1;
   return isdf_hip::irank(prob);
@@ -620,11 +677,9 @@ int isdf_hip_release_workspace(void) {
   isdf_hip::release_batch_plan();
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return ISDF_EHIP;
-  if ((int)isdf_hip::g_ws.size() > dev && isdf_hip::g_ws[dev].ptr) {
-    if (hipFree(isdf_hip::g_ws[dev].ptr) != hipSuccess) return ISDF_EHIP;
-    isdf_hip::g_ws[dev].ptr = nullptr;
-    isdf_hip::g_ws[dev].bytes = 0;
-  }
+  if (isdf_hip::free_pool(isdf_hip::g_ws) != ISDF_OK ||
+      isdf_hip::free_pool(isdf_hip::g_stage) != ISDF_OK)
+    return ISDF_EHIP;
   if ((int)isdf_hip::g_aux.size() > dev && isdf_hip::g_aux[dev].hpiv) {
     if (hipHostFree(isdf_hip::g_aux[dev].hpiv) != hipSuccess) return ISDF_EHIP;
     isdf_hip::g_aux[dev].hpiv = nullptr;

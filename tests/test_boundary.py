@@ -32,8 +32,12 @@ def test_library_exports_every_declared_symbol():
                          text=True).stdout
     for name in declared:
         assert re.search(r"\bT %s$" % name, out, re.M), name
-    # the C++ drop-in entry point, isdf::isdffunc::ISDF(double*, Domain, ...)
+    # the C++ drop-in entry points, isdf::isdffunc::ISDF(double*, Domain, ...): the
+    # device-pointer one (..., bool check) and the host-pointer one (..., int
+    # scalblocksize, std::string s, std::string s1, bool check)
     assert "_ZN4isdf8isdffunc4ISDFEPdNS_6DomainEiiiiiiiS1_Piib" in out
+    assert ("_ZN4isdf8isdffunc4ISDFEPdNS_6DomainEiiiiiiiS1_PiiiNSt7__cxx1112basic_stringIcSt11"
+            "char_traitsIcESaIcEEES9_b") in out
 
 
 def test_struct_layout_matches_c_header(tmp_path):
@@ -124,3 +128,63 @@ def test_batched_rejects_before_device_work(prec, msg):
     inst[0]["nv2"] = 4  # differing pair sets
     with pytest.raises(ValueError, match="pair sets|mixed forms"):
         isdf.solve_batched(inst, precision=isdf.ISDF_PREC_FP32, stream=0)
+
+
+def test_solve_host_rejects_before_device_work():
+    L = _lib.lib()
+    dom = isdf.Domain((10.0,) * 3, (8, 8, 8))
+    p = isdf._problem(dom, 8, 16, 8, 4, 8, 4, 64, 5, isdf.ISDF_PREC_FP64)
+    buf = np.zeros(512 * 64, np.float64)
+    piv = np.zeros(512, np.int32)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = L.isdf_hip_solve_host(ctypes.byref(p), None, buf.ctypes.data_as(dp),
+                               piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None, None, None)
+    assert rc == isdf.ISDF_EINVAL and "NULL buffer" in _lib.last_error()
+    p.nv1 = 9
+    rc = L.isdf_hip_solve_host(ctypes.byref(p), buf.ctypes.data_as(dp), buf.ctypes.data_as(dp),
+                               piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None, None, None)
+    assert rc == isdf.ISDF_EINVAL and "pair ranges" in _lib.last_error()
+    with pytest.raises(ValueError, match="theta must be"):
+        isdf.solve_host(np.zeros((16, 512)), dom, 8, 16, 8, 4, 8, 4, 64, np.zeros(10), piv, 5)
+
+
+def test_host_interface_dispatch_on_s_and_s1():
+    """isdf.cpp:44, :60-74, :98-113: piv is zeroed first; s1 other than Kmeans
+    or QRCP is "Wrong select points method"; an unknown s computes nothing."""
+    dom = isdf.Domain((10.0,) * 3, (8, 8, 8))
+    psi = np.zeros((16, 512))
+    theta = np.full((64, 512), 3.0)
+    piv = np.full(512, 5, np.int32)
+    assert isdf.ISDF_host(psi, dom, 8, 16, 8, 4, 8, 4, 64, theta, piv, 5, s="other") == {}
+    assert (piv == 0).all() and (theta == 3.0).all()
+    with pytest.raises(RuntimeError, match="Wrong select points method"):
+        isdf.ISDF_host(psi, dom, 8, 16, 8, 4, 8, 4, 64, theta, piv, 5, s1="Random")
+    with pytest.raises(RuntimeError, match="QRCP point selection is not built"):
+        isdf.ISDF_host(psi, dom, 8, 16, 8, 4, 8, 4, 64, theta, piv, 5, s="MPI", s1="QRCP")
+
+
+CALLER_HOST = os.path.join(ROOT, "tests", "dropin", "_build", "caller_host")
+
+
+@pytest.mark.parametrize("s,s1,status,msg", [
+    ("other", "Kmeans", 0, None),
+    ("scalapack", "Random", 2, "Wrong select points method"),
+    ("MPI", "QRCP", 2, "QRCP point selection is not built"),
+])
+def test_cpp_host_interface_dispatch(tmp_path, s, s1, status, msg):
+    """The C++ host-pointer ISDF (built from include/isdf.hpp without -DGPU) on
+    the paths that end before any device work: ErrorHandling's log lines and
+    exception, and the zeroed piv of an unknown s."""
+    ng, nstate, rk = 512, 16, 64
+    np.zeros(ng * nstate).tofile(tmp_path / "psi.bin")
+    args = [CALLER_HOST, str(tmp_path / "psi.bin"), "8", "8", "8", "10", "10", "10", "8",
+            str(nstate), "8", "4", "8", "4", str(rk), "5", "0", str(tmp_path / "theta.bin"),
+            str(tmp_path / "piv.bin"), str(tmp_path / "isdf.0"), s, s1, "32"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=60)
+    assert r.returncode == status, r.stderr
+    log = (tmp_path / "isdf.0").read_text()
+    assert log.startswith("rk=%d\n" % rk)
+    assert (np.fromfile(tmp_path / "piv.bin", np.int32) == 0).all()
+    assert (np.fromfile(tmp_path / "theta.bin") == 0.0).all()
+    if msg:
+        assert "ERROR!" in log and msg in log and msg in r.stderr
