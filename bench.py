@@ -46,7 +46,8 @@ def flops(w, rk):
     zct = 2.0 * w.ng * rk * k1 + (2.0 * w.ng * rk * k2 if dual else 0.0) + w.ng * rk
     potrs = 2.0 * w.ng * rk * rk
     potrf = rk ** 3 / 3.0
-    return dict(zct=zct, potrs=potrs, potrf=potrf)
+    fit = 2.0 * w.ng * rk * k1 * k2  # residual GEMM Theta (Ng x rk) x C (rk x K1K2)
+    return dict(zct=zct, potrs=potrs, potrf=potrf, fit=fit)
 
 
 def cpu_baseline(psi, w, iters, budget_s=20.0):
@@ -102,8 +103,6 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", dest="mixed", action="store_false",
                     help="skip the mixed-precision solves (FP32 and 3-way-split BF16)")
-    ap.add_argument("--fit-rows", type=int, default=2048,
-                    help="grid rows sampled for the fit-error comparison")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="run only a plain device copy of GIB GiB five times (the known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -206,25 +205,26 @@ def main():
 
     # Mixed forms on the same input, against the FP64 result of the timed run.
     if a.mixed:
-        rows = torch.from_numpy(np.sort(np.random.default_rng(0).choice(
-            w.ng, min(w.ng, a.fit_rows), replace=False))).to(dev)
-        fit64 = sampled_fit_error(torch, d_psi, d_theta, piv, w, rk, rows)
+        fit64, fit_ms = device_fit_error(torch, d_psi, d_theta, piv, dom, w)
         theta_norm = torch.linalg.norm(d_theta)
-        out["fit_error_fp64_sampled"] = fit64
+        out["fit_error_fp64"] = fit64
+        out["fit_error_kernel"] = {
+            "what": "isdf_hip_fit_error over all Ng rows and K1*K2 pairs (FP64 MFMA residual GEMM)",
+            "ms": fit_ms, "tflops": fl["fit"] / (fit_ms * 1e-3) / 1e12}
         mixed = []
         d_mix = torch.empty_like(d_theta)
         piv_m = np.zeros(w.ng, np.int32)
         for name, prec in (("bf16x3", isdf.ISDF_PREC_BF16X3), ("fp32", isdf.ISDF_PREC_FP32)):
             dtm, avgm, _ = timed(prec, d_mix, piv_m, max(1, a.steps // 2), 1)
             err = float(torch.linalg.norm(d_mix - d_theta) / theta_norm)
-            fitm = sampled_fit_error(torch, d_psi, d_mix, piv_m, w, rk, rows)
+            fitm, _ = device_fit_error(torch, d_psi, d_mix, piv_m, dom, w)
             steps_m = max(1, a.steps // 2)
             mixed.append({
                 "precision": name,
                 "solves_per_s": steps_m / dtm,
                 "speedup_vs_fp64": (steps_m / dtm) / (a.steps / dt),
                 "theta_rel_err_vs_fp64": err,
-                "fit_error_sampled": fitm,
+                "fit_error": fitm,
                 "fit_error_ratio_vs_fp64": fitm / fit64,
                 "piv_equal_fp64": bool(np.array_equal(piv_m, piv)),
                 "zct_tflops": fl["zct"] / (avgm["ms_zct"] * 1e-3) / 1e12,
@@ -329,18 +329,15 @@ def bench_batch(a):
     print(json.dumps(out))
 
 
-def sampled_fit_error(torch, d_psi, d_theta, piv, w, rk, rows):
-    """||Z - Theta C||_F / ||Z||_F over a fixed sample of grid rows (rows are
-    independent in both norms), Z(r, (i,j)) = psi_i(r) psi_j(r), C = the same
-    pair products at the pivots (src/common/checkcode.cpp:62-166)."""
-    s1, k1 = w.nocc - w.nv1, w.nv1 + w.nc1
-    s2, k2 = w.nocc - w.nv2, w.nv2 + w.nc2
-    pv = torch.from_numpy(piv[:rk].astype(np.int64)).to(d_psi.device)
-    a1, a2 = d_psi[s1:s1 + k1], d_psi[s2:s2 + k2]
-    zr = (a1[:, rows].T[:, :, None] * a2[:, rows].T[:, None, :]).reshape(len(rows), -1)
-    cm = (a1[:, pv].T[:, :, None] * a2[:, pv].T[:, None, :]).reshape(rk, -1)
-    th = d_theta[:, rows].T.contiguous()
-    return float(torch.linalg.norm(zr - th @ cm) / torch.linalg.norm(zr))
+def device_fit_error(torch, d_psi, d_theta, piv, dom, w):
+    """||Z - Theta C||_F / ||Z||_F over every grid row and pair
+    (src/common/checkcode.cpp:62-166), on the device; returns (fit, ms)."""
+    import libisdf_amd as isdf
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = isdf.fit_error(d_psi, d_theta, piv, dom, w.nocc, w.nstate, w.nv1, w.nc1, w.nv2, w.nc2,
+                       w.mu_points)
+    return r["fit_error"], (time.perf_counter() - t0) * 1e3
 
 
 if __name__ == "__main__":

@@ -143,6 +143,19 @@ typedef struct isdf_check {
 int isdf_hip_check(const isdf_problem* prob, const double* h_psi, const double* h_theta,
                    const int* h_piv, int do_coulomb, isdf_check* out);
 
+/* Fit error of a result on the device: the fit-error half of check=true,
+ * fit = ||Z - Theta C||_F / ||Z||_F over all Ng rows and K1*K2 pairs
+ * (src/common/checkcode.cpp:62-166).  d_psi and d_theta as isdf_hip_solve
+ * takes and writes them (device; Theta Ng x rk, ld Ng), h_piv host (the first
+ * rk entries are used and must be grid indices).  Z is never stored: the
+ * residual is formed in the epilogue of an FP64 MFMA GEMM Theta x C, so the
+ * cost is 2 Ng rk K1 K2 flops and the memory O(rk) columns of C at a time.
+ * Fills fit_error and z_norm; coulomb_error and coulomb_norm are set to -1.
+ * Deterministic.  Synchronous.  No counterpart entry point in LibISDF, whose
+ * validator runs on the host (isdf_hip_check is that one). */
+int isdf_hip_fit_error(const isdf_problem* prob, const double* d_psi, const double* d_theta,
+                       const int* h_piv, void* stream, isdf_check* out);
+
 /* Report of a batched solve. */
 typedef struct isdf_batch_report {
   double ms_total;        /* the whole batch, HIP events on the call's stream */
@@ -171,7 +184,8 @@ int isdf_hip_solve_batched(int nbatch, const isdf_problem* probs, const double* 
 /* Library version string. */
 const char* isdf_hip_version(void);
 
-/* Releases the cached device workspace of the calling device and the batch
+/* Releases the cached device buffers of the calling device (the solve
+ * workspace, the host-interface copies, the fit-error buffer) and the batch
  * plan (buffers and captured graph). */
 int isdf_hip_release_workspace(void);
 

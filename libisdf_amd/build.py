@@ -34,6 +34,7 @@ SOURCES = {
     "kmeans.hip": ["-ffp-contract=off"],
     "basis.hip": [],
     "lowprec.hip": [],
+    "fit.hip": [],
     "solve.cpp": ["-x", "hip"],
     "batch.cpp": ["-x", "hip"],
     "check.cpp": [],

@@ -613,6 +613,7 @@ int problem_rank(const isdf_problem* p) { return irank(p); }
 int validate_problem(const isdf_problem* p, long long* ng_out) { return validate(p, ng_out); }
 std::mutex& solve_mutex() { return g_mutex; }
 void release_batch_plan();
+int release_fit_buffer();
 
 int set_error(int code, const std::string& msg) {
   std::lock_guard<std::mutex> lk(g_err_mutex);
@@ -678,7 +679,8 @@ int isdf_hip_release_workspace(void) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return ISDF_EHIP;
   if (isdf_hip::free_pool(isdf_hip::g_ws) != ISDF_OK ||
-      isdf_hip::free_pool(isdf_hip::g_stage) != ISDF_OK)
+      isdf_hip::free_pool(isdf_hip::g_stage) != ISDF_OK ||
+      isdf_hip::release_fit_buffer() != ISDF_OK)
     return ISDF_EHIP;
   if ((int)isdf_hip::g_aux.size() > dev && isdf_hip::g_aux[dev].hpiv) {
     if (hipHostFree(isdf_hip::g_aux[dev].hpiv) != hipSuccess) return ISDF_EHIP;

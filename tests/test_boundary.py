@@ -188,3 +188,13 @@ def test_cpp_host_interface_dispatch(tmp_path, s, s1, status, msg):
     assert (np.fromfile(tmp_path / "theta.bin") == 0.0).all()
     if msg:
         assert "ERROR!" in log and msg in log and msg in r.stderr
+
+
+def test_fit_error_rejects_before_device_work():
+    dom = isdf.Domain((10.0,) * 3, (8, 8, 8))
+    piv = np.zeros(512, np.int32)
+    piv[3] = 512
+    with pytest.raises(ValueError, match=r"piv\[3\] is not a grid index"):
+        isdf.fit_error(1, 1, piv, dom, 8, 16, 8, 4, 8, 4, 64, stream=0)
+    with pytest.raises(ValueError, match="pair ranges"):
+        isdf.fit_error(1, 1, np.zeros(512, np.int32), dom, 8, 16, 9, 4, 8, 4, 64, stream=0)
