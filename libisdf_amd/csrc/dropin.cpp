@@ -3,7 +3,7 @@
 // The device-pointer overload mirrors the GPU driver of LibISDF,
 // src/common/isdf.cpp:132-209: rank rule, piv zeroing, the solve, the isdfOFS
 // log lines, and the check=true validator (src/common/checkcode.cpp, reached
-// from isdf.cpp:193-206).  Unlike the reference there is no row->column
+// from isdf.cpp:193-206), whose fit-error half runs on the device.  Unlike the reference there is no row->column
 // relayout: at one rank that relayout is the identity, so Theta is solved
 // directly in d_thetaCol.  The host-pointer overload mirrors the CPU driver,
 // src/common/isdf.cpp:8-127, with its s / s1 dispatch, on the same GPU solve.
@@ -19,6 +19,12 @@
 
 #include "../../include/isdf.hpp"
 #include "../../include/isdf_hip.h"
+
+namespace isdf_hip {
+// check.cpp: the host validator with either half optional.
+int check_host(const isdf_problem* p, const double* psi, const double* theta, const int* piv,
+               bool do_fit, bool do_coulomb, isdf_check* out);
+}  // namespace isdf_hip
 
 namespace isdf {
 
@@ -71,12 +77,8 @@ void log_solve(const isdf_report& rep, const std::vector<int>& trace) {
   isdfOFS << "Time for ISDF time =" << rep.ms_total * 1e-3 << " [s]" << std::endl;
 }
 
-// check=true on host copies: the error lines of checkcode.cpp:164-166 and :310-312.
-void check_and_log(const isdf_problem& p, const double* psi, const double* theta,
-                   const int* piv) {
-  isdf_check chk{};
-  const int rc = isdf_hip_check(&p, psi, theta, piv, 1, &chk);
-  if (rc != ISDF_OK) ErrorHandling(isdf_hip_last_error());
+// The error lines of checkcode.cpp:164-166 and :310-312.
+void log_check(const isdf_check& chk) {
   isdfOFS << "sqrt((rhoij-rho^hat)^2) =" << chk.fit_error * chk.z_norm << std::endl;
   isdfOFS << "sqrt(rhoij^2)= " << chk.z_norm << std::endl;
   isdfOFS << "sqrt((rhoij-rho^hat)^2)/sqrt(rhoij^2)= " << chk.fit_error << std::endl;
@@ -104,6 +106,11 @@ void ISDF(double* d_psirow, Domain domain, int nocc, int nstate, int nv1, int nc
   log_solve(rep, trace);
 
   if (check) {
+    // Fit error on the device (all rows and pairs); the Coulomb half on host
+    // copies, as checkcode does it.
+    isdf_check fit{}, chk{};
+    rc = isdf_hip_fit_error(&p, d_psirow, d_thetaCol, piv, nullptr, &fit);
+    if (rc != ISDF_OK) ErrorHandling(isdf_hip_last_error());
     const long long ng = (long long)p.ngrid[0] * p.ngrid[1] * p.ngrid[2];
     std::vector<double> psi((size_t)(ng * nstate)), theta((size_t)(ng * rk));
     if (hipMemcpy(psi.data(), d_psirow, sizeof(double) * psi.size(), hipMemcpyDeviceToHost) !=
@@ -111,7 +118,11 @@ void ISDF(double* d_psirow, Domain domain, int nocc, int nstate, int nv1, int nc
         hipMemcpy(theta.data(), d_thetaCol, sizeof(double) * theta.size(),
                   hipMemcpyDeviceToHost) != hipSuccess)
       ErrorHandling("check: device to host copy failed");
-    check_and_log(p, psi.data(), theta.data(), piv);
+    rc = isdf_hip::check_host(&p, psi.data(), theta.data(), piv, false, true, &chk);
+    if (rc != ISDF_OK) ErrorHandling(isdf_hip_last_error());
+    chk.fit_error = fit.fit_error;
+    chk.z_norm = fit.z_norm;
+    log_check(chk);
   }
 }
 
@@ -140,7 +151,12 @@ void ISDF(double* psirow, Domain domain, int nocc, int nstate, int nv1, int nc1,
   int rc = isdf_hip_solve_host(&p, psirow, thetaCol, piv, nullptr, &rep, &dbg);
   if (rc != ISDF_OK) ErrorHandling(isdf_hip_last_error());
   log_solve(rep, trace);
-  if (check) check_and_log(p, psirow, thetaCol, piv);
+  if (check) {
+    isdf_check chk{};
+    rc = isdf_hip_check(&p, psirow, thetaCol, piv, 1, &chk);
+    if (rc != ISDF_OK) ErrorHandling(isdf_hip_last_error());
+    log_check(chk);
+  }
 }
 
 }  // namespace isdffunc
