@@ -164,6 +164,19 @@ def main():
     potrs_tf = fl["potrs"] / (avg["ms_potrs"] * 1e-3) / 1e12
     zct_tf = fl["zct"] / (avg["ms_zct"] * 1e-3) / 1e12
     traffic = TRAFFIC.get(a.workload)
+    # The same FP64 solve with no CUs left to the concurrent Cholesky branch:
+    # the ZC^T kernel then holds the whole chip while it runs (its standalone
+    # rate); the default solve trades some ZC^T rate for the overlap.
+    saved = os.environ.get("ISDF_ZCT_RESERVE_CUS")
+    os.environ["ISDF_ZCT_RESERVE_CUS"] = "0"
+    try:
+        _, avg_alone, _ = timed(isdf.ISDF_PREC_FP64, d_theta, piv, 2, 1)
+    finally:
+        if saved is None:
+            del os.environ["ISDF_ZCT_RESERVE_CUS"]
+        else:
+            os.environ["ISDF_ZCT_RESERVE_CUS"] = saved
+    zct_alone_tf = fl["zct"] / (avg_alone["ms_zct"] * 1e-3) / 1e12
     out = {
         "metric": METRIC,
         "value": a.steps / dt,
@@ -188,6 +201,10 @@ def main():
             "rk_kmeans": r0["rk_kmeans"],
         },
         "zct_tflops": zct_tf,
+        "zct_alone": {"tflops": zct_alone_tf, "frac_fp64_peak": zct_alone_tf / FP64_PEAK_TFLOPS,
+                      "ms": avg_alone["ms_zct"],
+                      "note": "FP64 ZC^T with no CUs reserved for the concurrent Cholesky "
+                              "branch (ISDF_ZCT_RESERVE_CUS=0); zct_tflops is the in-solve rate"},
         "stage_ms": {k[3:]: round(v, 4) for k, v in avg.items()},
         "roofline": {
             "kernel": "k_potrs (blocked in-place triangular solves, FP64 MFMA)",
