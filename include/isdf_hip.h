@@ -156,6 +156,18 @@ int isdf_hip_check(const isdf_problem* prob, const double* h_psi, const double* 
 int isdf_hip_fit_error(const isdf_problem* prob, const double* d_psi, const double* d_theta,
                        const int* h_piv, void* stream, isdf_check* out);
 
+/* check=true on the device: the fit error of isdf_hip_fit_error and, with
+ * do_coulomb != 0, the Coulomb error of checkcode.cpp:168-312 (same fields as
+ * isdf_hip_check).  The Coulomb half takes the forward 3-D DFT of the K1*K2
+ * pair densities and the rk Theta columns (direct per-axis sums), scales
+ * them by sqrt(V(G)), and forms ijVkl and Theta^T V Theta as FP64 MFMA Gram
+ * products over 2 Ng real rows, then ||ijVkl - C^T M C||_F.  Cost about
+ * 4 (K1 K2)^2 Ng flops; device memory about 8 (2 Ng (K1 K2 + rk) + 3 (K1 K2)^2)
+ * bytes (at most 65536 pairs).  No counterpart entry point in LibISDF, whose
+ * validator runs on the host.  Deterministic.  Synchronous. */
+int isdf_hip_check_device(const isdf_problem* prob, const double* d_psi, const double* d_theta,
+                          const int* h_piv, int do_coulomb, void* stream, isdf_check* out);
+
 /* Report of a batched solve. */
 typedef struct isdf_batch_report {
   double ms_total;        /* the whole batch, HIP events on the call's stream */

@@ -19,7 +19,7 @@ from ._lib import (ISDF_EHIP, ISDF_EINVAL, ISDF_ENOMEM, ISDF_ENOTSPD, ISDF_OK,
                    ISDF_PREC_BF16X3, ISDF_PREC_FP32, ISDF_PREC_FP64, ISDF_PREC_REFINE)
 
 __all__ = ["Domain", "ISDF", "ISDF_host", "isdf_rank", "solve", "solve_host", "solve_batched",
-           "check", "fit_error", "ISDF_PREC_FP64", "ISDF_PREC_FP32",
+           "check", "check_device", "fit_error", "ISDF_PREC_FP64", "ISDF_PREC_FP32",
            "ISDF_PREC_BF16X3", "ISDF_PREC_REFINE"]
 
 
@@ -219,6 +219,26 @@ def check(psi, theta, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, 
 
 
 _validate = check
+
+
+def check_device(d_psi, d_theta, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points,
+                 coulomb=True, stream=None, ld_psi=0):
+    """isdf_hip_check_device: check=true on the device (checkcode.cpp:54-313),
+    the fit error and, with coulomb=True, the FFT Coulomb error.  Arguments as
+    fit_error().  Returns the dict check() returns."""
+    L = _lib.lib()
+    p = _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, 0, ISDF_PREC_FP64, ld_psi)
+    piv = np.ascontiguousarray(piv, np.int32)
+    out = _lib.IsdfCheck()
+    rc = L.isdf_hip_check_device(ctypes.byref(p), ctypes.c_void_p(_dev_ptr(d_psi)),
+                                 ctypes.c_void_p(_dev_ptr(d_theta)),
+                                 piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                 int(bool(coulomb)), ctypes.c_void_p(_stream_ptr(stream)),
+                                 ctypes.byref(out))
+    if rc != ISDF_OK:
+        _raise(rc)
+    return dict(fit_error=out.fit_error, z_norm=out.z_norm, coulomb_error=out.coulomb_error,
+                coulomb_norm=out.coulomb_norm)
 
 
 def fit_error(d_psi, d_theta, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points,

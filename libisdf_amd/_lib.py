@@ -22,6 +22,7 @@ EXPORTS = (
     "isdf_hip_release_workspace",
     "isdf_hip_check",
     "isdf_hip_fit_error",
+    "isdf_hip_check_device",
     "isdf_hip_solve_batched",
 )
 
@@ -140,6 +141,10 @@ def lib():
     L.isdf_hip_fit_error.argtypes = [ctypes.POINTER(IsdfProblem), ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
                                      ctypes.c_void_p, ctypes.POINTER(IsdfCheck)]
+    L.isdf_hip_check_device.restype = ctypes.c_int
+    L.isdf_hip_check_device.argtypes = [ctypes.POINTER(IsdfProblem), ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(IsdfCheck)]
     L.isdf_hip_solve_batched.restype = ctypes.c_int
     L.isdf_hip_solve_batched.argtypes = [
         ctypes.c_int, ctypes.POINTER(IsdfProblem), ctypes.POINTER(ctypes.c_void_p),

@@ -35,6 +35,7 @@ SOURCES = {
     "basis.hip": [],
     "lowprec.hip": [],
     "fit.hip": [],
+    "coulomb.hip": [],
     "solve.cpp": ["-x", "hip"],
     "batch.cpp": ["-x", "hip"],
     "check.cpp": [],

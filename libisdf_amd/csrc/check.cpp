@@ -122,11 +122,9 @@ std::vector<double> coulomb_kernel(const int ng[3], const double len[3]) {
 
 }  // namespace
 
-namespace isdf_hip {
-
-// The validator with either half optional; a skipped half leaves its fields at -1.
-int check_host(const isdf_problem* p, const double* psi, const double* theta, const int* piv,
-               bool do_fit, bool do_coulomb, isdf_check* out) {
+extern "C" int isdf_hip_check(const isdf_problem* p, const double* psi, const double* theta,
+                              const int* piv, int do_coulomb, isdf_check* out) {
+  using isdf_hip::set_error;
   if (!p || !psi || !theta || !piv || !out) return set_error(ISDF_EINVAL, "NULL argument");
   const long long ng = (long long)p->ngrid[0] * p->ngrid[1] * p->ngrid[2];
   const long long ld = p->ld_psi ? p->ld_psi : ng;
@@ -138,10 +136,8 @@ int check_host(const isdf_problem* p, const double* psi, const double* theta, co
   const long long npair = (long long)K1 * K2;
   const int nt = nthreads();
 
-  out->fit_error = -1.0;
-  out->z_norm = -1.0;
   // Fit error, checkcode.cpp:62-166.
-  if (do_fit) {
+  {
     const int blk = 32;
     std::vector<double> cm((size_t)rk * blk);
     std::vector<double> e_t(nt), z_t(nt);
@@ -275,11 +271,4 @@ int check_host(const isdf_problem* p, const double* psi, const double* theta, co
   out->coulomb_error = std::sqrt(e) / std::sqrt(nn);
   out->coulomb_norm = std::sqrt(nn);
   return ISDF_OK;
-}
-
-}  // namespace isdf_hip
-
-extern "C" int isdf_hip_check(const isdf_problem* p, const double* psi, const double* theta,
-                              const int* piv, int do_coulomb, isdf_check* out) {
-  return isdf_hip::check_host(p, psi, theta, piv, true, do_coulomb != 0, out);
 }

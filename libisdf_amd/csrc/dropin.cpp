@@ -3,7 +3,7 @@
 // The device-pointer overload mirrors the GPU driver of LibISDF,
 // src/common/isdf.cpp:132-209: rank rule, piv zeroing, the solve, the isdfOFS
 // log lines, and the check=true validator (src/common/checkcode.cpp, reached
-// from isdf.cpp:193-206), whose fit-error half runs on the device.  Unlike the reference there is no row->column
+// from isdf.cpp:193-206), which runs on the device (isdf_hip_check_device).  Unlike the reference there is no row->column
 // relayout: at one rank that relayout is the identity, so Theta is solved
 // directly in d_thetaCol.  The host-pointer overload mirrors the CPU driver,
 // src/common/isdf.cpp:8-127, with its s / s1 dispatch, on the same GPU solve.
@@ -19,12 +19,6 @@
 
 #include "../../include/isdf.hpp"
 #include "../../include/isdf_hip.h"
-
-namespace isdf_hip {
-// check.cpp: the host validator with either half optional.
-int check_host(const isdf_problem* p, const double* psi, const double* theta, const int* piv,
-               bool do_fit, bool do_coulomb, isdf_check* out);
-}  // namespace isdf_hip
 
 namespace isdf {
 
@@ -106,22 +100,10 @@ void ISDF(double* d_psirow, Domain domain, int nocc, int nstate, int nv1, int nc
   log_solve(rep, trace);
 
   if (check) {
-    // Fit error on the device (all rows and pairs); the Coulomb half on host
-    // copies, as checkcode does it.
-    isdf_check fit{}, chk{};
-    rc = isdf_hip_fit_error(&p, d_psirow, d_thetaCol, piv, nullptr, &fit);
+    // Both halves of checkcode on the device, on the caller's buffers.
+    isdf_check chk{};
+    rc = isdf_hip_check_device(&p, d_psirow, d_thetaCol, piv, 1, nullptr, &chk);
     if (rc != ISDF_OK) ErrorHandling(isdf_hip_last_error());
-    const long long ng = (long long)p.ngrid[0] * p.ngrid[1] * p.ngrid[2];
-    std::vector<double> psi((size_t)(ng * nstate)), theta((size_t)(ng * rk));
-    if (hipMemcpy(psi.data(), d_psirow, sizeof(double) * psi.size(), hipMemcpyDeviceToHost) !=
-            hipSuccess ||
-        hipMemcpy(theta.data(), d_thetaCol, sizeof(double) * theta.size(),
-                  hipMemcpyDeviceToHost) != hipSuccess)
-      ErrorHandling("check: device to host copy failed");
-    rc = isdf_hip::check_host(&p, psi.data(), theta.data(), piv, false, true, &chk);
-    if (rc != ISDF_OK) ErrorHandling(isdf_hip_last_error());
-    chk.fit_error = fit.fit_error;
-    chk.z_norm = fit.z_norm;
     log_check(chk);
   }
 }

@@ -201,8 +201,11 @@ int fit_buffer(size_t bytes, void** out) {
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-int fit_error_locked(const isdf_problem* p, const double* d_psi, const double* d_theta,
-                     const int* h_piv, hipStream_t st, isdf_check* out) {
+}  // namespace
+
+// The fit error; the caller holds the library lock.
+int fit_error_unlocked(const isdf_problem* p, const double* d_psi, const double* d_theta,
+                       const int* h_piv, hipStream_t st, isdf_check* out) {
   long long ng = 0;
   if (int rc = validate_problem(p, &ng)) return rc;
   if (!d_psi || !d_theta || !h_piv || !out) return set_error(ISDF_EINVAL, "NULL argument");
@@ -264,8 +267,6 @@ int fit_error_locked(const isdf_problem* p, const double* d_psi, const double* d
   return ISDF_OK;
 }
 
-}  // namespace
-
 int release_fit_buffer() {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return ISDF_EHIP;
@@ -285,7 +286,7 @@ extern "C" int isdf_hip_fit_error(const isdf_problem* prob, const double* d_psi,
   std::lock_guard<std::mutex> lk(isdf_hip::solve_mutex());
   isdf_hip::set_error(0, "");
   const int rc =
-      isdf_hip::fit_error_locked(prob, d_psi, d_theta, h_piv, (hipStream_t)stream, out);
+      isdf_hip::fit_error_unlocked(prob, d_psi, d_theta, h_piv, (hipStream_t)stream, out);
   if (rc == ISDF_EHIP) (void)hipStreamSynchronize((hipStream_t)stream);
   return rc;
 }

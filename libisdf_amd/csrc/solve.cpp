@@ -614,6 +614,7 @@ int validate_problem(const isdf_problem* p, long long* ng_out) { return validate
 std::mutex& solve_mutex() { return g_mutex; }
 void release_batch_plan();
 int release_fit_buffer();
+int release_coulomb_buffer();
 
 int set_error(int code, const std::string& msg) {
   std::lock_guard<std::mutex> lk(g_err_mutex);
@@ -680,7 +681,8 @@ int isdf_hip_release_workspace(void) {
   if (hipGetDevice(&dev) != hipSuccess) return ISDF_EHIP;
   if (isdf_hip::free_pool(isdf_hip::g_ws) != ISDF_OK ||
       isdf_hip::free_pool(isdf_hip::g_stage) != ISDF_OK ||
-      isdf_hip::release_fit_buffer() != ISDF_OK)
+      isdf_hip::release_fit_buffer() != ISDF_OK ||
+      isdf_hip::release_coulomb_buffer() != ISDF_OK)
     return ISDF_EHIP;
   if ((int)isdf_hip::g_aux.size() > dev && isdf_hip::g_aux[dev].hpiv) {
     if (hipHostFree(isdf_hip::g_aux[dev].hpiv) != hipSuccess) return ISDF_EHIP;

@@ -92,3 +92,39 @@ def test_fit_error_deterministic_cfg1():
     theta = d_theta.cpu().numpy()
     fo = oracle.fit_error(psi, w.nocc, w.nv1, w.nc1, w.nv2, w.nc2, theta, piv)
     assert fit["fit_error"] == pytest.approx(fo, rel=1e-11)
+
+
+def test_fit_error_and_host_solve_with_strided_psi(si8):
+    """ld_psi > Ng: psi rows sit in a wider buffer (isdf_problem.ld_psi), for the
+    device fit error and for the host-buffer solve."""
+    psi, ref = si8
+    a = ref["args"]
+    nstate, ng = psi.shape
+    L, G, rk = (10.216,) * 3, (16, 16, 16), a["mu_points"]
+    dom = isdf.Domain(L, G)
+    wide = np.full((nstate, ng + 77), np.nan)
+    wide[:, :ng] = psi
+    d_psi = torch.from_numpy(np.ascontiguousarray(psi)).cuda()
+    d_wide = torch.from_numpy(wide).cuda()
+    d_theta = torch.empty((rk, ng), dtype=torch.float64, device="cuda")
+    piv = np.zeros(ng, np.int32)
+    isdf.solve(d_psi, dom, a["nocc"], nstate, a["nv1"], a["nc1"], a["nv2"], a["nc2"], rk, d_theta,
+               piv, a["kmeans_max_iter"])
+    torch.cuda.synchronize()
+    f = isdf.fit_error(d_psi, d_theta, piv, dom, a["nocc"], nstate, a["nv1"], a["nc1"], a["nv2"],
+                       a["nc2"], rk)
+    fw = isdf.fit_error(d_wide, d_theta, piv, dom, a["nocc"], nstate, a["nv1"], a["nc1"],
+                        a["nv2"], a["nc2"], rk, ld_psi=ng + 77)
+    assert fw == f
+    L_ = isdf._lib.lib()
+    p = isdf._problem(dom, a["nocc"], nstate, a["nv1"], a["nc1"], a["nv2"], a["nc2"], rk,
+                      a["kmeans_max_iter"], isdf.ISDF_PREC_FP64, ng + 77)
+    th = np.empty((rk, ng))
+    piv2 = np.zeros(ng, np.int32)
+    import ctypes
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = L_.isdf_hip_solve_host(ctypes.byref(p), wide.ctypes.data_as(dp), th.ctypes.data_as(dp),
+                                piv2.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None, None,
+                                None)
+    assert rc == 0, isdf._lib.last_error()
+    assert np.array_equal(piv2, piv) and np.array_equal(th, d_theta.cpu().numpy())
