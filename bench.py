@@ -106,6 +106,10 @@ def main():
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="run only a plain device copy of GIB GiB five times (the known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--check", action="store_true",
+                    help="instead of the solve benchmark: time check=true on the device (fit and "
+                         "Coulomb errors) against the host validator on the workload's FP64 "
+                         "result (small workloads: the Coulomb half costs ~4 (K1 K2)^2 Ng flops)")
     a = ap.parse_args()
     if a.copy_probe > 0:
         import torch
@@ -130,6 +134,8 @@ def main():
 
     if a.workload == "cfg5":
         return bench_batch(a)
+    if a.check:
+        return bench_check(a)
     w = synth.WORKLOADS[a.workload]
     psi = synth.wavefunctions(w)
     rk = isdf.isdf_rank(w.nv1, w.nc1, w.nv2, w.nc2, w.mu_points)
@@ -253,6 +259,42 @@ def main():
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(psi, w, r0["kmeans_iters"])
     print(json.dumps(out))
+
+
+def bench_check(a):
+    """check=true (checkcode.cpp:54-313) on the FP64 result of one solve:
+    isdf_hip_check_device (fit + FFT Coulomb errors) timed over --steps calls,
+    then the host validator once, on the same Theta and piv."""
+    import torch
+    import libisdf_amd as isdf
+    from libisdf_amd import synth
+    w = synth.WORKLOADS[a.workload]
+    psi = synth.wavefunctions(w)
+    rk = isdf.isdf_rank(w.nv1, w.nc1, w.nv2, w.nc2, w.mu_points)
+    d_psi = torch.from_numpy(psi).cuda()
+    d_theta = torch.empty((rk, w.ng), dtype=torch.float64, device="cuda")
+    piv = np.zeros(w.ng, np.int32)
+    dom = isdf.Domain(w.length, w.ngrid)
+    args = (w.nocc, w.nstate, w.nv1, w.nc1, w.nv2, w.nc2, w.mu_points)
+    isdf.solve(d_psi, dom, *args[:6], w.mu_points, d_theta, piv, w.kmeans_iters)
+    for _ in range(a.warmup):
+        isdf.check_device(d_psi, d_theta, piv, dom, *args)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        dev = isdf.check_device(d_psi, d_theta, piv, dom, *args)
+    dt = (time.perf_counter() - t0) / a.steps
+    th = d_theta.cpu().numpy()
+    t0 = time.perf_counter()
+    host = isdf.check(psi, th, piv, dom, *args)
+    dt_host = time.perf_counter() - t0
+    print(json.dumps({
+        "what": "check=true on the device (isdf_hip_check_device) vs the host validator",
+        "workload": w.name, "ng": w.ng, "pairs": (w.nv1 + w.nc1) * (w.nv2 + w.nc2), "rk": rk,
+        "device_ms": dt * 1e3, "host_s": dt_host, "device": dev, "host": host,
+        "coulomb_rel_diff": abs(dev["coulomb_error"] - host["coulomb_error"])
+        / host["coulomb_error"],
+        "fit_rel_diff": abs(dev["fit_error"] - host["fit_error"]) / host["fit_error"]}))
 
 
 def bench_batch(a):
