@@ -110,6 +110,9 @@ def main():
                     help="instead of the solve benchmark: time check=true on the device (fit and "
                          "Coulomb errors) against the host validator on the workload's FP64 "
                          "result (small workloads: the Coulomb half costs ~4 (K1 K2)^2 Ng flops)")
+    ap.add_argument("--profile-solve", choices=("fp64", "fp32", "bf16x3"), default=None,
+                    help="instead of the benchmark: one solve of the workload in this precision "
+                         "and nothing else (a fixed program for rocprofv3 counter passes)")
     a = ap.parse_args()
     if a.copy_probe > 0:
         import torch
@@ -136,6 +139,21 @@ def main():
         return bench_batch(a)
     if a.check:
         return bench_check(a)
+    if a.profile_solve:
+        w = synth.WORKLOADS[a.workload]
+        psi = synth.wavefunctions(w)
+        rk = isdf.isdf_rank(w.nv1, w.nc1, w.nv2, w.nc2, w.mu_points)
+        prec = {"fp64": isdf.ISDF_PREC_FP64, "fp32": isdf.ISDF_PREC_FP32,
+                "bf16x3": isdf.ISDF_PREC_BF16X3}[a.profile_solve]
+        d_psi = torch.from_numpy(psi).cuda()
+        d_theta = torch.empty((rk, w.ng), dtype=torch.float64, device="cuda")
+        piv = np.zeros(w.ng, np.int32)
+        rep = isdf.solve(d_psi, isdf.Domain(w.length, w.ngrid), w.nocc, w.nstate, w.nv1, w.nc1,
+                         w.nv2, w.nc2, w.mu_points, d_theta, piv, w.kmeans_iters, precision=prec)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_solve": a.profile_solve, "workload": w.name,
+                          "stage_ms": {k[3:]: v for k, v in rep.items() if k.startswith("ms_")}}))
+        return
     w = synth.WORKLOADS[a.workload]
     psi = synth.wavefunctions(w)
     rk = isdf.isdf_rank(w.nv1, w.nc1, w.nv2, w.nc2, w.mu_points)
@@ -228,20 +246,25 @@ def main():
 
     # Mixed forms on the same input, against the FP64 result of the timed run.
     if a.mixed:
-        fit64, fit_ms = device_fit_error(torch, d_psi, d_theta, piv, dom, w)
+        # Timed mixed solves first; the full-size fit errors (seconds of sustained FP64 MFMA
+        # work at 96^3) come after every timed run, so they cannot heat or throttle it.
         theta_norm = torch.linalg.norm(d_theta)
+        steps_m = max(1, a.steps // 2)
+        runs = []
+        for name, prec in (("bf16x3", isdf.ISDF_PREC_BF16X3), ("fp32", isdf.ISDF_PREC_FP32)):
+            d_mix = torch.empty_like(d_theta)
+            piv_m = np.zeros(w.ng, np.int32)
+            dtm, avgm, _ = timed(prec, d_mix, piv_m, steps_m, 1)
+            err = float(torch.linalg.norm(d_mix - d_theta) / theta_norm)
+            runs.append((name, dtm, avgm, err, d_mix, piv_m))
+        fit64, fit_ms = device_fit_error(torch, d_psi, d_theta, piv, dom, w)
         out["fit_error_fp64"] = fit64
         out["fit_error_kernel"] = {
             "what": "isdf_hip_fit_error over all Ng rows and K1*K2 pairs (FP64 MFMA residual GEMM)",
             "ms": fit_ms, "tflops": fl["fit"] / (fit_ms * 1e-3) / 1e12}
         mixed = []
-        d_mix = torch.empty_like(d_theta)
-        piv_m = np.zeros(w.ng, np.int32)
-        for name, prec in (("bf16x3", isdf.ISDF_PREC_BF16X3), ("fp32", isdf.ISDF_PREC_FP32)):
-            dtm, avgm, _ = timed(prec, d_mix, piv_m, max(1, a.steps // 2), 1)
-            err = float(torch.linalg.norm(d_mix - d_theta) / theta_norm)
+        for name, dtm, avgm, err, d_mix, piv_m in runs:
             fitm, _ = device_fit_error(torch, d_psi, d_mix, piv_m, dom, w)
-            steps_m = max(1, a.steps // 2)
             mixed.append({
                 "precision": name,
                 "solves_per_s": steps_m / dtm,
@@ -254,7 +277,7 @@ def main():
                 "theta_tflops": fl["potrs"] / (avgm["ms_potrs"] * 1e-3) / 1e12,
                 "stage_ms": {k[3:]: round(v, 4) for k, v in avgm.items()},
             })
-        del d_mix
+        del runs
         out["mixed"] = mixed
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(psi, w, r0["kmeans_iters"])

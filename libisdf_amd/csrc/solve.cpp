@@ -263,9 +263,21 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
   hipStream_t s2 = aux->s;
   ISDF_HIP_TRY(hipEventRecord(aux->fork, st));
   ISDF_HIP_TRY(hipStreamWaitEvent(s2, aux->fork, 0));
+  const int zv = env_variant("ISDF_LP_ZCT_TILE");  // the format of the packed rows follows it
+  const int tv = mixed_theta_variant(mode, refine, dual);
+  const bool zsplit = lp_theta_zt_split(mode, tv);
+  // Main stream first: Z C^T once the packed psi rows are in place.  It is
+  // queued before the helper branch (rk/64 Cholesky steps of three launches
+  // each, then the inverse): queued after it, Z C^T would start only once the
+  // host had queued the whole branch.
+  ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->pack, 0));
+  if (int rc = launch_zct_lp(mode, ng, rk, ldz, m.A1, m.A2, kp1, dual ? kp2 : kp1, m.B1, m.B2,
+                             dual, m.Zt, zv, st,
+                             reserve_cus(true, ng, rk, kp1 + (dual ? kp2 : 0)), zsplit))
+    return rc;
+  if (int rc = ev.rec(5, st)) return rc;
   if (int rc = ev.rec(6, s2)) return rc;
   // C C^T: the contraction on the pivot rows alone (rk x rk x K).
-  const int zv = env_variant("ISDF_LP_ZCT_TILE");  // the format of the packed rows follows it
   if (int rc = launch_zct_lp(mode, rk, rk, ldz, m.B1, m.B2, kp1, dual ? kp2 : kp1, m.B1, m.B2,
                              dual, m.Pz, zv, s2))
     return rc;
@@ -280,20 +292,10 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
   ISDF_HIP_TRY(hipMemsetAsync(info_dev, 0, sizeof(int), s2));
   if (int rc = launch_potrf(P, rk, Dinv, info_dev, s2)) return rc;
   if (int rc = launch_make_fg(P, rk, Dinv, Dinv128, Ft, Gt, s2)) return rc;
-  const int tv = mixed_theta_variant(mode, refine, dual);
-  const bool zsplit = lp_theta_zt_split(mode, tv);
   if (int rc = launch_inverse(lp_theta_w_mode(mode, tv), rk, Ft, Gt, m.W, ldz, m.Wf, s2))
     return rc;
   if (int rc = ev.rec(8, s2)) return rc;
   ISDF_HIP_TRY(hipEventRecord(aux->join, s2));
-
-  // Main stream: Z C^T once the packed psi rows are in place.
-  ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->pack, 0));
-  if (int rc = launch_zct_lp(mode, ng, rk, ldz, m.A1, m.A2, kp1, dual ? kp2 : kp1, m.B1, m.B2,
-                             dual, m.Zt, zv, st,
-                             reserve_cus(true, ng, rk, kp1 + (dual ? kp2 : 0)), zsplit))
-    return rc;
-  if (int rc = ev.rec(5, st)) return rc;
   ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->join, 0));
   if (int rc = ev.rec(9, st)) return rc;
   if (int rc = launch_theta_lp(mode, ng, rk, ldz, m.Wf, m.Zt, d_theta, ng, false, tv, st))
@@ -515,9 +517,17 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
     hipStream_t s2s = aux->s;
     ISDF_HIP_TRY(hipEventRecord(aux->fork, st));
     ISDF_HIP_TRY(hipStreamWaitEvent(s2s, aux->fork, 0));
-    if (int rc = ev.rec(6, s2s)) return rc;
-    // Same tile as the Ng-row Z C^T below, so P equals its pivot rows.
+    // Main stream: Z C^T into d_theta (getbasis_scalapack.cpp:297-369).  It is
+    // queued before the helper branch (rk/64 Cholesky steps of three launches
+    // each): queued after it, Z C^T would start only once the host had queued
+    // the whole branch.
     const int zv = zct_pick_variant(ng64, env_variant("ISDF_ZCT_TILE"));
+    if (int rc = launch_zct(d_psi, ld, ng, psimuT, rk, s1, K1, s2, K2, dual, d_theta, ng, zv, st,
+                            reserve_cus(false, ng64, rk, K1 + (dual ? K2 : 0))))
+      return rc;
+    if (int rc = ev.rec(5, st)) return rc;
+    if (int rc = ev.rec(6, s2s)) return rc;
+    // Same tile as the Ng-row Z C^T, so P equals its pivot rows.
     if (int rc = launch_zct(psimuT, rk, rk, psimuT, rk, s1, K1, s2, K2, dual, P, rk, zv, s2s))
       return rc;
     if (int rc = launch_trace_eps(P, rk, eps_dev, s2s)) return rc;
@@ -530,11 +540,6 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
     if (int rc = launch_make_fg(P, rk, Dinv, Dinv128, Ft, Gt, s2s)) return rc;
     if (int rc = ev.rec(8, s2s)) return rc;
     ISDF_HIP_TRY(hipEventRecord(aux->join, s2s));
-    // Main stream: Z C^T into d_theta (getbasis_scalapack.cpp:297-369).
-    if (int rc = launch_zct(d_psi, ld, ng, psimuT, rk, s1, K1, s2, K2, dual, d_theta, ng, zv, st,
-                            reserve_cus(false, ng64, rk, K1 + (dual ? K2 : 0))))
-      return rc;
-    if (int rc = ev.rec(5, st)) return rc;
     ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->join, 0));
     if (int rc = ev.rec(9, st)) return rc;
     // Two right-side triangular solves in place (getbasis_scalapack.cpp:469-478).
