@@ -79,3 +79,20 @@ def test_device_coulomb_ragged_and_planted_error(case):
         assert c["coulomb_error"] == pytest.approx(o, rel=1e-6)
         assert c["coulomb_error"] == pytest.approx(h["coulomb_error"], rel=1e-6)
         assert c["fit_error"] == pytest.approx(h["fit_error"], rel=1e-12)
+
+
+def test_device_check_with_strided_psi(si8):
+    """ld_psi > Ng (NaN padding beyond Ng): the same numbers as the compact buffer."""
+    psi, ref = si8
+    a = ref["args"]
+    L, G = (10.216,) * 3, (16, 16, 16)
+    args = (a["nocc"], a["nstate"], a["nv1"], a["nc1"], a["nv2"], a["nc2"], a["mu_points"])
+    d_psi, d_theta, piv = solve(psi, L, G, *args, a["kmeans_max_iter"])
+    nstate, ng = psi.shape
+    wide = np.full((nstate, ng + 33), np.nan)
+    wide[:, :ng] = psi
+    dom = isdf.Domain(L, G)
+    c = isdf.check_device(d_psi, d_theta, piv, dom, *args)
+    cw = isdf.check_device(torch.from_numpy(wide).cuda(), d_theta, piv, dom, *args,
+                           ld_psi=ng + 33)
+    assert cw == c
