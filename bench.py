@@ -113,6 +113,8 @@ def main():
     ap.add_argument("--profile-solve", choices=("fp64", "fp32", "bf16x3"), default=None,
                     help="instead of the benchmark: one solve of the workload in this precision "
                          "and nothing else (a fixed program for rocprofv3 counter passes)")
+    ap.add_argument("--select", choices=("kmeans", "qrcp"), default="kmeans",
+                    help="interpolation-point selection for --profile-solve")
     a = ap.parse_args()
     if a.copy_probe > 0:
         import torch
@@ -148,10 +150,13 @@ def main():
         d_psi = torch.from_numpy(psi).cuda()
         d_theta = torch.empty((rk, w.ng), dtype=torch.float64, device="cuda")
         piv = np.zeros(w.ng, np.int32)
+        sel = isdf.ISDF_SELECT_QRCP if a.select == "qrcp" else isdf.ISDF_SELECT_KMEANS
         rep = isdf.solve(d_psi, isdf.Domain(w.length, w.ngrid), w.nocc, w.nstate, w.nv1, w.nc1,
-                         w.nv2, w.nc2, w.mu_points, d_theta, piv, w.kmeans_iters, precision=prec)
+                         w.nv2, w.nc2, w.mu_points, d_theta, piv, w.kmeans_iters, precision=prec,
+                         select=sel)
         torch.cuda.synchronize()
-        print(json.dumps({"profile_solve": a.profile_solve, "workload": w.name,
+        print(json.dumps({"profile_solve": a.profile_solve, "select": a.select,
+                          "workload": w.name,
                           "stage_ms": {k[3:]: v for k, v in rep.items() if k.startswith("ms_")}}))
         return
     w = synth.WORKLOADS[a.workload]

@@ -111,8 +111,9 @@ void ISDF(double* d_psirow, Domain domain, int nocc, int nstate, int nv1, int nc
 // include/isdf.hpp:39 of the reference: host buffers (psirow Ng x nstate,
 // thetaCol Ng x rk, piv Ng; column-major, one rank), computed on the GPU.
 // s = "scalapack" or "MPI" (the same Theta at one rank); any other s only
-// zeroes piv, as the reference does.  s1 = "Kmeans"; "QRCP" is not built and
-// throws; anything else throws "Wrong select points method".  scalblocksize
+// zeroes piv, as the reference does.  s1 = "Kmeans" or "QRCP" (pivoted QR of
+// the Gaussian-sketched pair products, getpoints_QRCP.cpp); anything else
+// throws "Wrong select points method".  scalblocksize
 // (the ScaLAPACK block size) has no role at one rank and must be positive.
 void ISDF(double* psirow, Domain domain, int nocc, int nstate, int nv1, int nc1, int nv2,
           int nc2, int mu_points, double* thetaCol, int* piv, int KmeansMaxIter_ISDF,

@@ -69,9 +69,20 @@ typedef struct isdf_problem {
   int mu_points;         /* 0 -> IRound rule */
   int kmeans_max_iter;   /* KmeansMaxIter_ISDF */
   int precision;         /* ISDF_PREC_* */
-  int reserved;
+  int select;            /* ISDF_SELECT_*: interpolation-point selection (single solve) */
   long long ld_psi;      /* 0 -> Ng */
 } isdf_problem;
+
+/* Interpolation-point selection, the reference's s1 argument
+ * (src/common/isdf.cpp:62-74, :100-113):
+ *   ISDF_SELECT_KMEANS  density-weighted K-means ("Kmeans"), the GPU ISDF's method
+ *   ISDF_SELECT_QRCP    pivoted QR of the Gaussian-sketched pair products ("QRCP",
+ *                       getpoints_QRCP.cpp); the first rk pivots, with the
+ *                       reference's srand48(41) sketch; h_piv[rk..] hold the
+ *                       remaining kept grid columns in the order the pivoting left
+ *                       them, then zeros (the reference continues the
+ *                       factorization past rk).  Single solves only. */
+enum { ISDF_SELECT_KMEANS = 0, ISDF_SELECT_QRCP = 1 };
 
 typedef struct isdf_report {
   int rk;            /* columns of theta */

@@ -16,10 +16,11 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ISDF_EHIP, ISDF_EINVAL, ISDF_ENOMEM, ISDF_ENOTSPD, ISDF_OK,
-                   ISDF_PREC_BF16X3, ISDF_PREC_FP32, ISDF_PREC_FP64, ISDF_PREC_REFINE)
+                   ISDF_PREC_BF16X3, ISDF_PREC_FP32, ISDF_PREC_FP64, ISDF_PREC_REFINE,
+                   ISDF_SELECT_KMEANS, ISDF_SELECT_QRCP)
 
 __all__ = ["Domain", "ISDF", "ISDF_host", "isdf_rank", "solve", "solve_host", "solve_batched",
-           "check", "check_device", "fit_error", "ISDF_PREC_FP64", "ISDF_PREC_FP32",
+           "check", "check_device", "fit_error", "ISDF_SELECT_KMEANS", "ISDF_SELECT_QRCP", "ISDF_PREC_FP64", "ISDF_PREC_FP32",
            "ISDF_PREC_BF16X3", "ISDF_PREC_REFINE"]
 
 
@@ -41,7 +42,7 @@ class Domain:
 
 
 def _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, max_iter,
-             precision=ISDF_PREC_FP64, ld_psi=0):
+             precision=ISDF_PREC_FP64, ld_psi=0, select=ISDF_SELECT_KMEANS):
     p = _lib.IsdfProblem()
     for d in range(3):
         p.ngrid[d] = int(domain.numGrid[d])
@@ -51,6 +52,7 @@ def _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, max_iter,
     p.mu_points = int(mu_points)
     p.kmeans_max_iter = int(max_iter)
     p.precision = int(precision)
+    p.select = int(select)
     p.ld_psi = int(ld_psi)
     return p
 
@@ -98,12 +100,15 @@ def _raise(rc):
 
 
 def solve(d_psi, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_theta, piv,
-          max_iter, stream=None, precision=ISDF_PREC_FP64, debug=None, ld_psi=0):
+          max_iter, stream=None, precision=ISDF_PREC_FP64, debug=None, ld_psi=0,
+          select=ISDF_SELECT_KMEANS):
     """Run the C-ABI solve.  piv: numpy int32 array of Ng entries (written).
     debug: optional dict with any of 'trace' (numpy int32 array), 'weights',
-    'labels', 'cct' (device tensors).  Returns the report as a dict."""
+    'labels', 'cct' (device tensors).  select: ISDF_SELECT_KMEANS (the GPU
+    ISDF's method) or ISDF_SELECT_QRCP.  Returns the report as a dict."""
     L = _lib.lib()
-    p = _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, max_iter, precision, ld_psi)
+    p = _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, max_iter, precision, ld_psi,
+                 select)
     ng = domain.NumGridTotal()
     if not (isinstance(piv, np.ndarray) and piv.dtype == np.int32 and piv.flags.c_contiguous
             and piv.size >= ng):
@@ -130,14 +135,15 @@ def solve(d_psi, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_theta, p
 
 
 def solve_host(psi, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, theta, piv, max_iter,
-               precision=ISDF_PREC_FP64, debug=None, stream=0):
+               precision=ISDF_PREC_FP64, debug=None, stream=0, select=ISDF_SELECT_KMEANS):
     """isdf_hip_solve_host: the solve of solve() on host buffers.  psi: numpy
     float64 (nstate, Ng) C-contiguous (the column-major Ng x nstate matrix);
     theta: numpy float64 (rk, Ng) C-contiguous, written; piv as in solve().
     debug: optional dict with 'trace' (numpy int32 array).  stream: a
     hipStream_t as an int, 0 for the default stream."""
     L = _lib.lib()
-    p = _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, max_iter, precision)
+    p = _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, max_iter, precision,
+                 select=select)
     ng = domain.NumGridTotal()
     rk = isdf_rank(nv1, nc1, nv2, nc2, mu_points)
     for name, a, n in (("psi", psi, ng * max(nstate, 0)), ("theta", theta, ng * max(rk, 0))):
@@ -271,8 +277,8 @@ def ISDF_host(psirow, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, theta
     written.  piv: host int32 numpy array of Ng entries, zeroed and written.
     s: "scalapack" or "MPI" (both give the same Theta at one rank; the
     reference's two basis routes differ only in their distribution).  Any other
-    s does nothing after zeroing piv, as in the reference.  s1: "Kmeans";
-    "QRCP" raises (not built); anything else raises "Wrong select points
+    s does nothing after zeroing piv, as in the reference.  s1: "Kmeans" or
+    "QRCP" (ISDF_SELECT_QRCP); anything else raises "Wrong select points
     method" (isdf.cpp:73, :112).  scalblocksize is the ScaLAPACK block size of the
     reference; it has no role in the one-rank solve and must be positive.
     """
@@ -282,14 +288,13 @@ def ISDF_host(psirow, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, theta
     piv[:ng] = 0
     if s not in ("scalapack", "MPI"):
         return {}
-    if s1 == "QRCP":
-        raise RuntimeError("QRCP point selection is not built in libisdf_hip; use s1 = \"Kmeans\"")
-    if s1 != "Kmeans":
+    if s1 not in ("Kmeans", "QRCP"):
         raise RuntimeError("Wrong select points method")
     if scalblocksize <= 0:
         raise ValueError("scalblocksize must be positive")
+    sel = ISDF_SELECT_QRCP if s1 == "QRCP" else ISDF_SELECT_KMEANS
     rep = solve_host(psirow, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, thetaCol, piv,
-                     KmeansMaxIter_ISDF)
+                     KmeansMaxIter_ISDF, select=sel)
     if check:
         rep.update(_validate(psirow, thetaCol, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2,
                              mu_points))

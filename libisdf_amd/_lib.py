@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(_HERE, "libisdf_hip.so")
 
 ISDF_OK, ISDF_EINVAL, ISDF_ENOTSPD, ISDF_EHIP, ISDF_ENOMEM = 0, 1, 2, 3, 4
 ISDF_PREC_FP64, ISDF_PREC_FP32, ISDF_PREC_BF16X3, ISDF_PREC_REFINE = 0, 1, 2, 16
+ISDF_SELECT_KMEANS, ISDF_SELECT_QRCP = 0, 1
 
 # Every symbol the public header declares; tests check the library exports them.
 EXPORTS = (
@@ -40,7 +41,7 @@ class IsdfProblem(ctypes.Structure):
         ("mu_points", ctypes.c_int),
         ("kmeans_max_iter", ctypes.c_int),
         ("precision", ctypes.c_int),
-        ("reserved", ctypes.c_int),
+        ("select", ctypes.c_int),
         ("ld_psi", ctypes.c_longlong),
     ]
 

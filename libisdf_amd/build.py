@@ -36,6 +36,7 @@ SOURCES = {
     "lowprec.hip": [],
     "fit.hip": [],
     "coulomb.hip": [],
+    "qrcp.hip": [],
     "solve.cpp": ["-x", "hip"],
     "batch.cpp": ["-x", "hip"],
     "check.cpp": [],

@@ -443,6 +443,8 @@ int solve_batched_locked(int nb, const isdf_problem* probs, const double* const*
     long long ng = 0;
     if (int rc = validate_problem(&probs[i], &ng))
       return set_error(rc, "batch instance " + std::to_string(i) + ": " + isdf_hip_last_error());
+    if (probs[i].select != ISDF_SELECT_KMEANS)
+      return set_error(ISDF_EINVAL, "batched solve: K-means point selection only");
     if (probs[i].precision != probs[0].precision)
       return set_error(ISDF_EINVAL, "batched solve: every instance needs the same precision");
     if (probs[i].nv1 != probs[i].nv2 || probs[i].nc1 != probs[i].nc2)

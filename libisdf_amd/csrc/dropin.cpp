@@ -122,20 +122,21 @@ void ISDF(double* psirow, Domain domain, int nocc, int nstate, int nv1, int nc1,
   // isdf.cpp:60 and :98: "MPI" and "scalapack" differ only in how the basis
   // is distributed over ranks; any other s computes nothing.
   if (s != "MPI" && s != "scalapack") return;
-  if (s1 == "QRCP") ErrorHandling("QRCP point selection is not built in libisdf_hip; use s1 = \"Kmeans\".");
-  if (s1 != "Kmeans") ErrorHandling("Wrong select points method");
+  if (s1 != "Kmeans" && s1 != "QRCP") ErrorHandling("Wrong select points method");
   if (scalblocksize <= 0) ErrorHandling("scalblocksize must be positive.");
+  isdf_problem q = p;
+  q.select = s1 == "QRCP" ? ISDF_SELECT_QRCP : ISDF_SELECT_KMEANS;
 
   isdf_report rep{};
   std::vector<int> trace((size_t)(KmeansMaxIter_ISDF > 0 ? KmeansMaxIter_ISDF : 1));
   isdf_debug dbg{};
   dbg.h_trace = trace.data();
-  int rc = isdf_hip_solve_host(&p, psirow, thetaCol, piv, nullptr, &rep, &dbg);
+  int rc = isdf_hip_solve_host(&q, psirow, thetaCol, piv, nullptr, &rep, &dbg);
   if (rc != ISDF_OK) ErrorHandling(isdf_hip_last_error());
   log_solve(rep, trace);
   if (check) {
     isdf_check chk{};
-    rc = isdf_hip_check(&p, psirow, thetaCol, piv, 1, &chk);
+    rc = isdf_hip_check(&q, psirow, thetaCol, piv, 1, &chk);
     if (rc != ISDF_OK) ErrorHandling(isdf_hip_last_error());
     log_check(chk);
   }

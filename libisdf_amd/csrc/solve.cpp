@@ -21,6 +21,12 @@
 
 namespace isdf_hip {
 
+// qrcp.hip: the QRCP point selection.
+int qrcp_num_pre(int rk);
+size_t qrcp_bytes(long long ng, int rk);
+int qrcp_select(const isdf_problem* p, const double* d_psi, long long ld, int ng, int rk,
+                void* scratch, int* piv_dev, hipStream_t st);
+
 namespace {
 std::mutex g_mutex;
 std::string g_last_error;
@@ -199,6 +205,8 @@ int validate(const isdf_problem* p, long long* ng_out) {
     if (mode == ISDF_PREC_FP64 && (p->precision & ISDF_PREC_REFINE))
       return set_error(ISDF_EINVAL, "ISDF_PREC_REFINE applies to the mixed forms only");
   }
+  if (p->select != ISDF_SELECT_KMEANS && p->select != ISDF_SELECT_QRCP)
+    return set_error(ISDF_EINVAL, "unknown point selection");
   int rk = irank(p);
   if (rk <= 0) return set_error(ISDF_EINVAL, "rk must be positive");
   if (rk > ng) return set_error(ISDF_EINVAL, "mu_points exceeds the number of grid points");
@@ -352,6 +360,9 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   off += align256(sizeof(double) * (size_t)rk * rk);
   const size_t off_sc = off;
   off += 256;
+  const bool qrcp = p->select == ISDF_SELECT_QRCP;
+  const size_t off_qr = off;
+  if (qrcp) off += align256(qrcp_bytes(ng, rk));
   // Mixed forms: packed psi rows, gathered Psi(mu) rows, Zt, W (FP64), Wf, and
   // the copy of C C^T + eps I the correction step multiplies by.
   size_t off_A1 = 0, off_A2 = 0, off_B1 = 0, off_B2 = 0, off_Zt = 0, off_W = 0, off_Wf = 0,
@@ -426,6 +437,21 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   // piv zeroed first (src/common/isdf.cpp:174-175).
   std::memset(h_piv, 0, sizeof(int) * (size_t)ng);
 
+  int npt = 0, iters = 0, rku = rk;
+  bool degenerate = false;
+  if (qrcp) {
+    // QRCP point selection (src/common/isdf.cpp:105-109, getpoints_QRCP.cpp).
+    if (int rc = ev.rec(1, st)) return rc;
+    if (int rc = aux_pinned(aux, (size_t)ng)) return rc;
+    if (int rc = qrcp_select(p, d_psi, ld, ng, rk, base + off_qr, piv_dev, st)) return rc;
+    if (int rc = ev.rec(2, st)) return rc;
+    ISDF_HIP_TRY(hipMemcpyAsync(aux->hpiv, piv_dev, sizeof(int) * (size_t)ng,
+                                hipMemcpyDeviceToHost, st));
+    aux->hscal[0] = 0;
+    aux->hscal[1] = rk;
+    npt = ng;
+  } else {
+
   // 1. weights (getpoints_Kmeans_MPI.cpp:33-45).
   if (int rc = launch_weights(d_psi, ld, ng, s1, s1 + K1, s2, s2 + K2, kb.w, st)) return rc;
   if (dbg && dbg->d_weights)
@@ -437,10 +463,9 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   // npt is read back (one wait: the degenerate case is handled on the host);
   // after that nothing is read until the end of the solve.  All max_iter
   // Lloyd iterations are queued; after convergence they return at once.
-  int npt = 0, iters = 0, rku = rk;
   if (int rc = aux_pinned(aux, (size_t)ng)) return rc;
   if (int rc = kmeans_prepare(&kb, p->ngrid, p->length, 1e-8, &npt, st)) return rc;
-  bool degenerate = npt < rk;
+  degenerate = npt < rk;
   if (!degenerate) {
     if (int rc = kmeans_setup_points(&kb, p->ngrid, p->length, npt, rk, st)) return rc;
     if (int rc = kmeans_iterate(&kb, rk, max_iter, 1e-5, nullptr, st)) return rc;
@@ -483,11 +508,12 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
     ISDF_HIP_TRY(hipMemcpyAsync(piv_dev, h_piv, sizeof(int) * (size_t)ng, hipMemcpyHostToDevice,
                                 st));
   }
+  }  // K-means selection
   if (int rc = ev.rec(3, st)) return rc;
-  if (dbg && dbg->d_labels && !degenerate)
+  if (dbg && dbg->d_labels && !degenerate && !qrcp)
     ISDF_HIP_TRY(hipMemcpyAsync(dbg->d_labels, kb.label, sizeof(int) * ng,
                                 hipMemcpyDeviceToDevice, st));
-  if (dbg && dbg->h_trace && max_iter > 0 && !degenerate) {
+  if (dbg && dbg->h_trace && max_iter > 0 && !degenerate && !qrcp) {
     // Debug output: the iteration count is needed now (one extra wait).
     int it = 0;
     ISDF_HIP_TRY(hipMemcpyAsync(&it, kb.scal + KS_ITERS, sizeof(int), hipMemcpyDeviceToHost, st));

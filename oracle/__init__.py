@@ -282,3 +282,66 @@ def coulomb_error(psi, length, ngrid, nocc, nv1, nc1, nv2, nc2, theta, piv, radi
     M = ((Tg.conj() * V[None, :]) @ Tg.T).real  # (:249-288)
     CMC = Cm @ M @ Cm.T  # (:291-295)
     return float(np.linalg.norm(ijVkl - CMC) / np.linalg.norm(ijVkl))
+
+
+# ---------------------------------------------------------------------------
+# QRCP point selection, src/cpu/getpoints_QRCP.cpp:11-500 at one rank.
+# ---------------------------------------------------------------------------
+
+def _iround(a):
+    # include/common/utility.hpp:18-23
+    if a > 0:
+        return int(a) if a - int(a) < 0.5 else int(a) + 1
+    return int(a) if int(a) - a < 0.5 else int(a) - 1
+
+
+def drand48_stream(seed):
+    """POSIX drand48 after srand48(seed): X <- (0x5DEECE66D X + 0xB) mod 2^48,
+    each value X / 2^48 (src/common/isdf.cpp:107; utility.hpp:455-457)."""
+    x = ((seed & 0xFFFFFFFF) << 16) | 0x330E
+    while True:
+        x = (0x5DEECE66D * x + 0xB) & ((1 << 48) - 1)
+        yield x / float(1 << 48)
+
+
+def gaussian_sketch(K1, K2, numpre, seed=41):
+    """G1 (K1 x numPre) then G2 (K2 x numPre), column-major fill, Box-Muller on
+    drand48 pairs (utility.hpp:460-471, getpoints_QRCP.cpp:182-189)."""
+    import math
+    rng = drand48_stream(seed)
+    out = []
+    for K in (K1, K2):
+        g = np.empty(K * numpre)
+        for t in range(K * numpre):
+            a = next(rng)
+            b = next(rng)
+            g[t] = math.sqrt(-2.0 * math.log(a)) * math.cos(2.0 * 3.14159265358979323846 * b)
+        out.append(g.reshape(numpre, K).T.copy())  # column-major (K x numPre)
+    return out
+
+
+def qrcp_points(psi, nocc, nv1, nc1, nv2, nc2, rk):
+    """getpoints_QRCP at one rank: sketch, Orth (the left singular vectors,
+    lapack::Orth = dgesvd), MG(i + j numPre, r) = phiG(r, i) psiG(r, j), the
+    columns above 1e-5 of the largest squared norm, and the column-pivoted QR
+    (here LAPACK dgeqp3 through scipy; the reference calls ScaLAPACK pdgeqpf).
+    Returns (piv: Ng ints, the kept columns in pivot order then zeros;
+    numPre; number of kept columns).  Small problems only (MG is formed)."""
+    import scipy.linalg
+    psi = _psi_arr(psi)
+    nstate, ng = psi.shape
+    s1, K1 = nocc - nv1, nv1 + nc1
+    s2, K2 = nocc - nv2, nv2 + nc2
+    numpre = _iround(np.sqrt(rk * 1.5))
+    G1, G2 = gaussian_sketch(K1, K2, numpre)
+    G1 = np.linalg.svd(G1, full_matrices=False)[0]
+    G2 = np.linalg.svd(G2, full_matrices=False)[0]
+    psiG = psi[s1:s1 + K1].T @ G1      # Ng x numPre  (getpoints_QRCP.cpp:200-202)
+    phiG = psi[s2:s2 + K2].T @ G2      # (:204-206)
+    MG = (phiG.T[None, :, :] * psiG.T[:, None, :]).reshape(numpre * numpre, ng)  # row i + j np
+    nrm = (MG * MG).sum(axis=0)
+    idx = np.nonzero(nrm > nrm.max() * 1e-5)[0]
+    _, _, perm = scipy.linalg.qr(MG[:, idx], pivoting=True, mode="economic")
+    piv = np.zeros(ng, np.int32)
+    piv[:len(idx)] = idx[perm]
+    return piv, numpre, len(idx)

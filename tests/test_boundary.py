@@ -70,16 +70,18 @@ def test_rank_rule_matches_oracle():
     (dict(max_iter=-1), "kmeans_max_iter"),
     (dict(precision=7), "unknown precision"),
     (dict(precision=isdf.ISDF_PREC_REFINE), "mixed forms only"),
+    (dict(select=2), "unknown point selection"),
 ])
 def test_invalid_arguments_rejected_before_device_work(kw, msg):
     dom = isdf.Domain((10.0,) * 3, (16, 16, 16))
     a = dict(nocc=8, nstate=16, nv1=8, nc1=4, nv2=8, nc2=4, mu_points=64, max_iter=5,
-             precision=isdf.ISDF_PREC_FP64)
+             precision=isdf.ISDF_PREC_FP64, select=0)
     a.update(kw)
     piv = np.zeros(4096, np.int32)
     with pytest.raises(ValueError, match=msg):
         isdf.solve(1, dom, a["nocc"], a["nstate"], a["nv1"], a["nc1"], a["nv2"], a["nc2"],
-                   a["mu_points"], 1, piv, a["max_iter"], stream=0, precision=a["precision"])
+                   a["mu_points"], 1, piv, a["max_iter"], stream=0, precision=a["precision"],
+                   select=a.get("select", 0))
 
 
 def test_check_validator_reproduces_reference_si8(si8):
@@ -159,8 +161,8 @@ def test_host_interface_dispatch_on_s_and_s1():
     assert (piv == 0).all() and (theta == 3.0).all()
     with pytest.raises(RuntimeError, match="Wrong select points method"):
         isdf.ISDF_host(psi, dom, 8, 16, 8, 4, 8, 4, 64, theta, piv, 5, s1="Random")
-    with pytest.raises(RuntimeError, match="QRCP point selection is not built"):
-        isdf.ISDF_host(psi, dom, 8, 16, 8, 4, 8, 4, 64, theta, piv, 5, s="MPI", s1="QRCP")
+    with pytest.raises(RuntimeError, match="Wrong select points method"):
+        isdf.ISDF_host(psi, dom, 8, 16, 8, 4, 8, 4, 64, theta, piv, 5, s="MPI", s1="qrcp")
 
 
 CALLER_HOST = os.path.join(ROOT, "tests", "dropin", "_build", "caller_host")
@@ -169,7 +171,7 @@ CALLER_HOST = os.path.join(ROOT, "tests", "dropin", "_build", "caller_host")
 @pytest.mark.parametrize("s,s1,status,msg", [
     ("other", "Kmeans", 0, None),
     ("scalapack", "Random", 2, "Wrong select points method"),
-    ("MPI", "QRCP", 2, "QRCP point selection is not built"),
+    ("MPI", "qrcp", 2, "Wrong select points method"),
 ])
 def test_cpp_host_interface_dispatch(tmp_path, s, s1, status, msg):
     """The C++ host-pointer ISDF (built from include/isdf.hpp without -DGPU) on

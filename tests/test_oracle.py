@@ -311,3 +311,53 @@ def test_synthetic_generator_is_orthonormal_and_seeded():
     S = psi @ psi.T * (w.L ** 3 / w.ng)
     np.testing.assert_allclose(S, np.eye(w.nstate), atol=1e-12)
     np.testing.assert_array_equal(psi, synth.wavefunctions(w))
+
+
+# --- QRCP point selection (getpoints_QRCP.cpp) -------------------------------
+# No golden vectors exist for this path in the reference tree.  Its oracle is
+# pinned by: the random stream against glibc's own drand48 after srand48(41);
+# the pivots against a brute-force greedy maximum-residual selection (the
+# definition of column-pivoted QR) on small cases; and the invariance the
+# device path relies on (any orthonormal basis of the sketch spans gives the
+# same pivots).
+
+def _qrcp_case(seed, grid=(10, 9, 8), nstate=20, nocc=10, nv1=8, nc1=6, nv2=7, nc2=5, rk=60):
+    rng = np.random.default_rng(seed)
+    ng = grid[0] * grid[1] * grid[2]
+    psi = rng.standard_normal((nstate, ng))
+    return psi, (nocc, nv1, nc1, nv2, nc2, rk)
+
+
+def test_qrcp_random_stream_is_glibc_drand48():
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    libc.drand48.restype = ctypes.c_double
+    libc.srand48(41)
+    g = oracle.drand48_stream(41)
+    assert all(libc.drand48() == next(g) for _ in range(2000))
+
+
+def test_qrcp_oracle_is_greedy_max_residual_and_basis_invariant():
+    psi, (nocc, nv1, nc1, nv2, nc2, rk) = _qrcp_case(5)
+    piv, numpre, nkept = oracle.qrcp_points(psi, nocc, nv1, nc1, nv2, nc2, rk)
+    assert nkept >= rk and numpre == oracle._iround(np.sqrt(rk * 1.5))
+    # brute force: rebuild MG, pick the largest residual column step by step
+    K1, K2 = nv1 + nc1, nv2 + nc2
+    G1, G2 = oracle.gaussian_sketch(K1, K2, numpre)
+    Q1, _ = np.linalg.qr(G1)      # a different orthonormal basis than the SVD's
+    Q2, _ = np.linalg.qr(G2)
+    psiG = psi[nocc - nv1:nocc + nc1].T @ Q1
+    phiG = psi[nocc - nv2:nocc + nc2].T @ Q2
+    MG = (phiG.T[None, :, :] * psiG.T[:, None, :]).reshape(numpre * numpre, -1)
+    R = MG.copy()
+    chosen = []
+    for k in range(rk):
+        nrm = np.linalg.norm(R, axis=0)
+        nrm[chosen] = -1.0
+        j = int(np.argmax(nrm))
+        top = np.sort(nrm)[-2:]
+        assert top[1] - top[0] > 1e-9 * top[1]  # no near-tie in this case
+        chosen.append(j)
+        q = R[:, j] / np.linalg.norm(R[:, j])
+        R -= np.outer(q, q @ R)
+    assert list(piv[:rk]) == chosen
