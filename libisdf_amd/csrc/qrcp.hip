@@ -37,11 +37,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
 #include "../../include/isdf_hip.h"
 #include "isdf_internal.h"
+#include "mfma_gemm.h"
 
 namespace isdf_hip {
 
@@ -261,15 +263,27 @@ __global__ __launch_bounds__(256) void k_qr_pivot(int k, int nparts, const doubl
                                                   double* __restrict__ a0n2,
                                                   double* __restrict__ s2,
                                                   double* __restrict__ a) {
-  __shared__ int spv;
-  if (threadIdx.x == 0) {
+  __shared__ double sv[256];
+  __shared__ int sp[256];
+  {
     double v = -1.0;
-    int p = 0x7fffffff;
-    for (int b = 0; b < nparts; b++) argmax_pair(v, p, pv[b], pp[b]);
-    spv = p;
+    int q = 0x7fffffff;
+    for (int b = threadIdx.x; b < nparts; b += 256) argmax_pair(v, q, pv[b], pp[b]);
+    sv[threadIdx.x] = v;
+    sp[threadIdx.x] = q;
   }
   __syncthreads();
-  const int p = spv;
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      double v = sv[threadIdx.x];
+      int q = sp[threadIdx.x];
+      argmax_pair(v, q, sv[threadIdx.x + h], sp[threadIdx.x + h]);
+      sv[threadIdx.x] = v;
+      sp[threadIdx.x] = q;
+    }
+    __syncthreads();
+  }
+  const int p = sp[0];
   if (p != k) {
     if (threadIdx.x == 0) {
       int t = perm[p];
@@ -320,15 +334,18 @@ __global__ __launch_bounds__(256) void k_qr_dots(const double* __restrict__ Q, i
   if (lane == 0) c[t] = s;
 }
 
-// a[i] -= sum_t Q[i, t] c[t], t < k.
-__global__ __launch_bounds__(256) void k_qr_axpy(const double* __restrict__ Q, int m, int k,
-                                                 const double* __restrict__ c,
+// a[i] -= sum_t Q[i, t] c[t], t < k: one wave per row i of the row-major copy
+// Qr[i * ldr + t] (lanes over t, then a fixed butterfly).
+__global__ __launch_bounds__(256) void k_qr_axpy(const double* __restrict__ Qr, int ldr, int m,
+                                                 int k, const double* __restrict__ c,
                                                  double* __restrict__ a) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (i >= m) return;
+  const double* q = Qr + (long long)i * ldr;
   double s = 0.0;
-  for (int t = 0; t < k; t++) s += Q[(long long)t * m + i] * c[t];
-  a[i] -= s;
+  for (int t = lane; t < k; t += 64) s += q[t] * c[t];
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) a[i] -= s;
 }
 
 // Step k, part 3: q_k = a / |a| (every workgroup forms it in LDS; workgroup 0
@@ -336,7 +353,8 @@ __global__ __launch_bounds__(256) void k_qr_axpy(const double* __restrict__ Q, i
 // workgroup's tile of T: R(k, j) = sum_jj psiG(j, jj) sum_i q(i + jj np) phiG(j, i),
 // pdgeqpf's norm downdate, and the tile's partial argmax for step k + 1.
 __global__ void k_qr_update(int k, int n, int np, int ldp, const double* __restrict__ a,
-                            double* __restrict__ Q, const double* __restrict__ PHc,
+                            double* __restrict__ Q, double* __restrict__ Qr, int ldr,
+                            const double* __restrict__ PHc,
                             const double* __restrict__ PSc, double* __restrict__ vn1,
                             double* __restrict__ vn2, const double* __restrict__ a0n2,
                             double* __restrict__ s2, double* __restrict__ pv,
@@ -362,7 +380,10 @@ __global__ void k_qr_update(int k, int n, int np, int ldp, const double* __restr
   for (int e = threadIdx.x; e < m; e += T) {
     const double v = a[e] * inv;
     q[e] = v;
-    if (blockIdx.x == 0) Q[(long long)k * m + e] = v;
+    if (blockIdx.x == 0) {
+      Q[(long long)k * m + e] = v;
+      Qr[(long long)e * ldr + k] = v;
+    }
   }
   const int j0 = k + 1 + blockIdx.x * T;
   const int j = j0 + threadIdx.x;
@@ -419,6 +440,120 @@ __global__ void k_qr_update(int k, int n, int np, int ldp, const double* __restr
   }
 }
 
+// The same step on FP64 MFMA: for blocks of 64 positions (16 per wave; each
+// workgroup walks kQrBlocks blocks, so q is set up once per workgroup),
+// T(j, i) = sum_jj psiG(j, jj) q(i + jj np) as 16 x 16 x 4 products (A = the
+// psiG rows, B = q reshaped), then R(k, j) = sum_i phiG(j, i) T(j, i) with the
+// phiG tile staged in LDS and the 16 columns of a tile summed across their
+// lanes.  Used when q and the phiG tile fit 60 KB of LDS.
+constexpr int kQrBlocks = 4;
+__global__ __launch_bounds__(256) void k_qr_update_mfma(
+    int k, int n, int np, int ldp, const double* __restrict__ a, double* __restrict__ Q,
+    double* __restrict__ Qr, int ldr, const double* __restrict__ PHc,
+    const double* __restrict__ PSc, double* __restrict__ vn1, double* __restrict__ vn2,
+    const double* __restrict__ a0n2, double* __restrict__ s2, double* __restrict__ pv,
+    int* __restrict__ pp) {
+  extern __shared__ double lds[];
+  const int m = np * np;
+  double* q = lds;       // m
+  double* ph = lds + m;  // np x 64: ph[i * 64 + t]
+  __shared__ double red[256];
+  __shared__ int redp[256];
+  double s = 0.0;
+  for (int e = threadIdx.x; e < m; e += 256) s += a[e] * a[e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double nrm = sqrt(red[0]);
+  const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < m; e += 256) {
+    const double v = a[e] * inv;
+    q[e] = v;
+    if (blockIdx.x == 0) {
+      Q[(long long)k * m + e] = v;
+      Qr[(long long)e * ldr + k] = v;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  double best = -1.0;
+  int bestp = 0x7fffffff;
+  for (int blk = 0; blk < kQrBlocks; blk++) {
+    const int j0 = k + 1 + (blockIdx.x * kQrBlocks + blk) * 64;
+    if (j0 >= n) break;  // uniform across the workgroup
+    __syncthreads();     // q written (first block) / previous tile consumed
+    for (int e = threadIdx.x; e < np * 64; e += 256) {
+      const int i = e / 64, j = j0 + e % 64;
+      ph[e] = (j < n) ? PHc[(long long)i * ldp + j] : 0.0;
+    }
+    __syncthreads();
+    const int p0 = j0 + 16 * wave;
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int n0 = 0; n0 < np; n0 += 16) {
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+      const int ib = n0 + lr;
+      for (int k0 = 0; k0 < np; k0 += 4) {
+        const int jj = k0 + lg;
+        const int pa = p0 + lr;
+        const double av = (jj < np && pa < n) ? PSc[(long long)jj * ldp + pa] : 0.0;
+        const double bv = (jj < np && ib < np) ? q[ib + jj * np] : 0.0;
+        acc = mfma_f64(av, bv, acc);
+      }
+      if (ib < np) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) part[r] += acc[r] * ph[ib * 64 + 16 * wave + lg + 4 * r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      for (int off = 8; off >= 1; off >>= 1) part[r] += __shfl_xor(part[r], off, 64);
+    const int j = p0 + lg + 4 * lr;
+    if (lr < 4 && j < n) {
+      const double y = lr == 0 ? part[0] : lr == 1 ? part[1] : lr == 2 ? part[2] : part[3];
+      const double s2n = s2[j] + y * y;
+      s2[j] = s2n;
+      double w1 = vn1[j];
+      if (w1 != 0.0) {
+        double temp = fabs(y) / w1;
+        temp = 1.0 - temp * temp;
+        temp = temp > 0.0 ? temp : 0.0;
+        const double ratio = w1 / vn2[j];
+        const double temp2 = 1.0 + 0.05 * temp * ratio * ratio;
+        if (temp2 == 1.0) {
+          const double rest = a0n2[j] - s2n;
+          w1 = rest > 0.0 ? sqrt(rest) : 0.0;
+          vn2[j] = w1;
+        } else {
+          w1 = w1 * sqrt(temp);
+        }
+        vn1[j] = w1;
+      }
+      argmax_pair(best, bestp, w1, j);
+    }
+  }
+  red[threadIdx.x] = best;
+  redp[threadIdx.x] = bestp;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      double a1 = red[threadIdx.x];
+      int b1 = redp[threadIdx.x];
+      argmax_pair(a1, b1, red[threadIdx.x + h], redp[threadIdx.x + h]);
+      red[threadIdx.x] = a1;
+      redp[threadIdx.x] = b1;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pv[blockIdx.x] = red[0];
+    pp[blockIdx.x] = redp[0];
+  }
+}
+
 // piv[j] = perm[j] for j < n, 0 beyond (the reference zeroes pivQR_ first).
 __global__ __launch_bounds__(256) void k_qr_piv(const int* __restrict__ perm,
                                                 const int* __restrict__ scal, int ng,
@@ -442,7 +577,7 @@ size_t qrcp_bytes(long long ng, int rk) {
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const long long nparts = cdivq(ng, 32);  // partial maxima: tiles of at least 32 positions
   return al(8 * (size_t)(np * ng)) * 4 + al(8 * (size_t)ng) * 5 + al(4 * (size_t)ng) +
-         al(8 * (size_t)(m * rk)) + al(8 * (size_t)m) + al(8 * (size_t)rk) +
+         al(8 * (size_t)(m * rk)) * 2 + al(8 * (size_t)m) + al(8 * (size_t)rk) +
          al(8 * (size_t)nblk) + al(8 * (size_t)nparts) + al(4 * (size_t)nparts) + 256;
 }
 
@@ -495,6 +630,7 @@ int qrcp_select(const isdf_problem* p, const double* d_psi, long long ld, int ng
   double* s2v = (double*)take(8 * (size_t)ng);
   int* perm = (int*)take(4 * (size_t)ng);
   double* Q = (double*)take(8 * (size_t)m * rk);
+  double* Qr = (double*)take(8 * (size_t)m * rk);  // the same columns, row-major (ld rk)
   double* av = (double*)take(8 * (size_t)m);
   double* cv = (double*)take(8 * (size_t)rk);
   // Partial maxima: one per block of the argmax (256 positions) or of the
@@ -541,6 +677,8 @@ int qrcp_select(const isdf_problem* p, const double* d_psi, long long ld, int ng
   if (lds > 60 * 1024)
     return set_error(ISDF_EINVAL, "QRCP: numPre = " + std::to_string(np) +
                                       " is above the supported 80 (rk above ~4300)");
+  const size_t lds_mfma = 8 * ((size_t)m + (size_t)np * 64);
+  const bool use_mfma = lds_mfma <= 60 * 1024 && std::getenv("ISDF_QRCP_VALU") == nullptr;
   int nparts = (int)cdivq(nmg, 256);
   hipLaunchKernelGGL(k_qr_argmax, dim3((unsigned)nparts), dim3(256), 0, st, (const double*)vn1,
                      0, nmg, pv, pp);
@@ -551,16 +689,24 @@ int qrcp_select(const isdf_problem* p, const double* d_psi, long long ld, int ng
       for (int pass = 0; pass < 2; pass++) {
         hipLaunchKernelGGL(k_qr_dots, dim3((unsigned)cdivq(k, 4)), dim3(256), 0, st,
                            (const double*)Q, m, k, (const double*)av, cv);
-        hipLaunchKernelGGL(k_qr_axpy, dim3((unsigned)cdivq(m, 256)), dim3(256), 0, st,
-                           (const double*)Q, m, k, (const double*)cv, av);
+        hipLaunchKernelGGL(k_qr_axpy, dim3((unsigned)cdivq(m, 4)), dim3(256), 0, st,
+                           (const double*)Qr, rk, m, k, (const double*)cv, av);
       }
     }
     const int rest = nmg - (k + 1);
-    const int nb = std::max(1, (int)cdivq(rest, T));
-    hipLaunchKernelGGL(k_qr_update, dim3((unsigned)nb), dim3(T), lds, st, k, nmg, np, ng,
-                       (const double*)av, Q, (const double*)PHc, (const double*)PSc, vn1, vn2,
-                       (const double*)a0n2, s2v, pv, pp);
-    nparts = nb;
+    if (use_mfma) {
+      const int nb = std::max(1, (int)cdivq(rest, 64 * kQrBlocks));
+      hipLaunchKernelGGL(k_qr_update_mfma, dim3((unsigned)nb), dim3(256), lds_mfma, st, k, nmg,
+                         np, ng, (const double*)av, Q, Qr, rk, (const double*)PHc,
+                         (const double*)PSc, vn1, vn2, (const double*)a0n2, s2v, pv, pp);
+      nparts = nb;
+    } else {
+      const int nb = std::max(1, (int)cdivq(rest, T));
+      hipLaunchKernelGGL(k_qr_update, dim3((unsigned)nb), dim3(T), lds, st, k, nmg, np, ng,
+                         (const double*)av, Q, Qr, rk, (const double*)PHc, (const double*)PSc,
+                         vn1, vn2, (const double*)a0n2, s2v, pv, pp);
+      nparts = nb;
+    }
     ISDF_HIP_LAUNCHED("QRCP step");
   }
   hipLaunchKernelGGL(k_qr_piv, dim3((unsigned)nblk), dim3(256), 0, st, (const int*)perm,

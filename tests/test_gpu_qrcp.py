@@ -80,3 +80,15 @@ def test_qrcp_host_interface_and_limits():
     # numPre = IRound(sqrt(1.5 rk)) larger than a pair set: rejected
     with pytest.raises(ValueError, match="exceeds a pair set"):
         gpu_qrcp(psi, grid, L, nocc, nstate, 3, 2, nv2, nc2, 100)
+
+
+def test_qrcp_vector_and_mfma_updates_agree(monkeypatch):
+    """The R(k, .) update runs on FP64 MFMA when q and the sketch tile fit its
+    LDS budget, else on the vector FMA kernel (ISDF_QRCP_VALU forces it): the
+    same pivots either way on a non-degenerate case."""
+    grid, L, nstate, nocc, nv1, nc1, nv2, nc2, rk = CASES[1]
+    psi = make_psi(grid, L, nstate, 93)
+    _, piv_m, _ = gpu_qrcp(psi, grid, L, nocc, nstate, nv1, nc1, nv2, nc2, rk)
+    monkeypatch.setenv("ISDF_QRCP_VALU", "1")
+    _, piv_v, _ = gpu_qrcp(psi, grid, L, nocc, nstate, nv1, nc1, nv2, nc2, rk)
+    np.testing.assert_array_equal(piv_m, piv_v)
