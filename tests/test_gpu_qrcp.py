@@ -92,3 +92,26 @@ def test_qrcp_vector_and_mfma_updates_agree(monkeypatch):
     monkeypatch.setenv("ISDF_QRCP_VALU", "1")
     _, piv_v, _ = gpu_qrcp(psi, grid, L, nocc, nstate, nv1, nc1, nv2, nc2, rk)
     np.testing.assert_array_equal(piv_m, piv_v)
+
+
+def test_cpp_host_dropin_with_qrcp(tmp_path):
+    """The C++ host-pointer ISDF (include/isdf.hpp:39) with s1 = "QRCP", as the
+    reference's example calls it, gives the C-ABI QRCP solve's piv and Theta."""
+    import os
+    import subprocess
+    grid, L, nstate, nocc, nv1, nc1, nv2, nc2, rk = CASES[0]
+    psi = make_psi(grid, L, nstate, 94)
+    ng = psi.shape[1]
+    theta, piv, _ = gpu_qrcp(psi, grid, L, nocc, nstate, nv1, nc1, nv2, nc2, rk)
+    np.ascontiguousarray(psi).tofile(tmp_path / "psi.bin")
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dropin", "_build",
+                       "caller_host")
+    args = [exe, str(tmp_path / "psi.bin"), *map(str, grid), *map(str, L), str(nocc), str(nstate),
+            str(nv1), str(nc1), str(nv2), str(nc2), str(rk), "10", "0",
+            str(tmp_path / "theta.bin"), str(tmp_path / "piv.bin"), str(tmp_path / "isdf.0"),
+            "scalapack", "QRCP", "32"]
+    print("drop-in caller qrcp", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "piv.bin", np.int32), piv)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "theta.bin").reshape(rk, ng), theta)
