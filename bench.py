@@ -129,7 +129,7 @@ def main():
         return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs the ISDF path as a single rank with no "
-              "collectives (every GPU box here has one GPU); --gpus %d is not supported." % a.gpus,
+              "collectives; --gpus %d is not supported." % a.gpus,
               file=sys.stderr)
         sys.exit(2)
 
