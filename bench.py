@@ -128,7 +128,7 @@ def main():
                           "launches": 5}))
         return
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build runs the ISDF path as a single rank with no "
+        print("bench.py measures one GPU: the ISDF path runs as a single rank with no "
               "collectives; --gpus %d is not supported." % a.gpus,
               file=sys.stderr)
         sys.exit(2)
