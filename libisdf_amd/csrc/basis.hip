@@ -637,7 +637,9 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_potrs(double* __restrict__ T, l
   }
 }
 
-typedef MfmaTile<128, 128, 16, 2, 2> XxtTile;
+// 64 x 64 tiles: at rk 1024 the lower triangle is 136 tiles instead of 36 of
+// 128 x 128, and the product sits on the mixed forms' critical path.
+typedef MfmaTile<64, 64, 16, 2, 2> XxtTile;
 
 // W = X X^T for the upper-triangular X = L^{-T} (n x n, column-major), so
 // W = L^{-T} L^{-1} = (L L^T)^{-1}.  One workgroup per lower tile (ti >= tj);
