@@ -808,6 +808,97 @@ int launch_make_fg(const double* L, int rk, const double* Dinv, double* Dinv128,
   return 0;
 }
 
+int launch_dinv128(const double* L, int rk, const double* Dinv, double* Dinv128, hipStream_t st) {
+  hipLaunchKernelGGL(k_dinv128<false>, dim3(cdiv(rk, kSB)), dim3(256), 0, st, L, rk, Dinv,
+                     Dinv128, (const CholJob*)nullptr);
+  ISDF_HIP_LAUNCHED("k_dinv128");
+  return 0;
+}
+
+// Y = L^{-1} by recursive doubling, then its transpose.  Y starts as the
+// block diagonal of the 128-block inverses; at half-size s, each pair of
+// s-blocks (Y11 at (t, t), Y22 at (t+s, t+s), L21 the factor's block between)
+// gets its lower-left block Y21 = -Y22 (L21 Y11), so every level is two
+// batched GEMMs and rk = 1024 takes three levels.
+__global__ __launch_bounds__(256) void k_linv_init(const double* __restrict__ Dinv128, int n,
+                                                   double* __restrict__ Y) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e % n), c = (int)(e / n);
+  const int J = r / kSB;
+  Y[e] = (c / kSB == J) ? Dinv128[(long long)J * kSB * kSB + (c % kSB) * kSB + r % kSB] : 0.0;
+}
+
+typedef MfmaTile<64, 64, 16, 2, 2, false> LinvTile;  // B read k-contiguous
+
+// STAGE 0: T_p = L21 Y11 (s1 x s, ld s); STAGE 1: Y21 = -Y22 T_p.  Pair p =
+// blockIdx.z starts at t = 2 s p; its lower block has s1 = min(s, n - t - s) rows.
+template <int STAGE>
+__global__ __launch_bounds__(LinvTile::NT) void k_linv_gemm(const double* __restrict__ L,
+                                                            double* __restrict__ Y,
+                                                            double* __restrict__ Tb, int n,
+                                                            int s) {
+  __shared__ double lds[LinvTile::LDS_DOUBLES];
+  const long long t = 2LL * s * blockIdx.z;
+  if (t + s >= n) return;
+  const int s1 = (int)std::min<long long>(s, n - t - s);
+  const int m0 = blockIdx.x * LinvTile::BM, n0 = blockIdx.y * LinvTile::BN;
+  if (m0 >= s1) return;
+  const int mlim = std::min(LinvTile::BM, s1 - m0), nlim = std::min(LinvTile::BN, s - n0);
+  double* Tp = Tb + (long long)blockIdx.z * s * s;
+  typename LinvTile::Acc acc;
+  LinvTile::zero(acc);
+  if constexpr (STAGE == 0) {
+    // A(m, k) = L(t+s+m0+m, t+k); B(k, c) = Y(t+k, t+n0+c).
+    LinvTile::run(L + t * n + t + s + m0, n, Y + (t + n0) * n + t, n, s, mlim, nlim, lds, acc);
+    LinvTile::epilogue(acc, [&](int m, int c, double v) {
+      if (m < mlim && c < nlim) Tp[(long long)(n0 + c) * s + m0 + m] = v;
+    });
+  } else {
+    // A(m, k) = Y(t+s+m0+m, t+s+k); B(k, c) = T_p(k, n0+c).
+    LinvTile::run(Y + (t + s) * n + t + s + m0, n, Tp + (long long)n0 * s, s, s1, mlim, nlim, lds,
+                  acc);
+    LinvTile::epilogue(acc, [&](int m, int c, double v) {
+      if (m < mlim && c < nlim) Y[(t + n0 + c) * n + t + s + m0 + m] = -v;
+    });
+  }
+}
+
+// X = Y^T (n x n, column-major), through 32 x 32 LDS tiles.
+__global__ __launch_bounds__(256) void k_transpose_sq(const double* __restrict__ Y, int n,
+                                                      double* __restrict__ X) {
+  __shared__ double tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int j = ty; j < 32; j += 8) {
+    const int r = bx + tx, c = by + j;
+    tile[j][tx] = (r < n && c < n) ? Y[(long long)c * n + r] : 0.0;
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int r = by + tx, c = bx + j;
+    if (r < n && c < n) X[(long long)c * n + r] = tile[tx][j];
+  }
+}
+
+int launch_linv_transposed(const double* L, int n, const double* Dinv128, double* Y, double* Tb,
+                           double* X, hipStream_t st) {
+  hipLaunchKernelGGL(k_linv_init, dim3(cdiv((long long)n * n, 256)), dim3(256), 0, st, Dinv128, n,
+                     Y);
+  ISDF_HIP_LAUNCHED("k_linv_init");
+  for (int s = kSB; s < n; s *= 2) {
+    const unsigned np = cdiv(n, 2 * s);
+    const dim3 grid(cdiv(s, LinvTile::BM), cdiv(s, LinvTile::BN), np);
+    hipLaunchKernelGGL(k_linv_gemm<0>, grid, dim3(LinvTile::NT), 0, st, L, Y, Tb, n, s);
+    hipLaunchKernelGGL(k_linv_gemm<1>, grid, dim3(LinvTile::NT), 0, st, L, Y, Tb, n, s);
+    ISDF_HIP_LAUNCHED("k_linv_gemm");
+  }
+  hipLaunchKernelGGL(k_transpose_sq, dim3(cdiv(n, 32), cdiv(n, 32)), dim3(256), 0, st,
+                     (const double*)Y, n, X);
+  ISDF_HIP_LAUNCHED("k_transpose_sq");
+  return 0;
+}
+
 int potrs_variants() { return 7; }
 
 int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* Ft, const double* Gt,

@@ -216,6 +216,12 @@ int launch_make_fg_batched(const CholJob* jobs, int nb, int max_n, hipStream_t s
 int launch_potrs_identity_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st);
 // W = X X^T for upper-triangular X = L^{-T}: W = (L L^T)^{-1}.
 int launch_xxt(const double* X, int n, double* W, hipStream_t st);
+// The 128-block inverses of the factor alone (the first half of launch_make_fg).
+int launch_dinv128(const double* L, int rk, const double* Dinv, double* Dinv128, hipStream_t st);
+// X = L^{-T} (n x n, column-major) by recursive doubling on L^{-1} from the
+// 128-block inverses; Y and Tb are n x n scratch.
+int launch_linv_transposed(const double* L, int n, const double* Dinv128, double* Y, double* Tb,
+                           double* X, hipStream_t st);
 int launch_xxt_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st);
 
 // ---------------------------------------------------------------------------
@@ -242,12 +248,12 @@ int launch_zct_lp(int mode, int ng, int rk, int ldz, const void* A1, const void*
 // P(mu, nu) = Zt[piv[mu]][nu] (column-major FP64; piv NULL: row mu) and the eps shift.
 int launch_cct_lp(const float* Zt, int ldz, const int* piv, int rk, double* P, double* eps_dev,
                   hipStream_t st);
-// W = (L L^T)^{-1} (FP64, column-major rk x rk) from the forward-solve operand
-// (Gt is overwritten: it holds L^{-T} on the way), and operand rows nu of
-// W(k, nu), k < ldw (zero for k >= rk), in operand format wmode (1 FP32, 2
-// pre-split bf16 planes).
-int launch_inverse(int wmode, int rk, const double* Ft, double* Gt, double* W, int ldw,
-                   void* Wf, hipStream_t st);
+// W = (L L^T)^{-1} (FP64, column-major rk x rk) from the factor L and its
+// 128-block inverses (Y and X are rk x rk scratch: L^{-1} and L^{-T} on the
+// way), and operand rows nu of W(k, nu), k < ldw (zero for k >= rk), in
+// operand format wmode (1 FP32, 2 pre-split bf16 planes).
+int launch_inverse(int wmode, int rk, const double* L, const double* Dinv128, double* Y,
+                   double* X, double* W, int ldw, void* Wf, hipStream_t st);
 // Z C^T tile variant after defaults and the dual-product overrides, and the
 // format of the operand rows it reads (1 FP32, 2 pre-split bf16 planes; the
 // mode argument of launch_pack_psi / launch_gather_rows): BF16X3 variants 3

@@ -692,13 +692,6 @@ __global__ __launch_bounds__(256) void k_cct_lp(const float* __restrict__ Zt, in
   P[t] = (double)Zt[row * ldz + nu];
 }
 
-// T = I (n x n, column-major FP64).
-__global__ __launch_bounds__(256) void k_identity(double* __restrict__ T, int n) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)n * n) return;
-  T[t] = (t % n == t / n) ? 1.0 : 0.0;
-}
-
 // Operand row nu of W: W(k, nu) (FP32 or pre-split, ldw columns), zero for
 // k >= rk (W column-major FP64).
 template <bool SPLIT>
@@ -1011,15 +1004,15 @@ int launch_cct_lp(const float* Zt, int ldz, const int* piv, int rk, double* P, d
   return launch_trace_eps(P, rk, eps_dev, st);
 }
 
-int launch_inverse(int wmode, int rk, const double* Ft, double* Gt, double* W, int ldw,
-                   void* Wf, hipStream_t st) {
-  // X = I L^{-T} in Gt (the backward-solve operand is not needed by the mixed
-  // forms): the forward pass of the blocked solve with its 16-row tile (rk/16
-  // workgroups), skipping the blocks the identity leaves zero; then W = X X^T.
-  hipLaunchKernelGGL(k_identity, dim3(cdiv((long long)rk * rk, 256)), dim3(256), 0, st, Gt, rk);
-  ISDF_HIP_LAUNCHED("k_identity");
-  if (int rc = launch_potrs(Gt, rk, rk, rk, Ft, Gt, 5, st, true)) return rc;
-  if (int rc = launch_xxt(Gt, rk, W, st)) return rc;
+int launch_inverse(int wmode, int rk, const double* L, const double* Dinv128, double* Y,
+                   double* X, double* W, int ldw, void* Wf, hipStream_t st) {
+  // X = L^{-T} by recursive doubling on L^{-1} (W's buffer holds the GEMM
+  // temporaries until W is written), then W = X X^T.  Before this, a
+  // triangular solve of the identity (rk/16 workgroups walking the block
+  // columns in sequence) took 400 us at rk 1024 on the mixed forms' critical
+  // path; the doubling is log2(rk/128) levels of two batched GEMMs.
+  if (int rc = launch_linv_transposed(L, rk, Dinv128, Y, W, X, st)) return rc;
+  if (int rc = launch_xxt(X, rk, W, st)) return rc;
   const dim3 grid(cdiv((long long)rk * ldw, 256));
   if (wmode == 2)
     hipLaunchKernelGGL(k_pack_w<true>, grid, dim3(256), 0, st, W, rk, ldw, Wf);

@@ -299,8 +299,11 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
   if (int rc = ev.rec(7, s2)) return rc;
   ISDF_HIP_TRY(hipMemsetAsync(info_dev, 0, sizeof(int), s2));
   if (int rc = launch_potrf(P, rk, Dinv, info_dev, s2)) return rc;
-  if (int rc = launch_make_fg(P, rk, Dinv, Dinv128, Ft, Gt, s2)) return rc;
-  if (int rc = launch_inverse(lp_theta_w_mode(mode, tv), rk, Ft, Gt, m.W, ldz, m.Wf, s2))
+  // Only the 128-block inverses: the mixed forms need W, not the blocked
+  // solve's F / G operands (Ft and Gt serve as scratch below).
+  if (int rc = launch_dinv128(P, rk, Dinv, Dinv128, s2)) return rc;
+  if (int rc = launch_inverse(lp_theta_w_mode(mode, tv), rk, P, Dinv128, Ft, Gt, m.W, ldz, m.Wf,
+                              s2))
     return rc;
   if (int rc = ev.rec(8, s2)) return rc;
   ISDF_HIP_TRY(hipEventRecord(aux->join, s2));
