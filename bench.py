@@ -28,6 +28,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ISDF solves/s and ZC^T TFLOP/s, mixed vs FP64 plain at fixed fit error, 1 MI355X"
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3     # dense FP32 MFMA
+BF16_PEAK_TFLOPS = 2500.0    # dense bf16 MFMA
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per k_potrs launch from two rocprofv3 counter passes (FETCH_SIZE x 2.000,
@@ -280,6 +282,12 @@ def main():
                 "piv_equal_fp64": bool(np.array_equal(piv_m, piv)),
                 "zct_tflops": fl["zct"] / (avgm["ms_zct"] * 1e-3) / 1e12,
                 "theta_tflops": fl["potrs"] / (avgm["ms_potrs"] * 1e-3) / 1e12,
+                # Fraction of the MFMA peak the Theta product runs on: FP32 counts its
+                # nominal flops; BF16X3 runs six bf16 products per nominal multiply-add.
+                "theta_frac_mfma_peak": (
+                    fl["potrs"] / (avgm["ms_potrs"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS
+                    if name == "fp32" else
+                    6.0 * fl["potrs"] / (avgm["ms_potrs"] * 1e-3) / 1e12 / BF16_PEAK_TFLOPS),
                 "stage_ms": {k[3:]: round(v, 4) for k, v in avgm.items()},
             })
         del runs
