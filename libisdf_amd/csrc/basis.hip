@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
 
 #include "isdf_internal.h"
 #include "mfma_gemm.h"
@@ -259,17 +260,105 @@ __device__ __forceinline__ double wave_bcast(double v, int src) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Factor the jb x jb diagonal block at J0 with one wave, write L_JJ back and
-// its inverse into Dinv[J] (64 x 64, column-major, zero padded).  Lane r keeps
-// row r of the block in registers (fully unrolled, static indices); column j
-// of each step reaches the other lanes by readlane broadcasts.  The inverse
-// is formed right-looking (column `lane` of inv(L_JJ)): the subtractions of one
-// step are independent, and each entry still accumulates its terms in
-// ascending t before the division by L_ii.
+constexpr int kDiagCW = kNB / 4;  // columns (and inverse rows) per wave of the diagonal factor
+
+// Step J of the diagonal-block factorization.  Wave w keeps columns
+// [16w, 16w+16) of every row (lane r = row r) in v.  The owner of column J
+// forms it (pivot, sqrt, correctly rounded divisions) and publishes L(:,J) in
+// LDS (two buffers, by the parity of J); after the barrier every wave takes
+// the rank-1 update of all its columns with no mask: L(g,J) = 0 for g <= J
+// (those lanes publish 0), so the columns left of J and column J itself are
+// unchanged.  A non-positive pivot is recorded in *fail_s (first failing
+// step + 1) and replaced by 1 so that the remaining steps stay finite
+// (nothing is written back then).  Rows and columns past jb hold the identity.
+template <int J>
+__device__ __forceinline__ void potrf_diag_step(double (&v)[kDiagCW], int lane, int w,
+                                                int* fail_s, double* col, double* ldiag) {
+  constexpr int o = J / kDiagCW, jj = J % kDiagCW;
+  double* c = col + (J & 1) * kNB;
+  if (w == o) {
+    double piv = wave_bcast(v[jj], J);
+    const bool bad = !(piv > 0.0);
+    if (bad && *fail_s == 0) *fail_s = J + 1;
+    piv = bad ? 1.0 : piv;
+    const double d = sqrt(piv);
+    const double q = v[jj] / d;
+    const double lij = lane > J ? q : 0.0;
+    v[jj] = lane == J ? d : lij;
+    ldiag[J] = d;
+    c[lane] = lij;
+  }
+  __syncthreads();
+  const double a = c[lane];
+  const double* cw = c + w * kDiagCW;
+#pragma unroll
+  for (int k = 0; k < kDiagCW; k += 2) {
+    const double2 l = *reinterpret_cast<const double2*>(cw + k);
+    v[k] -= a * l.x;
+    v[k + 1] -= a * l.y;
+  }
+}
+
+// Step T of the inverse.  Wave w keeps rows [16w, 16w+16) of column `lane` of
+// inv(L_JJ) in x; the owner of row T divides it by L_TT and publishes it, and
+// every row takes x_i -= L_iT x_T (Ls holds the strictly lower part, so the
+// rows at and above T are unchanged).
+template <int T>
+__device__ __forceinline__ void trinv_step(double (&x)[kDiagCW], int lane, int w, const double* Ls,
+                                           const double* ldiag, double* xt) {
+  constexpr int o = T / kDiagCW, tt = T % kDiagCW;
+  double* p = xt + (T & 1) * kNB;
+  if (w == o) {
+    x[tt] = x[tt] / ldiag[T];
+    p[lane] = x[tt];
+  }
+  __syncthreads();
+  const double xT = p[lane];
+  const double* lw = Ls + T * kNB + w * kDiagCW;
+#pragma unroll
+  for (int i = 0; i < kDiagCW; i += 2) {
+    const double2 q = *reinterpret_cast<const double2*>(lw + i);
+    x[i] -= q.x * xT;
+    x[i + 1] -= q.y * xT;
+  }
+}
+
+template <int... J>
+__device__ __forceinline__ void potrf_diag_steps(double (&v)[kDiagCW], int lane, int w,
+                                                 int* fail_s, double* col, double* ldiag,
+                                                 std::integer_sequence<int, J...>) {
+  (potrf_diag_step<J>(v, lane, w, fail_s, col, ldiag), ...);
+}
+
+template <int... T>
+__device__ __forceinline__ void trinv_steps(double (&x)[kDiagCW], int lane, int w,
+                                            const double* Ls, const double* ldiag, double* xt,
+                                            std::integer_sequence<int, T...>) {
+  (trinv_step<T>(x, lane, w, Ls, ldiag, xt), ...);
+}
+
+// Factor the jb x jb diagonal block at J0, write L_JJ back and its inverse
+// into Dinv[J] (64 x 64, column-major, zero padded).  Four waves, one per
+// SIMD: lane r holds row r, wave w its columns [16w, 16w+16) (static indices:
+// every step is its own instance), so one step's products are spread over
+// the four SIMDs.  Each entry takes its updates in ascending j and each
+// division is correctly rounded, so the factor is the one the unblocked
+// algorithm gives; the entries right of the diagonal (k > r) are updated too
+// (never read), and the finished columns take exact zero updates (the sign
+// of an exact zero aside, they are unchanged).  The inverse is formed
+// right-looking, column `lane` of
+// inv(L_JJ), rows split over the waves: each entry accumulates its terms in
+// ascending t before the division by L_tt.  Past jb the block is the
+// identity in both passes.
 template <bool BATCH>
-__global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int n, int J0,
-                                                   double* __restrict__ Dinv,
-                                                   int* __restrict__ info, const CholJob* jobs) {
+__global__ __launch_bounds__(256) void k_potrf_diag(double* __restrict__ A, int n, int J0,
+                                                    double* __restrict__ Dinv,
+                                                    int* __restrict__ info, const CholJob* jobs) {
+  __shared__ __align__(16) double col[2 * kNB];
+  __shared__ __align__(16) double xt[2 * kNB];
+  __shared__ __align__(16) double Ls[kNB * kNB];  // strictly lower L_JJ, column-major
+  __shared__ double ldiag[kNB];
+  __shared__ int fail_s;
   if constexpr (BATCH) {
     const CholJob& J = jobs[blockIdx.z];
     A = J.P;
@@ -279,63 +368,45 @@ __global__ __launch_bounds__(64) void k_potrf_diag(double* __restrict__ A, int n
     if (J0 >= n) return;
   }
   if (*info != 0) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int jb = min(kNB, n - J0);
   const double* Ab = A + (long long)J0 * n + J0;
-  double v[kNB];  // v[k] = A[lane][k], k <= lane
+  if (threadIdx.x == 0) fail_s = 0;
+  double v[kDiagCW];  // v[k] = A[lane][16w + k] on and below the diagonal
 #pragma unroll
-  for (int k = 0; k < kNB; k++) v[k] = (k < jb && lane < jb && lane >= k) ? Ab[(long long)k * n + lane] : 0.0;
-  int fail = 0;
-#pragma unroll
-  for (int j = 0; j < kNB; j++) {
-    if (j < jb && !fail) {
-      const double piv = wave_bcast(v[j], j);
-      if (!(piv > 0.0)) {
-        fail = j + 1;
-      } else {
-        const double d = sqrt(piv);
-        double lij = 0.0;
-        if (lane == j) v[j] = d;
-        if (lane > j && lane < jb) {
-          lij = v[j] / d;
-          v[j] = lij;
-        }
-#pragma unroll
-        for (int k = j + 1; k < kNB; k++) {
-          const double lkj = wave_bcast(lij, k);
-          if (k <= lane) v[k] -= lij * lkj;
-        }
-      }
-    }
+  for (int k = 0; k < kDiagCW; k++) {
+    const int g = w * kDiagCW + k;
+    v[k] = (g < jb && lane < jb && lane >= g) ? Ab[(long long)g * n + lane] : (g == lane ? 1.0 : 0.0);
   }
+  __syncthreads();
+  potrf_diag_steps(v, lane, w, &fail_s, col, ldiag, std::make_integer_sequence<int, kNB>{});
+  __syncthreads();
+  const int fail = fail_s;
   if (fail) {
-    if (lane == 0) atomicCAS(info, 0, J0 + fail);
+    if (threadIdx.x == 0) atomicCAS(info, 0, J0 + fail);
     return;
   }
   double* Aw = A + (long long)J0 * n + J0;
 #pragma unroll
-  for (int k = 0; k < kNB; k++)
-    if (k < jb && lane < jb && lane >= k) Aw[(long long)k * n + lane] = v[k];
-  // Column c = lane of inv(L_JJ): x = e_c, then for t ascending x_t /= L_tt and
-  // x_i -= L_it x_t (i > t).  L_it is lane i's v[t].
-  const int c = lane;
-  double x[kNB];
-#pragma unroll
-  for (int i = 0; i < kNB; i++) x[i] = (i == c) ? 1.0 : 0.0;
-#pragma unroll
-  for (int t = 0; t < kNB; t++) {
-    if (t < jb) {
-      x[t] = x[t] / wave_bcast(v[t], t);
-#pragma unroll
-      for (int i = t + 1; i < kNB; i++) {
-        const double lit = wave_bcast(v[t], i);
-        x[i] -= lit * x[t];
-      }
-    }
+  for (int k = 0; k < kDiagCW; k++) {
+    const int g = w * kDiagCW + k;
+    Ls[g * kNB + lane] = lane > g ? v[k] : 0.0;
+    if (g < jb && lane < jb && lane >= g) Aw[(long long)g * n + lane] = v[k];
   }
+  __syncthreads();
+  // Column c = lane of inv(L_JJ): x = e_c, then for t ascending x_t /= L_tt
+  // and x_i -= L_it x_t (i > t).
+  double x[kDiagCW];
+#pragma unroll
+  for (int k = 0; k < kDiagCW; k++) x[k] = (w * kDiagCW + k == lane) ? 1.0 : 0.0;
+  trinv_steps(x, lane, w, Ls, ldiag, xt, std::make_integer_sequence<int, kNB>{});
   double* D = Dinv + (long long)(J0 / kNB) * kNB * kNB;
 #pragma unroll
-  for (int i = 0; i < kNB; i++) D[(long long)c * kNB + i] = (c < jb && i < jb) ? x[i] : 0.0;
+  for (int k = 0; k < kDiagCW; k++) {
+    const int i = w * kDiagCW + k;
+    D[(long long)lane * kNB + i] = (lane < jb && i < jb) ? x[k] : 0.0;
+  }
 }
 
 // 64 x 64 tiles: the trailing update of one step is a rank-64 product, so
@@ -779,7 +850,7 @@ int launch_potrf(double* P, int rk, double* Dinv, int* info_dev, hipStream_t st)
   for (int J = 0; J < nJ; J++) {
     const int J0 = J * kNB;
     const int jb = std::min(kNB, rk - J0);
-    hipLaunchKernelGGL(k_potrf_diag<false>, dim3(1), dim3(64), 0, st, P, rk, J0, Dinv, info_dev,
+    hipLaunchKernelGGL(k_potrf_diag<false>, dim3(1), dim3(256), 0, st, P, rk, J0, Dinv, info_dev,
                        (const CholJob*)nullptr);
     ISDF_HIP_LAUNCHED("k_potrf_diag");
     const int rest = rk - J0 - jb;
@@ -944,7 +1015,7 @@ int launch_potrf_batched(const CholJob* jobs, int nb, int max_n, hipStream_t st)
   const int nJ = (max_n + kNB - 1) / kNB;
   for (int J = 0; J < nJ; J++) {
     const int J0 = J * kNB;
-    hipLaunchKernelGGL(k_potrf_diag<true>, dim3(1, 1, nb), dim3(64), 0, st, (double*)nullptr, 0, J0,
+    hipLaunchKernelGGL(k_potrf_diag<true>, dim3(1, 1, nb), dim3(256), 0, st, (double*)nullptr, 0, J0,
                        (double*)nullptr, (int*)nullptr, jobs);
     ISDF_HIP_LAUNCHED("k_potrf_diag (batched)");
     const int rest = max_n - J0 - kNB;
