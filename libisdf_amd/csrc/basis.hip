@@ -262,101 +262,99 @@ __device__ __forceinline__ double wave_bcast(double v, int src) {
 
 constexpr int kDiagCW = kNB / 4;  // columns (and inverse rows) per wave of the diagonal factor
 
-// Step J of the diagonal-block factorization.  Wave w keeps columns
-// [16w, 16w+16) of every row (lane r = row r) in v.  The owner of column J
-// forms it (pivot, sqrt, correctly rounded divisions) and publishes L(:,J) in
-// LDS (two buffers, by the parity of J); after the barrier every wave takes
-// the rank-1 update of all its columns with no mask: L(g,J) = 0 for g <= J
-// (those lanes publish 0), so the columns left of J and column J itself are
-// unchanged.  A non-positive pivot is recorded in *fail_s (first failing
-// step + 1) and replaced by 1 so that the remaining steps stay finite
-// (nothing is written back then).  Rows and columns past jb hold the identity.
+// Interval J of the diagonal-block factor and inverse, run in lockstep.
+// Wave w keeps columns [16w, 16w+16) of every row (lane r = row r) of the
+// factor in v, and rows [16w, 16w+16) of column `lane` of the inverse in x.
+// In interval J (J = 0..64, one barrier each):
+//  * every wave applies factor step J-1, v(:,k) -= L(r,J-1) L(k,J-1), to all
+//    its columns with no mask: L(g,J-1) = 0 for g <= J-1 (those lanes publish
+//    0), so finished columns are unchanged;
+//  * every wave applies inverse step J-2, x_i -= L(i,J-2) x_{J-2}, likewise
+//    (L(i,J-2) = 0 for i <= J-2);
+//  * the owner of column J forms it (pivot, sqrt, correctly rounded
+//    divisions) and publishes L(:,J) and L_JJ; a non-positive pivot is
+//    recorded in *fail_s (first failing step + 1) and replaced by 1 so that
+//    the rest stays finite (nothing is written back then);
+//  * the owner of inverse row J-1 divides it by L_{J-1,J-1} and publishes it.
+// L(:,J) lives in col[J mod 3] from interval J to J+2, x_T in xt[T mod 2]
+// from interval T+1 to T+2.
 template <int J>
-__device__ __forceinline__ void potrf_diag_step(double (&v)[kDiagCW], int lane, int w,
-                                                int* fail_s, double* col, double* ldiag) {
-  constexpr int o = J / kDiagCW, jj = J % kDiagCW;
-  double* c = col + (J & 1) * kNB;
-  if (w == o) {
-    double piv = wave_bcast(v[jj], J);
-    const bool bad = !(piv > 0.0);
-    if (bad && *fail_s == 0) *fail_s = J + 1;
-    piv = bad ? 1.0 : piv;
-    const double d = sqrt(piv);
-    const double q = v[jj] / d;
-    const double lij = lane > J ? q : 0.0;
-    v[jj] = lane == J ? d : lij;
-    ldiag[J] = d;
-    c[lane] = lij;
+__device__ __forceinline__ void potrf_diag_interval(double (&v)[kDiagCW], double (&x)[kDiagCW],
+                                                    int lane, int w, int* fail_s, double* col,
+                                                    double* ldiag, double* xt) {
+  if constexpr (J >= 1 && J <= kNB) {
+    const double* c = col + ((J - 1) % 3) * kNB;
+    const double a = c[lane];
+    const double* cw = c + w * kDiagCW;
+#pragma unroll
+    for (int k = 0; k < kDiagCW; k += 2) {
+      const double2 l = *reinterpret_cast<const double2*>(cw + k);
+      v[k] -= a * l.x;
+      v[k + 1] -= a * l.y;
+    }
+  }
+  if constexpr (J >= 2 && J <= kNB + 1) {
+    constexpr int T = J - 2;
+    const double xT = xt[(T & 1) * kNB + lane];
+    const double* lw = col + (T % 3) * kNB + w * kDiagCW;
+#pragma unroll
+    for (int i = 0; i < kDiagCW; i += 2) {
+      const double2 q = *reinterpret_cast<const double2*>(lw + i);
+      x[i] -= q.x * xT;
+      x[i + 1] -= q.y * xT;
+    }
+  }
+  if constexpr (J < kNB) {
+    constexpr int o = J / kDiagCW, jj = J % kDiagCW;
+    if (w == o) {
+      double piv = wave_bcast(v[jj], J);
+      const bool bad = !(piv > 0.0);
+      if (bad && *fail_s == 0) *fail_s = J + 1;
+      piv = bad ? 1.0 : piv;
+      const double d = sqrt(piv);
+      const double q = v[jj] / d;
+      const double lij = lane > J ? q : 0.0;
+      v[jj] = lane == J ? d : lij;
+      ldiag[J] = d;
+      col[(J % 3) * kNB + lane] = lij;
+    }
+  }
+  if constexpr (J >= 1 && J <= kNB) {
+    constexpr int T = J - 1, o = T / kDiagCW, tt = T % kDiagCW;
+    if (w == o) {
+      x[tt] = x[tt] / ldiag[T];
+      xt[(T & 1) * kNB + lane] = x[tt];
+    }
   }
   __syncthreads();
-  const double a = c[lane];
-  const double* cw = c + w * kDiagCW;
-#pragma unroll
-  for (int k = 0; k < kDiagCW; k += 2) {
-    const double2 l = *reinterpret_cast<const double2*>(cw + k);
-    v[k] -= a * l.x;
-    v[k + 1] -= a * l.y;
-  }
-}
-
-// Step T of the inverse.  Wave w keeps rows [16w, 16w+16) of column `lane` of
-// inv(L_JJ) in x; the owner of row T divides it by L_TT and publishes it, and
-// every row takes x_i -= L_iT x_T (Ls holds the strictly lower part, so the
-// rows at and above T are unchanged).
-template <int T>
-__device__ __forceinline__ void trinv_step(double (&x)[kDiagCW], int lane, int w, const double* Ls,
-                                           const double* ldiag, double* xt) {
-  constexpr int o = T / kDiagCW, tt = T % kDiagCW;
-  double* p = xt + (T & 1) * kNB;
-  if (w == o) {
-    x[tt] = x[tt] / ldiag[T];
-    p[lane] = x[tt];
-  }
-  __syncthreads();
-  const double xT = p[lane];
-  const double* lw = Ls + T * kNB + w * kDiagCW;
-#pragma unroll
-  for (int i = 0; i < kDiagCW; i += 2) {
-    const double2 q = *reinterpret_cast<const double2*>(lw + i);
-    x[i] -= q.x * xT;
-    x[i + 1] -= q.y * xT;
-  }
 }
 
 template <int... J>
-__device__ __forceinline__ void potrf_diag_steps(double (&v)[kDiagCW], int lane, int w,
-                                                 int* fail_s, double* col, double* ldiag,
-                                                 std::integer_sequence<int, J...>) {
-  (potrf_diag_step<J>(v, lane, w, fail_s, col, ldiag), ...);
-}
-
-template <int... T>
-__device__ __forceinline__ void trinv_steps(double (&x)[kDiagCW], int lane, int w,
-                                            const double* Ls, const double* ldiag, double* xt,
-                                            std::integer_sequence<int, T...>) {
-  (trinv_step<T>(x, lane, w, Ls, ldiag, xt), ...);
+__device__ __forceinline__ void potrf_diag_intervals(double (&v)[kDiagCW], double (&x)[kDiagCW],
+                                                     int lane, int w, int* fail_s, double* col,
+                                                     double* ldiag, double* xt,
+                                                     std::integer_sequence<int, J...>) {
+  (potrf_diag_interval<J>(v, x, lane, w, fail_s, col, ldiag, xt), ...);
 }
 
 // Factor the jb x jb diagonal block at J0, write L_JJ back and its inverse
 // into Dinv[J] (64 x 64, column-major, zero padded).  Four waves, one per
 // SIMD: lane r holds row r, wave w its columns [16w, 16w+16) (static indices:
-// every step is its own instance), so one step's products are spread over
+// every interval is its own instance), so one step's products are spread over
 // the four SIMDs.  Each entry takes its updates in ascending j and each
 // division is correctly rounded, so the factor is the one the unblocked
 // algorithm gives; the entries right of the diagonal (k > r) are updated too
 // (never read), and the finished columns take exact zero updates (the sign
 // of an exact zero aside, they are unchanged).  The inverse is formed
-// right-looking, column `lane` of
-// inv(L_JJ), rows split over the waves: each entry accumulates its terms in
-// ascending t before the division by L_tt.  Past jb the block is the
-// identity in both passes.
+// right-looking, column `lane` of inv(L_JJ), rows split over the waves, one
+// step behind the factor: each entry accumulates its terms in ascending t
+// before the division by L_tt.  Past jb the block is the identity.
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_potrf_diag(double* __restrict__ A, int n, int J0,
                                                     double* __restrict__ Dinv,
                                                     int* __restrict__ info, const CholJob* jobs) {
-  __shared__ __align__(16) double col[2 * kNB];
+  __shared__ __align__(16) double col[3 * kNB];
   __shared__ __align__(16) double xt[2 * kNB];
-  __shared__ __align__(16) double Ls[kNB * kNB];  // strictly lower L_JJ, column-major
   __shared__ double ldiag[kNB];
   __shared__ int fail_s;
   if constexpr (BATCH) {
@@ -374,38 +372,28 @@ __global__ __launch_bounds__(256) void k_potrf_diag(double* __restrict__ A, int 
   const double* Ab = A + (long long)J0 * n + J0;
   if (threadIdx.x == 0) fail_s = 0;
   double v[kDiagCW];  // v[k] = A[lane][16w + k] on and below the diagonal
+  double x[kDiagCW];  // x[k] = inv(L_JJ)[16w + k][lane], from e_lane
 #pragma unroll
   for (int k = 0; k < kDiagCW; k++) {
     const int g = w * kDiagCW + k;
     v[k] = (g < jb && lane < jb && lane >= g) ? Ab[(long long)g * n + lane] : (g == lane ? 1.0 : 0.0);
+    x[k] = g == lane ? 1.0 : 0.0;
   }
   __syncthreads();
-  potrf_diag_steps(v, lane, w, &fail_s, col, ldiag, std::make_integer_sequence<int, kNB>{});
-  __syncthreads();
+  potrf_diag_intervals(v, x, lane, w, &fail_s, col, ldiag, xt,
+                       std::make_integer_sequence<int, kNB + 1>{});
   const int fail = fail_s;
   if (fail) {
     if (threadIdx.x == 0) atomicCAS(info, 0, J0 + fail);
     return;
   }
   double* Aw = A + (long long)J0 * n + J0;
-#pragma unroll
-  for (int k = 0; k < kDiagCW; k++) {
-    const int g = w * kDiagCW + k;
-    Ls[g * kNB + lane] = lane > g ? v[k] : 0.0;
-    if (g < jb && lane < jb && lane >= g) Aw[(long long)g * n + lane] = v[k];
-  }
-  __syncthreads();
-  // Column c = lane of inv(L_JJ): x = e_c, then for t ascending x_t /= L_tt
-  // and x_i -= L_it x_t (i > t).
-  double x[kDiagCW];
-#pragma unroll
-  for (int k = 0; k < kDiagCW; k++) x[k] = (w * kDiagCW + k == lane) ? 1.0 : 0.0;
-  trinv_steps(x, lane, w, Ls, ldiag, xt, std::make_integer_sequence<int, kNB>{});
   double* D = Dinv + (long long)(J0 / kNB) * kNB * kNB;
 #pragma unroll
   for (int k = 0; k < kDiagCW; k++) {
-    const int i = w * kDiagCW + k;
-    D[(long long)lane * kNB + i] = (lane < jb && i < jb) ? x[k] : 0.0;
+    const int g = w * kDiagCW + k;
+    if (g < jb && lane < jb && lane >= g) Aw[(long long)g * n + lane] = v[k];
+    D[(long long)lane * kNB + g] = (lane < jb && g < jb) ? x[k] : 0.0;
   }
 }
 
