@@ -224,19 +224,19 @@ int env_variant(const char* name) {
 // CUs the Ng-row Z C^T leaves free for the concurrent Cholesky branch (its
 // workgroups then walk the tiles).  Without them every small Cholesky kernel
 // waits for CUs held by the GEMM.  That matters only when the branch (a chain
-// of rk/64 latency-bound steps, ~0.11 ms each; the mixed forms add the
+// of rk/64 latency-bound steps, ~0.07 ms each; the mixed forms add the
 // inverse) is about as long as the GEMM (2 Ng rk K flops at ~45 TFLOP/s FP64,
-// ~140 nominal mixed); when the GEMM is much longer, reserved CUs only slow
-// it.  Measured: 64^3 FP64 18.15 -> 17.39 ms per solve with 48 CUs free,
-// 96^3 BF16X3 56.6 -> 54.6 ms with 64; 96^3 FP64 and the dual 96^3 BF16X3
-// case got slower with any reserve; 16-24 made the branch's first kernel wait
-// for the whole GEMM.
+// ~140 nominal mixed); when the GEMM is longer, reserved CUs only slow it.
+// Measured per solve: 64^3 FP64 17.65 ms with 48 CUs free, 17.43 with 32,
+// 18.66 with 24 (the branch's first kernel then waits for the whole GEMM);
+// 64^3 BF16X3 7.98 ms with 64, 8.04 with 48; 96^3 BF16X3 54.1 ms with 64,
+// 53.0 with none; 96^3 FP64 and the dual 96^3 case are slower with any.
 int reserve_cus(bool mixed, long long ng, int rk, int ktot) {
   if (const char* v = std::getenv("ISDF_ZCT_RESERVE_CUS")) return std::atoi(v);
   const double gemm_ms = 2.0 * (double)ng * rk * ktot / (mixed ? 140e9 : 45e9);
-  const double chol_ms = (rk / 64.0) * 0.11 * (mixed ? 1.6 : 1.0);
-  if (gemm_ms > 2.0 * chol_ms) return 0;
-  return mixed ? 64 : 48;
+  const double chol_ms = (rk / 64.0) * 0.07 * (mixed ? 1.6 : 1.0);
+  if (gemm_ms > 1.6 * chol_ms) return 0;
+  return mixed ? 64 : 32;
 }
 
 // Basis construction of the mixed forms.  Helper stream: the operand rows of
