@@ -899,32 +899,13 @@ __global__ __launch_bounds__(256) void k_centroid_sums(const int* __restrict__ s
 }
 
 // Division and empty-cluster compaction, utility.cpp:831-854 (swap-with-last,
-// re-testing the slot that received the last cluster).  One block.
-template <bool BATCH>
-__global__ __launch_bounds__(1024) void k_update(const double* __restrict__ sw,
-                                                 const double* __restrict__ sx,
-                                                 const double* __restrict__ sy,
-                                                 const double* __restrict__ sz,
-                                                 double* __restrict__ cx, double* __restrict__ cy,
-                                                 double* __restrict__ cz, int* __restrict__ map,
-                                                 int* scal, int* __restrict__ bbox, int rkmax,
-                                                 const KmJob* jobs) {
+// re-testing the slot that received the last cluster), by one block.
+__device__ void update_centroids(const double* __restrict__ sw, const double* __restrict__ sx,
+                                 const double* __restrict__ sy, const double* __restrict__ sz,
+                                 double* __restrict__ cx, double* __restrict__ cy,
+                                 double* __restrict__ cz, int* __restrict__ map, int* scal,
+                                 int* __restrict__ bbox, int rkmax) {
   __shared__ int nempty, rknew;
-  if constexpr (BATCH) {
-    const KmJob& J = jobs[blockIdx.y];
-    sw = J.b.sw;
-    sx = J.b.sx;
-    sy = J.b.sy;
-    sz = J.b.sz;
-    cx = J.b.cx;
-    cy = J.b.cy;
-    cz = J.b.cz;
-    map = J.b.map;
-    scal = J.b.scal;
-    bbox = J.b.bbox;
-    rkmax = J.b.rkmax;
-  }
-  if (scal[KS_ACTIVE] == 0) return;
   const int rk = scal[KS_RK];
   if (bbox)  // the next assignment rebuilds the cluster boxes
     for (int e = threadIdx.x; e < 6 * rkmax; e += blockDim.x)
@@ -974,8 +955,48 @@ __global__ __launch_bounds__(1024) void k_update(const double* __restrict__ sw,
   }
 }
 
-// Loop control, utility.cpp:813 and :906-921: the changed-label count is the
-// sum of the assignment's per-block counts.
+template <bool BATCH>
+__global__ __launch_bounds__(1024) void k_update(const double* __restrict__ sw,
+                                                 const double* __restrict__ sx,
+                                                 const double* __restrict__ sy,
+                                                 const double* __restrict__ sz,
+                                                 double* __restrict__ cx, double* __restrict__ cy,
+                                                 double* __restrict__ cz, int* __restrict__ map,
+                                                 int* scal, int* __restrict__ bbox, int rkmax,
+                                                 const KmJob* jobs) {
+  if constexpr (BATCH) {
+    const KmJob& J = jobs[blockIdx.y];
+    sw = J.b.sw;
+    sx = J.b.sx;
+    sy = J.b.sy;
+    sz = J.b.sz;
+    cx = J.b.cx;
+    cy = J.b.cy;
+    cz = J.b.cz;
+    map = J.b.map;
+    scal = J.b.scal;
+    bbox = J.b.bbox;
+    rkmax = J.b.rkmax;
+  }
+  if (scal[KS_ACTIVE] == 0) return;
+  update_centroids(sw, sx, sy, sz, cx, cy, cz, map, scal, bbox, rkmax);
+}
+
+// Loop control, utility.cpp:813 and :906-921, by one block: the changed-label
+// count is the sum of the assignment's per-block counts.
+__device__ void loop_control(int s, int max_iter, double maxF, int nblk,
+                             const int* __restrict__ changed_blk, int* trace, int* scal) {
+  int v = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) v += changed_blk[b];
+  int f;
+  block_excl_scan(v, &f);
+  if (threadIdx.x == 0) {
+    trace[s] = f;
+    scal[KS_ITERS] = s + 1;
+    if (!((double)f > maxF && s + 1 < max_iter)) scal[KS_ACTIVE] = 0;
+  }
+}
+
 template <bool BATCH>
 __global__ __launch_bounds__(1024) void k_control(int s, int max_iter, double maxF, int nblk,
                                                   const int* __restrict__ changed_blk, int* trace,
@@ -990,15 +1011,26 @@ __global__ __launch_bounds__(1024) void k_control(int s, int max_iter, double ma
     scal = J.b.scal;
   }
   if (scal[KS_ACTIVE] == 0) return;
-  int v = 0;
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x) v += changed_blk[b];
-  int f;
-  block_excl_scan(v, &f);
-  if (threadIdx.x == 0) {
-    trace[s] = f;
-    scal[KS_ITERS] = s + 1;
-    if (!((double)f > maxF && s + 1 < max_iter)) scal[KS_ACTIVE] = 0;
-  }
+  loop_control(s, max_iter, maxF, nblk, changed_blk, trace, scal);
+}
+
+// The loop control of iteration s-1 and, if the loop goes on, the centroid
+// update of iteration s, in one launch queued after the sums of iteration s.
+// Those sums read only the labels of iteration s-1, which the control does not
+// touch; when the control ends the loop, the sums are simply not used.
+__global__ __launch_bounds__(1024) void k_control_update(
+    int s, int max_iter, double maxF, int nblk, const int* __restrict__ changed_blk, int* trace,
+    const double* __restrict__ sw, const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ sz, double* __restrict__ cx, double* __restrict__ cy,
+    double* __restrict__ cz, int* __restrict__ map, int* scal, int* __restrict__ bbox,
+    int rkmax) {
+  __shared__ int go;
+  if (scal[KS_ACTIVE] == 0) return;
+  loop_control(s - 1, max_iter, maxF, nblk, changed_blk, trace, scal);
+  if (threadIdx.x == 0) go = scal[KS_ACTIVE];
+  __syncthreads();
+  if (!go) return;
+  update_centroids(sw, sx, sy, sz, cx, cy, cz, map, scal, bbox, rkmax);
 }
 
 // findMin(Dim=0), utility.cpp:926-945: per centroid, the first point at the
@@ -1338,6 +1370,9 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
   // queueing; unpolled (graph capture): every iteration is queued and the
   // kernels return at once after convergence.
   const int check_every = 4;
+  // Per iteration s: the sums, then the loop control of iteration s-1 with the
+  // centroid update of iteration s (one launch), then the assignment.
+  int last = 0;
   for (int s = 0; s < max_iter; s++) {
     if (b->grid_mode) {
       hipLaunchKernelGGL(k_centroid_sums_box<false>, dim3(rk), dim3(256), 0, st, b->ngrid[0],
@@ -1356,9 +1391,16 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
                          b->base, b->cnt_pts, b->WL, b->X, b->Y, b->Z, b->sw, b->sx, b->sy, b->sz,
                          b->scal);
     }
-    hipLaunchKernelGGL(k_update<false>, dim3(1), dim3(1024), 0, st, b->sw, b->sx, b->sy, b->sz, b->cx,
-                       b->cy, b->cz, b->map, b->scal, b->grid_mode ? b->bbox : (int*)nullptr,
-                       b->rkmax, (const KmJob*)nullptr);
+    if (s == 0)
+      hipLaunchKernelGGL(k_update<false>, dim3(1), dim3(1024), 0, st, b->sw, b->sx, b->sy, b->sz,
+                         b->cx, b->cy, b->cz, b->map, b->scal,
+                         b->grid_mode ? b->bbox : (int*)nullptr, b->rkmax, (const KmJob*)nullptr);
+    else
+      hipLaunchKernelGGL(k_control_update, dim3(1), dim3(1024), 0, st, s, max_iter, maxF,
+                         b->nblk_pts, b->changed_blk, b->trace, b->sw, b->sx, b->sy, b->sz, b->cx,
+                         b->cy, b->cz, b->map, b->scal, b->grid_mode ? b->bbox : (int*)nullptr,
+                         b->rkmax);
+    last = s;
     if (s < 3)
       hipLaunchKernelGGL((k_assign_culled<false, kCandCapWide>), dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X,
                          b->Y, b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 1, b->bbox, b->rkmax,
@@ -1367,8 +1409,6 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
       hipLaunchKernelGGL((k_assign_culled<false, kCandCap>), dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X,
                          b->Y, b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 1, b->bbox, b->rkmax,
                          b->grid_mode ? 1 : 0, b->ngrid[0], b->ngrid[1], b->changed_blk, (const KmJob*)nullptr);
-    hipLaunchKernelGGL(k_control<false>, dim3(1), dim3(1024), 0, st, s, max_iter, maxF, b->nblk_pts,
-                       b->changed_blk, b->trace, b->scal, (const KmJob*)nullptr);
     ISDF_HIP_LAUNCHED("k-means iteration");
     if (poll && (s + 1) % check_every == 0 && s + 1 < max_iter) {
       int act = 1;
@@ -1378,6 +1418,10 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
       if (!act) break;
     }
   }
+  // The loop control of the last queued assignment.
+  hipLaunchKernelGGL(k_control<false>, dim3(1), dim3(1024), 0, st, last, max_iter, maxF,
+                     b->nblk_pts, b->changed_blk, b->trace, b->scal, (const KmJob*)nullptr);
+  ISDF_HIP_LAUNCHED("k_control");
   if (!poll) return 0;
   int it = 0;
   ISDF_HIP_TRY(hipMemcpyAsync(&it, b->scal + KS_ITERS, sizeof(int), hipMemcpyDeviceToHost, st));
