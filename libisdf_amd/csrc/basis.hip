@@ -500,14 +500,19 @@ __global__ __launch_bounds__(256) void k_potrf_syrk(double* __restrict__ A, int 
 constexpr int kSB = 128;
 
 // Dinv128[J] = inv(L_JJ) for the 128-blocks: [[Da, 0], [-Db L_ba Da, Db]].
+// Both 64 x 64 products are register-blocked, 4 x 4 outputs per thread, with
+// k ascending over the full range: the terms outside the triangles are exact
+// zeros (Da and Db are zero above the diagonal), added to a sum that is still
+// zero (T) or added last (X), so each entry is the same sequence of roundings
+// as the triangular sums T(r,c) = sum_{k>=c} and X(r,c) = -sum_{k<=r}.
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_dinv128(const double* __restrict__ L, int n,
                                                  const double* __restrict__ Dinv64,
                                                  double* __restrict__ Dinv128,
                                                  const CholJob* jobs) {
-  __shared__ double da[kNB][kNB];   // da[col][row]
-  __shared__ double db[kNB][kNB];
-  __shared__ double lt[kNB][kNB];   // L_ba Da, [col][row]
+  __shared__ __align__(16) double daT[kNB][kNB];  // daT[k][c] = Da(k, c)
+  __shared__ __align__(16) double dbT[kNB][kNB];  // dbT[k][r] = Db(r, k)
+  __shared__ __align__(16) double lt[kNB][kNB];   // lt[k][r] = L_ba(r, k); then T row-major
   const int J = blockIdx.x, J0 = J * kSB;
   if constexpr (BATCH) {
     const CholJob& jb = jobs[blockIdx.z];
@@ -523,41 +528,67 @@ __global__ __launch_bounds__(256) void k_dinv128(const double* __restrict__ L, i
   const bool hasb = J0 + kNB < n;
   double* D = Dinv128 + (long long)J * kSB * kSB;
   for (int e = t; e < kNB * kNB; e += 256) {
-    const int r = e % kNB, c = e / kNB;
-    da[c][r] = Da[e];
-    db[c][r] = hasb ? Db[e] : 0.0;
+    const int r = e % kNB, c = e / kNB;  // Da, Db column-major: e = c*64 + r
+    daT[r][c] = Da[e];
+    dbT[c][r] = hasb ? Db[e] : 0.0;
     // L_ba(r, c) = L[J0+64+r, J0+c]
     lt[c][r] = (hasb && J0 + kNB + r < n) ? L[(long long)(J0 + c) * n + J0 + kNB + r] : 0.0;
   }
   __syncthreads();
-  // T = L_ba Da (64 x 64): thread owns rows r = t % 64, columns c = t/64 + 4q.
-  double tv[16];
-  const int r = t % kNB;
-  for (int q = 0; q < 16; q++) {
-    const int c = t / kNB + 4 * q;
-    double s = 0.0;
-    for (int k = c; k < kNB; k++) s += lt[k][r] * da[c][k];
-    tv[q] = s;
+  const int r0 = 4 * (t % 16), c0 = 4 * (t / 16);
+  double acc[4][4];
+  // T = L_ba Da: T(r, c) = sum_k L_ba(r, k) Da(k, c).
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+#pragma unroll 4
+  for (int k = 0; k < kNB; k++) {
+    const double2 a01 = *reinterpret_cast<const double2*>(&lt[k][r0]);
+    const double2 a23 = *reinterpret_cast<const double2*>(&lt[k][r0 + 2]);
+    const double2 b01 = *reinterpret_cast<const double2*>(&daT[k][c0]);
+    const double2 b23 = *reinterpret_cast<const double2*>(&daT[k][c0 + 2]);
+    const double a[4] = {a01.x, a01.y, a23.x, a23.y}, b[4] = {b01.x, b01.y, b23.x, b23.y};
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
   }
   __syncthreads();
-  for (int q = 0; q < 16; q++) lt[t / kNB + 4 * q][r] = tv[q];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) lt[r0 + i][c0 + j] = acc[i][j];  // T row-major
   __syncthreads();
-  // X = -Db T
-  for (int q = 0; q < 16; q++) {
-    const int c = t / kNB + 4 * q;
-    double s = 0.0;
-    for (int k = 0; k <= r; k++) s += db[k][r] * lt[c][k];
-    tv[q] = -s;
+  // X = -Db T: X(r, c) = -sum_k Db(r, k) T(k, c).
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+#pragma unroll 4
+  for (int k = 0; k < kNB; k++) {
+    const double2 a01 = *reinterpret_cast<const double2*>(&dbT[k][r0]);
+    const double2 a23 = *reinterpret_cast<const double2*>(&dbT[k][r0 + 2]);
+    const double2 b01 = *reinterpret_cast<const double2*>(&lt[k][c0]);
+    const double2 b23 = *reinterpret_cast<const double2*>(&lt[k][c0 + 2]);
+    const double a[4] = {a01.x, a01.y, a23.x, a23.y}, b[4] = {b01.x, b01.y, b23.x, b23.y};
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
   }
   for (int e = t; e < kSB * kSB; e += 256) {
     const int rr = e % kSB, cc = e / kSB;
     double v = 0.0;
-    if (rr < kNB && cc < kNB) v = da[cc][rr];
-    else if (rr >= kNB && cc >= kNB) v = db[cc - kNB][rr - kNB];
+    if (rr < kNB && cc < kNB) v = daT[rr][cc];
+    else if (rr >= kNB && cc >= kNB) v = dbT[cc - kNB][rr - kNB];
     D[e] = v;
   }
   __syncthreads();
-  for (int q = 0; q < 16; q++) D[(long long)(t / kNB + 4 * q) * kSB + kNB + r] = tv[q];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) D[(long long)(c0 + j) * kSB + kNB + r0 + i] = -acc[i][j];
 }
 
 typedef MfmaTile<128, 128, 16, 2, 2, false> FgTile;
