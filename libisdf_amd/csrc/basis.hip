@@ -727,9 +727,10 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_potrs(double* __restrict__ T, l
   }
 }
 
-// 64 x 64 tiles: at rk 1024 the lower triangle is 136 tiles instead of 36 of
-// 128 x 128, and the product sits on the mixed forms' critical path.
-typedef MfmaTile<64, 64, 16, 2, 2> XxtTile;
+// 32 x 32 tiles: the product sits on the mixed forms' critical path, and its
+// time is that of the tiles of the first block row (k runs over all of rk):
+// at rk 1024 the lower triangle is 528 tiles (136 of 64 x 64, 36 of 128 x 128).
+typedef MfmaTile<32, 32, 16, 2, 2> XxtTile;
 
 // W = X X^T for the upper-triangular X = L^{-T} (n x n, column-major), so
 // W = L^{-T} L^{-1} = (L L^T)^{-1}.  One workgroup per lower tile (ti >= tj);
