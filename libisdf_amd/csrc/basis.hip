@@ -920,7 +920,9 @@ __global__ __launch_bounds__(256) void k_linv_init(const double* __restrict__ Di
   Y[e] = (c / kSB == J) ? Dinv128[(long long)J * kSB * kSB + (c % kSB) * kSB + r % kSB] : 0.0;
 }
 
-typedef MfmaTile<64, 64, 16, 2, 2, false> LinvTile;  // B read k-contiguous
+// 32 x 32 tiles, B read k-contiguous: at rk 1024 the three levels are 64, 128
+// and 256 tiles (16, 32 and 64 of 64 x 64), on a chain that holds few CUs' work.
+typedef MfmaTile<32, 32, 16, 2, 2, false> LinvTile;
 
 // STAGE 0: T_p = L21 Y11 (s1 x s, ld s); STAGE 1: Y21 = -Y22 T_p.  Pair p =
 // blockIdx.z starts at t = 2 s p; its lower block has s1 = min(s, n - t - s) rows.
