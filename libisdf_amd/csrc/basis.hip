@@ -397,11 +397,15 @@ __global__ __launch_bounds__(256) void k_potrf_diag(double* __restrict__ A, int 
   }
 }
 
-// 64 x 64 tiles: the trailing update of one step is a rank-64 product, so
-// the work per tile is small and the step's latency is set by how many tiles
-// run side by side (128 x 128 tiles left most CUs idle).
+// The trailing update of one step is a rank-64 product, so the work per tile
+// is small and the step's latency is set by how many tiles run side by side:
+// 32 x 32 tiles (128 x 128 left most CUs idle, and 64 x 64 fill at most 120
+// CUs at rk 1024).  Every entry sums the same k range in the same 16-wide
+// chunks whatever the tile.  The panel's 1-D grid over rows needs tiles at
+// least kNB wide.
 typedef MfmaTile<64, 64, 16, 2, 2> PanelTile;
-typedef MfmaTile<64, 64, 16, 2, 2> SyrkTile;
+typedef MfmaTile<32, 32, 16, 2, 2> SyrkTile;
+static_assert(PanelTile::BN >= kNB, "the panel tile must cover the block's columns");
 
 // L_IJ = A_IJ inv(L_JJ)^T for the rows below the diagonal block (MFMA).
 // Each workgroup owns PanelTile::BM rows, so the in-place overwrite is safe once its
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(256) void k_potrf_panel(double* __restrict__ A, int
   }
 }
 
-// Trailing update A_IK -= L_IJ L_KJ^T on the lower 128 x 128 tiles (MFMA).
+// Trailing update A_IK -= L_IJ L_KJ^T on the lower tiles (MFMA).
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_potrf_syrk(double* __restrict__ A, int n, int J0,
                                                     const int* __restrict__ info,
