@@ -61,6 +61,12 @@ def lib():
         L.orc_isdf.argtypes = [_dp, ctypes.c_long, _dp, _ip] + [ctypes.c_int] * 7 + \
             [_dp, _ip, ctypes.c_int, _ip, _ip, _ip]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_glibc_rand.restype = None
+        L.orc_glibc_rand.argtypes = [ctypes.c_uint, ctypes.c_int, _ip]
+        L.orc_random_shuffle.restype = None
+        L.orc_random_shuffle.argtypes = [_ip, ctypes.c_long, ctypes.c_uint]
+        L.orc_set_scalar.restype = None
+        L.orc_set_scalar.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -112,6 +118,28 @@ def kmeans(weight, rk, max_iter, length, ngrid, kmeans_tol=1e-5, df_tol=1e-8):
     it = int(iters[0])
     return dict(piv=piv, rk_kmeans=int(rk_out[0]), iters=it, trace=trace[:it].copy(),
                 labels=labels, centroids=cent, npt=int(npt[0]), branch=br)
+
+
+def glibc_rand(seed, n):
+    """The first n values of glibc rand() after srand(seed), as the oracle restates it."""
+    out = np.zeros(n, np.int32)
+    lib().orc_glibc_rand(seed, n, _i(out))
+    return out
+
+
+def random_shuffle(a, seed=1):
+    """libstdc++ std::random_shuffle of the int32 array a (a copy is returned),
+    drawing from glibc rand() after srand(seed), as utility.cpp:679 does on the
+    tail of piv."""
+    a = np.array(a, np.int32, copy=True)
+    lib().orc_random_shuffle(_i(a), a.size, seed)
+    return a
+
+
+def set_scalar(on):
+    """Select the scalar (1) or the 4-wide vector (0, default) form of the
+    K-means distance loops; both give the same labels and piv."""
+    lib().orc_set_scalar(int(bool(on)))
 
 
 def gather(psi, piv, rk):

@@ -64,8 +64,10 @@ void orc_weights(const double* psi, long ld, int ng, int nocc, int nv1, int nc1,
 // glibc rand() (TYPE_3 additive feedback generator, default seed 1) and
 // libstdc++ std::random_shuffle, used only by the npt<rk branch of KMEAN
 // (src/common/utility.cpp:667-681).  Restated from the published algorithm
-// descriptions; the reference's own rand() state at that point is not
-// observable, so that branch is "parity unpinned".
+// descriptions and pinned against the C library's rand() and libstdc++'s
+// std::random_shuffle themselves (tests/test_oracle.py).  The one thing no
+// file of the reference records is whether anything in its process draws
+// from rand() before the shuffle; a fresh process (seed 1) is assumed.
 struct GlibcRand {
   std::vector<int32_t> hist;
   explicit GlibcRand(uint32_t seed) {
@@ -88,6 +90,25 @@ struct GlibcRand {
   }
 };
 
+// The first n values rand() returns after srand(seed) (glibc; seed 1 is the
+// state of a process that never called srand).
+void orc_glibc_rand(unsigned seed, int n, int* out) {
+  GlibcRand g(seed);
+  for (int i = 0; i < n; i++) out[i] = g.next();
+}
+
+// std::random_shuffle(first, first + len) of libstdc++ (bits/stl_algo.h): for
+// i = 1 .. len-1, j = rand() % (i + 1), swap(first[i], first[j]) when i != j,
+// with rand() in the state srand(seed) leaves.  utility.cpp:679 calls it on
+// piv[npt..n) with the process's rand() state, seed 1 when nothing else drew.
+void orc_random_shuffle(int* first, long len, unsigned seed) {
+  GlibcRand rng(seed);
+  for (long i = 1; i < len; i++) {
+    long j = rng.next() % (i + 1);
+    if (i != j) std::swap(first[i], first[j]);
+  }
+}
+
 // Reference coordinate formula, src/common/utility.cpp:711-713 (points) and
 // :758-760 (centroids).  Note x/dx0 need not equal the integer it came from;
 // the restatement keeps every rounding of the original expression.
@@ -104,26 +125,106 @@ static inline void grid_coords(int tmp, const int nG[3], const double dx[3],
 
 // pdist2 + findMin(Dim=1) fused: src/common/utility.cpp:598-615 and :535-596.
 // D(i,c) = (d1*d1 + d2*d2) + d3*d3 ; label(i) = first c reaching the minimum.
+// Four points go through one 4-wide vector at a time: every lane takes the
+// same operations in the same order as the scalar loop (no FMA), so the labels
+// are those of the scalar form (orc_set_scalar(1) selects it, for the tests).
+typedef double v4d __attribute__((vector_size(32)));
+typedef long long v4l __attribute__((vector_size(32)));
+static int g_scalar = 0;
+
+static inline int assign_one(double x, double y, double z, int rk, const double* cx,
+                             const double* cy, const double* cz) {
+  double best = 0.0;
+  int bi = 0;
+  for (int c = 0; c < rk; c++) {
+    double d1 = x - cx[c];
+    double d2 = y - cy[c];
+    double d3 = z - cz[c];
+    double d = d1 * d1 + d2 * d2 + d3 * d3;
+    if (c == 0 || d < best) {
+      best = d;
+      bi = c;
+    }
+  }
+  return bi;
+}
+
 static void assign_labels(int n, const double* X, const double* Y, const double* Z,
                           int rk, const double* cx, const double* cy, const double* cz,
                           int* label) {
+  const int n4 = g_scalar ? 0 : n / 4;
 #pragma omp parallel for schedule(static)
-  for (int i = 0; i < n; i++) {
-    double best = 0.0;
-    int bi = 0;
-    for (int c = 0; c < rk; c++) {
-      double d1 = X[i] - cx[c];
-      double d2 = Y[i] - cy[c];
-      double d3 = Z[i] - cz[c];
-      double d = d1 * d1 + d2 * d2 + d3 * d3;
-      if (c == 0 || d < best) {
-        best = d;
-        bi = c;
-      }
+  for (int q = 0; q < n4; q++) {
+    const int i = 4 * q;
+    const v4d px = {X[i], X[i + 1], X[i + 2], X[i + 3]};
+    const v4d py = {Y[i], Y[i + 1], Y[i + 2], Y[i + 3]};
+    const v4d pz = {Z[i], Z[i + 1], Z[i + 2], Z[i + 3]};
+    v4d d1 = px - cx[0], d2 = py - cy[0], d3 = pz - cz[0];
+    v4d best = d1 * d1 + d2 * d2 + d3 * d3;
+    v4l bi = {0, 0, 0, 0};
+    for (int c = 1; c < rk; c++) {
+      d1 = px - cx[c];
+      d2 = py - cy[c];
+      d3 = pz - cz[c];
+      const v4d d = d1 * d1 + d2 * d2 + d3 * d3;
+      const v4l lt = d < best;
+      best = lt ? d : best;
+      bi = lt ? (v4l){c, c, c, c} : bi;
     }
-    label[i] = bi;
+    for (int l = 0; l < 4; l++) label[i + l] = (int)bi[l];
   }
+#pragma omp parallel for schedule(static)
+  for (int i = 4 * n4; i < n; i++) label[i] = assign_one(X[i], Y[i], Z[i], rk, cx, cy, cz);
 }
+
+// findMin(Dim=0) for one centroid: the first point at the minimum distance.
+// Lane l of the vector form scans the points i = l (mod 4); the lane minima
+// are then combined by (distance, index), which is the first minimum overall.
+static int nearest_point(int n, const double* X, const double* Y, const double* Z, double cx,
+                         double cy, double cz) {
+  double best = 0.0;
+  int bi = 0;
+  int i0 = 0;
+  if (!g_scalar && n >= 4) {
+    const int n4 = n / 4;
+    v4d d1 = (v4d){X[0], X[1], X[2], X[3]} - cx;
+    v4d d2 = (v4d){Y[0], Y[1], Y[2], Y[3]} - cy;
+    v4d d3 = (v4d){Z[0], Z[1], Z[2], Z[3]} - cz;
+    v4d vb = d1 * d1 + d2 * d2 + d3 * d3;
+    v4l vi = {0, 1, 2, 3};
+    for (int q = 1; q < n4; q++) {
+      const int i = 4 * q;
+      d1 = (v4d){X[i], X[i + 1], X[i + 2], X[i + 3]} - cx;
+      d2 = (v4d){Y[i], Y[i + 1], Y[i + 2], Y[i + 3]} - cy;
+      d3 = (v4d){Z[i], Z[i + 1], Z[i + 2], Z[i + 3]} - cz;
+      const v4d d = d1 * d1 + d2 * d2 + d3 * d3;
+      const v4l lt = d < vb;
+      vb = lt ? d : vb;
+      vi = lt ? (v4l){i, i + 1, i + 2, i + 3} : vi;
+    }
+    best = vb[0];
+    bi = (int)vi[0];
+    for (int l = 1; l < 4; l++)
+      if (vb[l] < best || (vb[l] == best && vi[l] < bi)) {
+        best = vb[l];
+        bi = (int)vi[l];
+      }
+    i0 = 4 * n4;
+  }
+  for (int i = i0; i < n; i++) {
+    double d1 = X[i] - cx;
+    double d2 = Y[i] - cy;
+    double d3 = Z[i] - cz;
+    double d = d1 * d1 + d2 * d2 + d3 * d3;
+    if (i == 0 || d < best) {
+      best = d;
+      bi = i;
+    }
+  }
+  return bi;
+}
+
+void orc_set_scalar(int on) { g_scalar = on; }
 
 // KMEAN, src/common/utility.cpp:631-988, at one MPI rank (the reductions,
 // broadcasts and MINLOC all-reduce are identities there).
@@ -133,13 +234,15 @@ static void assign_labels(int n, const double* X, const double* Y, const double*
 //      does), *rk_out = number of distinct centers (the reference's local rk
 //      after unique(), :970-972), *iters = iterations run, trace[s] = changed
 //      label count of iteration s (the isdfOFS trace, :920), labels[n] = final
-//      labels (may be NULL), cent[3*rk] = final centroids x|y|z (may be NULL),
+//      labels (may be NULL), cent[3*rk] = final centroids x|y|z, blocks of rk
+//      with the first rk' (after compaction) entries written (may be NULL),
 //      *npt_out = points above the density threshold.
 // Returns 0 on the normal path, 1 on the npt<rk branch (:667-681).
 int orc_kmeans(int n, const double* weight, int rk, double kmeans_tol, int max_iter,
                double df_tol, const double length[3], const int ngrid[3], int* piv,
                int* rk_out, int* iters, int* trace, int* labels, double* cent,
                int* npt_out) {
+  const int rk0 = rk;
   int npt;
   std::vector<int> index(n, 0);  // value-initialised; entries past npt stay 0
   double maxW = 0.0;
@@ -170,13 +273,7 @@ int orc_kmeans(int n, const double* weight, int rk, double kmeans_tol, int max_i
         k1++;
       }
     }
-    GlibcRand rng(1u);
-    int* first = piv + npt;
-    long len = (long)n - npt;
-    for (long i = 1; i < len; i++) {
-      long j = rng.next() % (i + 1);
-      if (i != j) std::swap(first[i], first[j]);
-    }
+    orc_random_shuffle(piv + npt, (long)n - npt, 1u);
     if (rk_out) *rk_out = rk;
     if (iters) *iters = 0;
     return 1;
@@ -255,22 +352,9 @@ int orc_kmeans(int n, const double* weight, int rk, double kmeans_tol, int max_i
 
   // findMin(Dim=0): per centroid the first point at minimum distance (:926-945).
   std::vector<int> imin(rk);
-#pragma omp parallel for schedule(static)
-  for (int c = 0; c < rk; c++) {
-    double best = 0.0;
-    int bi = 0;
-    for (int i = 0; i < n; i++) {
-      double d1 = X[i] - cx[c];
-      double d2 = Y[i] - cy[c];
-      double d3 = Z[i] - cz[c];
-      double d = d1 * d1 + d2 * d2 + d3 * d3;
-      if (i == 0 || d < best) {
-        best = d;
-        bi = i;
-      }
-    }
-    imin[c] = indexLocal[bi];
-  }
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int c = 0; c < rk; c++)
+    imin[c] = indexLocal[nearest_point(n, X.data(), Y.data(), Z.data(), cx[c], cy[c], cz[c])];
   // unique (:617-629): sort + dedupe.
   std::sort(imin.begin(), imin.end());
   imin.erase(std::unique(imin.begin(), imin.end()), imin.end());
@@ -288,11 +372,11 @@ int orc_kmeans(int n, const double* weight, int rk, double kmeans_tol, int max_i
   }
   if (rk_out) *rk_out = rku;
   if (labels) std::memcpy(labels, label.data(), sizeof(int) * n);
-  if (cent) {
+  if (cent) {  // stride: the caller's rk; entries past the compacted count stay as given
     for (int c = 0; c < rk; c++) {
       cent[c] = cx[c];
-      cent[rk + c] = cy[c];
-      cent[2 * rk + c] = cz[c];
+      cent[rk0 + c] = cy[c];
+      cent[2 * rk0 + c] = cz[c];
     }
   }
   return 0;

@@ -154,6 +154,24 @@ def test_kmeans_matches_numpy_restatement(seed, n, rk, iters):
     assert a["rk_kmeans"] == b["rk_kmeans"]
 
 
+@pytest.mark.parametrize("seed,n,rk,iters", [(0, (6, 5, 7), 11, 30), (3, (13, 9, 6), 101, 40),
+                                             (4, (3, 3, 3), 27, 5)])
+def test_kmeans_vector_form_equals_scalar_form(seed, n, rk, iters):
+    # The 4-point vector loops (assignment and final nearest points) must give
+    # the scalar loops' labels, trace and piv; the sizes leave ragged tails.
+    rng = np.random.default_rng(seed)
+    w = rng.random(n[0] * n[1] * n[2]) ** 2
+    try:
+        oracle.set_scalar(1)
+        a = oracle.kmeans(w, rk, iters, (2.0, 3.5, 1.25), n)
+    finally:
+        oracle.set_scalar(0)
+    b = oracle.kmeans(w, rk, iters, (2.0, 3.5, 1.25), n)
+    assert a["trace"].tolist() == b["trace"].tolist()
+    np.testing.assert_array_equal(a["labels"], b["labels"])
+    np.testing.assert_array_equal(a["piv"], b["piv"])
+
+
 def test_kmeans_empty_clusters_and_zero_weight_tail():
     # w[0] = 0 and a band of zero weights: the points past npt then read grid
     # point 0 (utility.cpp:708 on the zero tail of `index`) with weight 0, so
@@ -173,6 +191,106 @@ def test_kmeans_empty_clusters_and_zero_weight_tail():
     head = a["piv"][:a["rk_kmeans"]]
     assert (np.diff(head) > 0).all()
     assert (np.diff(a["piv"][a["rk_kmeans"]:]) > 0).all()
+
+
+def test_kmeans_cluster_empties_and_is_compacted_by_hand():
+    # 3 x 4 x 1 grid, dx = 1, g = x + 3y.  Weight only at g = 1, 2, 3, 8, 11:
+    #   P0 = (1,0) w2, P1 = (2,0) w4, P2 = (0,1) w1, P3 = (2,2) w4, P4 = (2,3) w2.
+    # npt = 5 < Ng = 12, so points 5..11 read index 0 (utility.cpp:708 on the
+    # zero tail): seven copies of (0,0) carrying w[0] = 0.
+    # Initial centroids = the first rk = 3 points: c0 (1,0), c1 (2,0), c2 (0,1).
+    #   labels0: P0->0, P1->1, P2->2, P3->1 (4 < 5, 5), P4->2 (8 < 9 < 10),
+    #            copies->0 (c0 and c2 both at 1: first minimum).
+    # s=0 sums: c0 = (1,0); c1 = (16, 8)/8 = (2,1); c2 = (4, 7)/3 = (4/3, 7/3).
+    #   labels1: P0->0; P1: c0 1 == c1 1 -> 0; P2: c0 2 < c2 32/9 < c1 4 -> 0;
+    #            P3: c2 5/9 < c1 1 -> 2; P4: c2 8/9 -> 2; copies -> 0.
+    #   changed: P1, P2, P3 -> 3.  Cluster 1 now has no point.
+    # s=1 sums: c0 = (10, 1)/7, cnt1 = 0, c2 = (12, 14)/6.  Compaction
+    # (:831-854): slot 1 is empty, so rk = 2 and the entry of slot 2 moves into
+    # slot 1: centroids (10/7, 1/7), (2, 7/3).
+    #   labels2: P0, P1, P2, copies -> 0 (10/49, 17/49, 136/49, 101/49);
+    #            P3, P4 -> 1 (1/9, 4/9): their label changes 2 -> 1.  changed = 2.
+    # s=2: same centroids, same labels, changed = 0 <= 1e-5 * 12: stop.
+    # Final selection: c0 -> P0 (g 1), c1 -> P3 (g 8); piv = [1, 8, rest ascending].
+    w = np.zeros(12)
+    w[[1, 2, 3, 8, 11]] = [2.0, 4.0, 1.0, 4.0, 2.0]
+    a = oracle.kmeans(w, 3, 10, (3.0, 4.0, 1.0), (3, 4, 1))
+    assert a["npt"] == 5
+    assert a["iters"] == 3
+    assert a["trace"].tolist() == [3, 2, 0]
+    assert a["labels"].tolist() == [0, 0, 0, 1, 1] + [0] * 7
+    assert a["rk_kmeans"] == 2
+    assert a["piv"].tolist() == [1, 8, 0, 2, 3, 4, 5, 6, 7, 9, 10, 11]
+    np.testing.assert_allclose(a["centroids"][[0, 1, 3, 4]], [10 / 7, 2.0, 1 / 7, 7 / 3],
+                               rtol=1e-15)
+    b = np_kmeans(w, 3, 10, (3.0, 4.0, 1.0), (3, 4, 1))
+    assert b["trace"] == [3, 2, 0]
+
+
+# ------------------------------------------- npt < rk: rand() and random_shuffle
+
+def test_glibc_rand_stream_matches_the_c_library():
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    for seed in (1, 12345, 0xdeadbeef):
+        libc.srand(ctypes.c_uint(seed))
+        want = [libc.rand() for _ in range(3000)]
+        assert oracle.glibc_rand(seed, 3000).tolist() == want
+    libc.srand(1)
+
+
+SHUFFLE_HELPER = r"""
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+int main(int argc, char** argv) {
+  long n = std::atol(argv[1]);
+  if (argc > 2) std::srand((unsigned)std::strtoul(argv[2], 0, 10));
+  std::vector<int> v(n);
+  for (long i = 0; i < n; i++) v[i] = (int)(1000 + 3 * i);
+  std::random_shuffle(v.begin(), v.end());
+  for (long i = 0; i < n; i++) std::printf("%d\n", v[i]);
+  return 0;
+}
+"""
+
+
+def test_random_shuffle_matches_libstdcxx(tmp_path):
+    # The reference's npt < rk branch calls std::random_shuffle on piv[npt..n)
+    # (utility.cpp:679) with no srand: a fresh process draws from seed 1.
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "shuffle.cpp"
+    src.write_text(SHUFFLE_HELPER)
+    exe = tmp_path / "shuffle"
+    subprocess.check_call([gxx, "-std=c++14", "-O1", "-Wno-deprecated-declarations", str(src),
+                           "-o", str(exe)])
+    for n, seed in ((1, None), (2, None), (17, None), (5000, None), (4093, 1), (777, 99)):
+        args = [str(exe), str(n)] + ([] if seed is None else [str(seed)])
+        got = [int(x) for x in subprocess.check_output(args).split()]
+        want = oracle.random_shuffle(1000 + 3 * np.arange(n), 1 if seed is None else seed)
+        assert want.tolist() == got, (n, seed)
+
+
+def test_degenerate_branch_fill_and_tail_shuffle():
+    # npt < rk (utility.cpp:667-681): piv[k0] = i for the grid points i < npt
+    # that are above the threshold, the other i < npt go to piv[npt + k1], and
+    # the tail piv[npt..n) is shuffled in place.  Weights: points 0, 2, 5 above
+    # the threshold, npt = 3, rk = 4.
+    n = 10
+    w = np.zeros(n)
+    w[[0, 2, 5]] = 1.0
+    a = oracle.kmeans(w, 4, 5, (1.0, 1.0, 1.0), (10, 1, 1))
+    assert a["branch"] == 1 and a["npt"] == 3
+    # i = 0 -> piv[0] = 0; i = 1 -> piv[3] = 1; i = 2 -> piv[1] = 2; rest zero.
+    head = np.array([0, 2, 0], np.int32)
+    tail = np.array([1, 0, 0, 0, 0, 0, 0], np.int32)
+    np.testing.assert_array_equal(a["piv"][:3], head)
+    np.testing.assert_array_equal(a["piv"][3:], oracle.random_shuffle(tail, 1))
 
 
 def test_coordinates_reduce_to_lattice():
