@@ -19,7 +19,8 @@ from ._lib import (ISDF_EHIP, ISDF_EINVAL, ISDF_ENOMEM, ISDF_ENOTSPD, ISDF_OK,
                    ISDF_PREC_BF16X3, ISDF_PREC_FP32, ISDF_PREC_FP64, ISDF_PREC_REFINE,
                    ISDF_SELECT_KMEANS, ISDF_SELECT_QRCP)
 
-__all__ = ["Domain", "ISDF", "ISDF_host", "isdf_rank", "solve", "solve_host", "solve_batched",
+__all__ = ["Domain", "ISDF", "ISDF_host", "isdf_rank", "buffer_sizes", "solve", "solve_host",
+           "solve_batched",
            "check", "check_device", "fit_error", "ISDF_SELECT_KMEANS", "ISDF_SELECT_QRCP", "ISDF_PREC_FP64", "ISDF_PREC_FP32",
            "ISDF_PREC_BF16X3", "ISDF_PREC_REFINE"]
 
@@ -64,19 +65,47 @@ def isdf_rank(nv1, nc1, nv2, nc2, mu_points):
     return _lib.lib().isdf_hip_rank(ctypes.byref(p))
 
 
-def _dev_ptr(x):
+def buffer_sizes(ng, nstate, rk, ld_psi=0):
+    """Minimum element counts of the device buffers of one solve: psi is the
+    column-major Ng x nstate matrix with leading dimension ld (0 -> Ng), so it
+    spans ld*(nstate-1) + Ng doubles; Theta is Ng x rk with ld Ng."""
+    ld = ld_psi if ld_psi else ng
+    if ld < ng:
+        raise ValueError("ld_psi (%d) is below Ng (%d)" % (ld, ng))
+    return ld * (max(nstate, 1) - 1) + ng, ng * rk
+
+
+def _dev_ptr(x, need=None, name="buffer"):
+    """Device address of x.  For a tensor: on the current HIP device, float64,
+    contiguous and holding at least `need` elements (a short buffer would be
+    written or read past its end by the kernels).  An int is taken as a raw
+    device address whose size the caller vouches for."""
     if isinstance(x, int):
         return x
     if hasattr(x, "data_ptr"):
         import torch
         if not x.is_cuda:
-            raise ValueError("device buffer expected (a CUDA/HIP tensor)")
+            raise ValueError("%s: device buffer expected (a CUDA/HIP tensor)" % name)
         if x.dtype != torch.float64:
-            raise ValueError("float64 buffer expected")
+            raise ValueError("%s: float64 buffer expected" % name)
         if not x.is_contiguous():
-            raise ValueError("contiguous buffer expected")
+            raise ValueError("%s: contiguous buffer expected" % name)
+        if x.device.index != torch.cuda.current_device():
+            raise ValueError("%s is on device %d, the current device is %d"
+                             % (name, x.device.index, torch.cuda.current_device()))
+        if need is not None and x.numel() < need:
+            raise ValueError("%s holds %d elements, the problem needs %d"
+                             % (name, x.numel(), need))
         return x.data_ptr()
     raise TypeError("device buffer must be a torch tensor on the GPU or an int pointer")
+
+
+def _host_piv(piv, n, name="piv"):
+    if not (isinstance(piv, np.ndarray) and piv.dtype == np.int32 and piv.flags.c_contiguous
+            and piv.size >= n):
+        raise ValueError("%s must be a contiguous numpy int32 array of at least %d entries"
+                         % (name, n))
+    return piv
 
 
 def _stream_ptr(stream):
@@ -110,9 +139,11 @@ def solve(d_psi, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_theta, p
     p = _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, max_iter, precision, ld_psi,
                  select)
     ng = domain.NumGridTotal()
-    if not (isinstance(piv, np.ndarray) and piv.dtype == np.int32 and piv.flags.c_contiguous
-            and piv.size >= ng):
-        raise ValueError("piv must be a contiguous numpy int32 array of Ng entries")
+    _host_piv(piv, ng)
+    rk = isdf_rank(nv1, nc1, nv2, nc2, mu_points)
+    n_psi, n_theta = buffer_sizes(ng, nstate, rk, ld_psi)
+    psi_ptr = _dev_ptr(d_psi, n_psi, "d_psi")
+    theta_ptr = _dev_ptr(d_theta, n_theta, "d_theta")
     rep = _lib.IsdfReport()
     dbg = None
     if debug:
@@ -121,11 +152,17 @@ def solve(d_psi, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_theta, p
             t = debug["trace"]
             assert t.dtype == np.int32 and t.size >= max(max_iter, 1)
             dbg.h_trace = t.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
-        for key in ("weights", "labels", "cct"):
+        import torch
+        for key, dt, n in (("weights", torch.float64, ng), ("labels", torch.int32, ng),
+                           ("cct", torch.float64, rk * rk)):
             if key in debug:
-                setattr(dbg, "d_" + key, debug[key].data_ptr())
-    rc = L.isdf_hip_solve(ctypes.byref(p), ctypes.c_void_p(_dev_ptr(d_psi)),
-                          ctypes.c_void_p(_dev_ptr(d_theta)),
+                x = debug[key]
+                if not (x.is_cuda and x.dtype == dt and x.is_contiguous() and x.numel() >= n
+                        and x.device.index == torch.cuda.current_device()):
+                    raise ValueError("debug['%s'] must be a contiguous %s tensor of at least %d "
+                                     "elements on the current device" % (key, dt, n))
+                setattr(dbg, "d_" + key, x.data_ptr())
+    rc = L.isdf_hip_solve(ctypes.byref(p), ctypes.c_void_p(psi_ptr), ctypes.c_void_p(theta_ptr),
                           piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                           ctypes.c_void_p(_stream_ptr(stream)), ctypes.byref(rep),
                           ctypes.byref(dbg) if dbg is not None else None)
@@ -188,13 +225,12 @@ def solve_batched(instances, precision=ISDF_PREC_FP32, stream=None):
     for i, a in enumerate(instances):
         probs[i] = _problem(a["domain"], a["nocc"], a["nstate"], a["nv1"], a["nc1"], a["nv2"],
                             a["nc2"], a["mu_points"], a["max_iter"], precision)
-        piv = a["piv"]
         ng = a["domain"].NumGridTotal()
-        if not (isinstance(piv, np.ndarray) and piv.dtype == np.int32
-                and piv.flags.c_contiguous and piv.size >= ng):
-            raise ValueError("piv must be a contiguous numpy int32 array of Ng entries")
-        psis[i] = _dev_ptr(a["d_psi"])
-        thetas[i] = _dev_ptr(a["d_theta"])
+        piv = _host_piv(a["piv"], ng, "instance %d piv" % i)
+        rk = isdf_rank(a["nv1"], a["nc1"], a["nv2"], a["nc2"], a["mu_points"])
+        n_psi, n_theta = buffer_sizes(ng, a["nstate"], rk)
+        psis[i] = _dev_ptr(a["d_psi"], n_psi, "instance %d d_psi" % i)
+        thetas[i] = _dev_ptr(a["d_theta"], n_theta, "instance %d d_theta" % i)
         pivs[i] = piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
     reps = (_lib.IsdfReport * n)()
     brep = _lib.IsdfBatchReport()
@@ -227,6 +263,18 @@ def check(psi, theta, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, 
 _validate = check
 
 
+def _check_args(d_psi, d_theta, piv, domain, nstate, nv1, nc1, nv2, nc2, mu_points, ld_psi):
+    """Device addresses and the piv array of a check / fit-error call, with
+    the buffer sizes of buffer_sizes() and at least rk pivots."""
+    ng = domain.NumGridTotal()
+    rk = isdf_rank(nv1, nc1, nv2, nc2, mu_points)
+    n_psi, n_theta = buffer_sizes(ng, nstate, rk, ld_psi)
+    piv = np.ascontiguousarray(piv, np.int32)
+    if piv.size < rk:
+        raise ValueError("piv holds %d entries, rk = %d are needed" % (piv.size, rk))
+    return _dev_ptr(d_psi, n_psi, "d_psi"), _dev_ptr(d_theta, n_theta, "d_theta"), piv
+
+
 def check_device(d_psi, d_theta, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points,
                  coulomb=True, stream=None, ld_psi=0):
     """isdf_hip_check_device: check=true on the device (checkcode.cpp:54-313),
@@ -234,10 +282,11 @@ def check_device(d_psi, d_theta, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2, 
     fit_error().  Returns the dict check() returns."""
     L = _lib.lib()
     p = _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, 0, ISDF_PREC_FP64, ld_psi)
-    piv = np.ascontiguousarray(piv, np.int32)
+    psi_ptr, theta_ptr, piv = _check_args(d_psi, d_theta, piv, domain, nstate, nv1, nc1, nv2,
+                                          nc2, mu_points, ld_psi)
     out = _lib.IsdfCheck()
-    rc = L.isdf_hip_check_device(ctypes.byref(p), ctypes.c_void_p(_dev_ptr(d_psi)),
-                                 ctypes.c_void_p(_dev_ptr(d_theta)),
+    rc = L.isdf_hip_check_device(ctypes.byref(p), ctypes.c_void_p(psi_ptr),
+                                 ctypes.c_void_p(theta_ptr),
                                  piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                  int(bool(coulomb)), ctypes.c_void_p(_stream_ptr(stream)),
                                  ctypes.byref(out))
@@ -256,10 +305,11 @@ def fit_error(d_psi, d_theta, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_
     fit_error and z_norm."""
     L = _lib.lib()
     p = _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, 0, ISDF_PREC_FP64, ld_psi)
-    piv = np.ascontiguousarray(piv, np.int32)
+    psi_ptr, theta_ptr, piv = _check_args(d_psi, d_theta, piv, domain, nstate, nv1, nc1, nv2,
+                                          nc2, mu_points, ld_psi)
     out = _lib.IsdfCheck()
-    rc = L.isdf_hip_fit_error(ctypes.byref(p), ctypes.c_void_p(_dev_ptr(d_psi)),
-                              ctypes.c_void_p(_dev_ptr(d_theta)),
+    rc = L.isdf_hip_fit_error(ctypes.byref(p), ctypes.c_void_p(psi_ptr),
+                              ctypes.c_void_p(theta_ptr),
                               piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                               ctypes.c_void_p(_stream_ptr(stream)), ctypes.byref(out))
     if rc != ISDF_OK:
@@ -311,16 +361,15 @@ def ISDF(d_psirow, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_thetaC
     Ng x rk column-major ((rk, Ng) C-contiguous), written.  piv: host int32
     numpy array of Ng entries, written.  Raises RuntimeError with the
     reference's message when the C C^T Cholesky fails.  With check=True the
-    fit and Coulomb errors are returned in the report (keys 'fit_error',
-    'coulomb_error'), as checkcode prints them.
+    fit and Coulomb errors are computed on the device (check_device) and
+    returned in the report (keys 'fit_error', 'coulomb_error'), as checkcode
+    prints them.
     """
     rep = solve(d_psirow, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_thetaCol, piv,
                 KmeansMaxIter_ISDF, stream=stream)
     if check:
-        import torch
-        torch.cuda.synchronize()
-        psi_h = d_psirow.detach().cpu().numpy()
-        th_h = d_thetaCol.detach().cpu().numpy()
-        rep.update(_validate(psi_h, th_h, piv, domain, nocc, nstate, nv1, nc1, nv2, nc2,
-                             mu_points))
+        # Both halves of checkcode on the device, on the caller's buffers (as
+        # the C++ drop-in does): no host copy of psi or Theta.
+        rep.update(check_device(d_psirow, d_thetaCol, piv, domain, nocc, nstate, nv1, nc1, nv2,
+                                nc2, mu_points, coulomb=True, stream=stream))
     return rep

@@ -18,6 +18,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libisdf_hip.so")
+# Register-only arithmetic probes (bench.py --peak-probe); not part of the ISDF library.
+PROBE_LIB = os.path.join(HERE, "libisdf_probe.so")
 
 ARCH = os.environ.get("ISDF_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -58,22 +60,28 @@ def _compile(src, extra, verbose):
     return obj, "\n".join(lines)
 
 
-def build(verbose=False):
-    os.makedirs(BUILD, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        res = list(ex.map(lambda s: _compile(s, SOURCES[s], verbose), srcs))
-    objs = []
-    for (obj, log), s in zip(res, srcs):
-        objs.append(obj)
-        if verbose and log.strip():
-            print("== %s\n%s" % (s, log))
-    tmp = LIB + ".tmp"
+def _link(objs, lib):
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "-shared", "--offload-arch=" + ARCH, "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))] + ["probe.hip"]
+    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        res = list(ex.map(lambda s: _compile(s, SOURCES.get(s, []), verbose), srcs))
+    objs = []
+    for (obj, log), s in zip(res, srcs):
+        if s != "probe.hip":
+            objs.append(obj)
+        if verbose and log.strip():
+            print("== %s\n%s" % (s, log))
+    _link(objs, LIB)
+    _link([os.path.join(BUILD, "probe.hip.o")], PROBE_LIB)
     return LIB
 
 

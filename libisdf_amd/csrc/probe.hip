@@ -1,0 +1,105 @@
+// Register-only throughput probes for the FP64 and FP32 arithmetic units of
+// gfx950: what one MI355X sustains on v_mfma_f64_16x16x4f64, on v_fma_f64 and
+// on v_mfma_f32_16x16x4f32 with no memory traffic in the loop.  A measurement
+// tool for the roofline figures (libisdf_probe.so, loaded by bench.py
+// --peak-probe); the ISDF library does not use it.
+//
+// Each wave runs `iters` rounds of NCHAIN independent accumulator chains, so
+// the unit never waits for a dependent result; the accumulators are stored
+// once at the end so that nothing is optimised away.
+
+#include <hip/hip_runtime.h>
+
+namespace {
+
+constexpr int kChains = 8;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_mfma_f64(double a0, double b0, int iters, double* out) {
+  d4 acc[kChains];
+  for (int c = 0; c < kChains; c++) acc[c] = (d4){0.0, 0.0, 0.0, 0.0};
+  const double a = a0 + threadIdx.x * 1e-9, b = b0 - threadIdx.x * 1e-9;
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int c = 0; c < kChains; c++) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
+  }
+  double s = 0.0;
+  for (int c = 0; c < kChains; c++) s += acc[c][0] + acc[c][1] + acc[c][2] + acc[c][3];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_mfma_f32(float a0, float b0, int iters, double* out) {
+  f4 acc[kChains];
+  for (int c = 0; c < kChains; c++) acc[c] = (f4){0.f, 0.f, 0.f, 0.f};
+  const float a = a0 + threadIdx.x * 1e-6f, b = b0 - threadIdx.x * 1e-6f;
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int c = 0; c < kChains; c++) acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[c], 0, 0, 0);
+  }
+  double s = 0.0;
+  for (int c = 0; c < kChains; c++) s += acc[c][0] + acc[c][1] + acc[c][2] + acc[c][3];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// 16 independent FMA chains per lane (enough to cover the FP64 VALU latency).
+__global__ __launch_bounds__(256) void k_fma_f64(double a0, double b0, int iters, double* out) {
+  double acc[16];
+  for (int c = 0; c < 16; c++) acc[c] = c * 1e-3;
+  const double a = a0 + threadIdx.x * 1e-12, b = b0;
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int c = 0; c < 16; c++) acc[c] = __builtin_fma(acc[c], a, b);
+  }
+  double s = 0.0;
+  for (int c = 0; c < 16; c++) s += acc[c];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+}  // namespace
+
+extern "C" {
+
+// which: 0 v_mfma_f64_16x16x4f64, 1 v_fma_f64, 2 v_mfma_f32_16x16x4f32.
+// Launches `blocks` workgroups of 256 threads (4 waves), each running `iters`
+// rounds, twice (the first run warms the clocks up), and returns the second
+// run's rate in TFLOP/s through *tflops and its time through *ms.
+// Returns 0, or a hipError_t value.
+int isdf_probe_peak(int which, int blocks, int iters, double* tflops, double* ms) {
+  double* out = nullptr;
+  hipError_t e = hipMalloc(&out, sizeof(double) * (size_t)blocks * 256);
+  if (e != hipSuccess) return (int)e;
+  hipEvent_t t0, t1;
+  (void)hipEventCreate(&t0);
+  (void)hipEventCreate(&t1);
+  double flops_per_thread_iter = 0.0;
+  for (int rep = 0; rep < 2; rep++) {
+    (void)hipEventRecord(t0, nullptr);
+    if (which == 0) {
+      k_mfma_f64<<<blocks, 256>>>(1.0, 0.5, iters, out);
+      // 16x16x4 f64: 2 * 16 * 16 * 4 flops per wave instruction = 32 per lane.
+      flops_per_thread_iter = 32.0 * kChains;
+    } else if (which == 1) {
+      k_fma_f64<<<blocks, 256>>>(0.999, 1e-3, iters, out);
+      flops_per_thread_iter = 2.0 * 16;
+    } else {
+      k_mfma_f32<<<blocks, 256>>>(1.0f, 0.5f, iters, out);
+      flops_per_thread_iter = 32.0 * kChains;
+    }
+    (void)hipEventRecord(t1, nullptr);
+  }
+  e = hipEventSynchronize(t1);
+  float t = 0.f;
+  (void)hipEventElapsedTime(&t, t0, t1);
+  if (e == hipSuccess) e = hipGetLastError();
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  (void)hipFree(out);
+  if (e != hipSuccess) return (int)e;
+  *ms = t;
+  *tflops = flops_per_thread_iter * (double)iters * blocks * 256 / (t * 1e-3) / 1e12;
+  return 0;
+}
+
+}  // extern "C"

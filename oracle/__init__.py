@@ -373,3 +373,48 @@ def qrcp_points(psi, nocc, nv1, nc1, nv2, nc2, rk):
     piv = np.zeros(ng, np.int32)
     piv[:len(idx)] = idx[perm]
     return piv, numpre, len(idx)
+
+
+# --- BLAS-3 forms of the basis stages (CPU baseline timing) -------------------
+# The reference runs these stages through MKL's pdgemm, pdpotrf and pdtrsm
+# (src/cpu/getbasis_scalapack.cpp:297-369, :455, :469-478).  The C++
+# restatement above follows netlib loop orders, which pins results but runs at
+# scalar speed; these forms call the blocked, threaded BLAS/LAPACK that numpy
+# and scipy link (OpenBLAS), so the CPU baseline reflects what a BLAS-3 CPU
+# ISDF costs.  They agree with the restatement to rounding (tests/test_oracle.py).
+
+def zct_rows_blas(psi, psimu, nocc, nv1, nc1, nv2, nc2, rows):
+    """Z C^T at the given grid rows as dgemm(s) plus the Hadamard product;
+    returns (len(rows), rk) row-major (the transpose of zct_pick's layout)."""
+    s1, K1, s2, K2 = nocc - nv1, nv1 + nc1, nocc - nv2, nv2 + nc2
+    A1 = psi[s1:s1 + K1][:, rows].T
+    P1 = A1 @ psimu[:, s1:s1 + K1].T
+    if nv1 == nv2 and nc1 == nc2:
+        return P1 * P1
+    A2 = psi[s2:s2 + K2][:, rows].T
+    return P1 * (A2 @ psimu[:, s2:s2 + K2].T)
+
+
+def potrf_lapack(P):
+    """Lower Cholesky factor of the symmetric P (LAPACK dpotrf)."""
+    import scipy.linalg as sl
+    return sl.cholesky(P, lower=True, check_finite=False)
+
+
+def potrs_rows_blas(L, Z):
+    """Rows of Z (rows x rk, row-major) times (L L^T)^{-1}: the two right-side
+    triangular solves of getbasis_scalapack.cpp:469-478 as dtrsm calls."""
+    import scipy.linalg as sl
+    X = sl.solve_triangular(L, Z.T, lower=True, check_finite=False)
+    return sl.solve_triangular(L, X, lower=True, trans="T", check_finite=False).T
+
+
+def blas_threads():
+    """Threads of the BLAS numpy uses (threadpoolctl), or None."""
+    try:
+        import threadpoolctl
+        info = threadpoolctl.threadpool_info()
+        n = [i["num_threads"] for i in info if i.get("user_api") == "blas"]
+        return max(n) if n else None
+    except Exception:
+        return None

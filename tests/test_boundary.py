@@ -200,3 +200,27 @@ def test_fit_error_rejects_before_device_work():
         isdf.fit_error(1, 1, piv, dom, 8, 16, 8, 4, 8, 4, 64, stream=0)
     with pytest.raises(ValueError, match="pair ranges"):
         isdf.fit_error(1, 1, np.zeros(512, np.int32), dom, 8, 16, 9, 4, 8, 4, 64, stream=0)
+
+
+def test_device_buffer_size_rule():
+    # psi: column-major Ng x nstate with leading dimension ld spans
+    # ld (nstate - 1) + Ng doubles; Theta: Ng x rk.
+    assert isdf.buffer_sizes(1000, 8, 64) == (8000, 64000)
+    assert isdf.buffer_sizes(1000, 8, 64, ld_psi=1024) == (1024 * 7 + 1000, 64000)
+    assert isdf.buffer_sizes(10, 1, 3) == (10, 30)
+    with pytest.raises(ValueError, match="below Ng"):
+        isdf.buffer_sizes(1000, 8, 64, ld_psi=999)
+
+
+def test_device_entry_points_reject_host_and_short_buffers():
+    # No device work happens: every argument error is raised in Python first.
+    torch = pytest.importorskip("torch")
+    dom = isdf.Domain((4.0, 4.0, 4.0), (8, 8, 8))
+    piv = np.zeros(512, np.int32)
+    host = torch.zeros(16 * 512, dtype=torch.float64)
+    with pytest.raises(ValueError, match="device buffer expected"):
+        isdf.solve(host, dom, 8, 16, 8, 8, 8, 8, 64, host, piv, 10)
+    with pytest.raises(ValueError, match="at least 512"):
+        isdf.solve(host, dom, 8, 16, 8, 8, 8, 8, 64, host, piv[:100], 10)
+    with pytest.raises(ValueError, match="rk = 64 are needed"):
+        isdf.fit_error(host, host, piv[:10], dom, 8, 16, 8, 8, 8, 8, 64)

@@ -102,3 +102,44 @@ def test_batched_pivots_match_oracle():
         w = oracle.weights(b["psi"], b["nocc"], b["nocc"], b["nocc"], b["nocc"], b["nocc"])
         km = oracle.kmeans(w, b["mu"], b["iters"], b["L"], b["grid"])
         np.testing.assert_array_equal(a["piv"], km["piv"])
+
+
+def test_batched_ragged_rk_above_one_block():
+    # rk 136, 264 and 384 (nstate 34, 66, 96): several 128-wide block columns
+    # per instance, so the off-diagonal F / G blocks, the forward-only identity
+    # solve across block columns and the batched X X^T on instances of
+    # different n all run.
+    rng = np.random.default_rng(13)
+    batch = []
+    for i, (n, nstate) in enumerate(((24, 34), (28, 66), (32, 96))):
+        grid = (n, n, n - 4 * (i % 2))
+        L = tuple(0.32 * g for g in grid)
+        ng = grid[0] * grid[1] * grid[2]
+        psi = synth.cholesky_qr2(rng.standard_normal((nstate, ng)), L[0] * L[1] * L[2] / ng)
+        batch.append(dict(psi=psi, grid=grid, L=L, nstate=nstate, nocc=nstate // 2,
+                          mu=8 * (nstate // 2), iters=15))
+    assert [b["mu"] for b in batch] == [136, 264, 384]
+    for precision in (isdf.ISDF_PREC_FP32, isdf.ISDF_PREC_BF16X3):
+        inst = to_instances(batch)
+        reps, brep = isdf.solve_batched(inst, precision=precision)
+        torch.cuda.synchronize()
+        first = [a["d_theta"].cpu().numpy() for a in inst]
+        for b, a, th0 in zip(batch, inst, first):
+            w = oracle.weights(b["psi"], b["nocc"], b["nocc"], b["nocc"], b["nocc"], b["nocc"])
+            km = oracle.kmeans(w, b["mu"], b["iters"], b["L"], b["grid"])
+            np.testing.assert_array_equal(a["piv"], km["piv"])
+            th, piv, rep = single(a, precision)
+            np.testing.assert_array_equal(piv, km["piv"])
+            assert np.isfinite(th0).all()
+            err = np.linalg.norm(th0 - th) / np.linalg.norm(th)
+            assert err < 1e-12, err
+        for _ in range(2):
+            for a in inst:
+                a["d_theta"].fill_(0.0)
+            torch.cuda.synchronize()
+            _, brep2 = isdf.solve_batched(inst, precision=precision)
+            torch.cuda.synchronize()
+            assert brep2["graph_replayed"] == 1
+            for a, th0 in zip(inst, first):
+                np.testing.assert_array_equal(a["d_theta"].cpu().numpy().view(np.uint64),
+                                              th0.view(np.uint64))

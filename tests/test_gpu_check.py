@@ -96,3 +96,22 @@ def test_device_check_with_strided_psi(si8):
     cw = isdf.check_device(torch.from_numpy(wide).cuda(), d_theta, piv, dom, *args,
                            ld_psi=ng + 33)
     assert cw == c
+
+
+def test_python_isdf_check_runs_on_device_cfg1():
+    # ISDF(check=True) in Python checks on the device, as the C++ drop-in does,
+    # and agrees with the host validator on the same Theta and piv.
+    w = synth.WORKLOADS["cfg1"]
+    psi = synth.wavefunctions(w)
+    rk = w.mu_points
+    d_psi = torch.from_numpy(psi).cuda()
+    d_theta = torch.empty((rk, w.ng), dtype=torch.float64, device="cuda")
+    piv = np.zeros(w.ng, np.int32)
+    dom = isdf.Domain(w.length, w.ngrid)
+    rep = isdf.ISDF(d_psi, dom, w.nocc, w.nstate, w.nv1, w.nc1, w.nv2, w.nc2, w.mu_points,
+                    d_theta, piv, w.kmeans_iters, check=True)
+    host = isdf.check(psi, d_theta.cpu().numpy(), piv, dom, w.nocc, w.nstate, w.nv1, w.nc1,
+                      w.nv2, w.nc2, w.mu_points)
+    assert rep["fit_error"] == pytest.approx(host["fit_error"], rel=1e-10)
+    assert rep["coulomb_error"] == pytest.approx(host["coulomb_error"], rel=1e-8)
+    assert rep["coulomb_error"] > 0

@@ -383,6 +383,29 @@ def test_potrs_matches_scipy():
     np.testing.assert_allclose(T, ref, rtol=1e-11, atol=1e-12)
 
 
+@pytest.mark.parametrize("dual", [False, True])
+def test_blas3_basis_forms_agree_with_the_restatement(dual):
+    # The BLAS-3 forms the CPU baseline times give the restatement's results to rounding.
+    psi, n, nocc, nv1, nc1, nv2, nc2 = small_case()
+    if not dual:
+        nv2, nc2 = nv1, nc1
+    rk = 21
+    piv = np.random.default_rng(6).permutation(psi.shape[1]).astype(np.int32)
+    psimu = oracle.gather(psi, piv, rk)
+    rows = np.arange(psi.shape[1], dtype=np.int32)
+    zr = oracle.zct_pick(psi, psimu, nocc, nv1, nc1, nv2, nc2, rows)
+    zb = oracle.zct_rows_blas(psi, psimu, nocc, nv1, nc1, nv2, nc2, rows)
+    np.testing.assert_allclose(zb, zr.T, rtol=1e-12, atol=1e-13)
+    P, eps = oracle.cct_from_psimu(psi, psimu, piv, nocc, nv1, nc1, nv2, nc2)
+    Lc, info = oracle.potrf(P)
+    assert info == 0
+    Lb = oracle.potrf_lapack(P.T)
+    np.testing.assert_allclose(Lb, np.tril(Lc.T), rtol=1e-9, atol=1e-12 * np.abs(Lb).max())
+    T = oracle.potrs_rows(Lc, zr)
+    Tb = oracle.potrs_rows_blas(Lb, zb)
+    assert np.linalg.norm(Tb - T.T) / np.linalg.norm(T) < 1e-8
+
+
 # ---------------------------------------------------------------- validators
 
 def test_fit_error_against_dense_numpy():
