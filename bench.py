@@ -39,6 +39,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_potrs launch from two rocprofv3 counter passes (FETCH_SIZE x 2.000,
 # WRITE_SIZE x 1.000, factors from a copy kernel of known size), per workload.
 TRAFFIC = {
+    "cfg3": {"bytes": 2.9442e11,
+             "source": "profiles/k_potrs_k_zct_cfg3_fp64_hbm_bytes.md (265.28 GB read + "
+                       "29.15 GB written per launch; 57.98 GB algorithmic)"},
     "cfg2": {"bytes": 2.490e10,
              "source": "profiles/k_potrs_k_zct_cfg2_fp64_hbm_bytes.md (20.56 GB read + "
                        "4.341 GB written per launch)"},

@@ -7,7 +7,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libisdf_hip.so")
+# ISDF_HIP_LIB selects another build of the same library (the host-sanitizer
+# build, libisdf_amd/_san/libisdf_hip.so, for CPU-only test runs).
+LIB_PATH = os.environ.get("ISDF_HIP_LIB") or os.path.join(_HERE, "libisdf_hip.so")
 
 ISDF_OK, ISDF_EINVAL, ISDF_ENOTSPD, ISDF_EHIP, ISDF_ENOMEM = 0, 1, 2, 3, 4
 ISDF_PREC_FP64, ISDF_PREC_FP32, ISDF_PREC_BF16X3, ISDF_PREC_REFINE = 0, 1, 2, 16

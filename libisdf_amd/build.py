@@ -2,6 +2,7 @@
 
     python -m libisdf_amd.build            # compile everything
     python -m libisdf_amd.build --verbose  # also print kernel resource usage
+    python -m libisdf_amd.build --sanitize # host code with ASan + UBSan into _san/
 
 The shared library lands next to this file (libisdf_amd/libisdf_hip.so) so it
 travels with the source tree.  hipcc cross-compiles; no GPU is needed.
@@ -18,6 +19,13 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libisdf_hip.so")
+# Host-sanitizer build: the host half of every file with AddressSanitizer and
+# UndefinedBehaviorSanitizer (-Xarch_host: the device code is built as usual).
+# For CPU-only test runs with the clang ASan runtime preloaded.
+SAN_BUILD = os.path.join(HERE, "_build_san")
+SAN_LIB = os.path.join(HERE, "_san", "libisdf_hip.so")
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-g"]
 # Register-only arithmetic probes (bench.py --peak-probe); not part of the ISDF library.
 PROBE_LIB = os.path.join(HERE, "libisdf_probe.so")
 
@@ -46,8 +54,8 @@ SOURCES = {
 }
 
 
-def _compile(src, extra, verbose):
-    obj = os.path.join(BUILD, src + ".o")
+def _compile(src, extra, verbose, build_dir=BUILD):
+    obj = os.path.join(build_dir, src + ".o")
     cmd = [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
     if verbose and src.endswith(".hip"):
         cmd += ["-Rpass-analysis=kernel-resource-usage"]
@@ -60,9 +68,9 @@ def _compile(src, extra, verbose):
     return obj, "\n".join(lines)
 
 
-def _link(objs, lib):
+def _link(objs, lib, extra=()):
     tmp = lib + ".tmp"
-    cmd = [HIPCC, "-shared", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "--offload-arch=" + ARCH, "-o", tmp] + list(extra) + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
@@ -85,9 +93,22 @@ def build(verbose=False):
     return LIB
 
 
+def build_sanitized():
+    """The library with the host code under ASan + UBSan (SAN_LIB)."""
+    os.makedirs(SAN_BUILD, exist_ok=True)
+    os.makedirs(os.path.dirname(SAN_LIB), exist_ok=True)
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        res = list(ex.map(lambda s: _compile(s, SOURCES[s] + SAN_FLAGS, False, SAN_BUILD), srcs))
+    _link([obj for obj, _ in res], SAN_LIB, ["-Xarch_host", "-shared-libsan",
+                                             "-Xarch_host", "-fsanitize=address,undefined"])
+    return SAN_LIB
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--sanitize", action="store_true")
     a = ap.parse_args()
-    print(build(a.verbose))
+    print(build_sanitized() if a.sanitize else build(a.verbose))
     sys.exit(0)

@@ -15,7 +15,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle_isdf.so")
+# ISDF_ORACLE_LIB selects another build (the sanitizer build, oracle/_san/).
+_LIB_PATH = os.environ.get("ISDF_ORACLE_LIB") or os.path.join(_HERE, "liboracle_isdf.so")
 _lib = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
