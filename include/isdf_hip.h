@@ -132,6 +132,30 @@ int isdf_hip_solve(const isdf_problem* prob, const double* d_psi, double* d_thet
 int isdf_hip_solve_host(const isdf_problem* prob, const double* h_psi, double* h_theta,
                         int* h_piv, void* stream, isdf_report* rep, const isdf_debug* dbg);
 
+/* isdf_hip_solve without any device or host allocation.  No counterpart in
+ * LibISDF: it is for callers that cannot allocate (code running between
+ * allocation-free phases, tight serving loops, harnesses that forbid
+ * hipMalloc).  It runs the kernels of isdf_hip_solve, so Theta and piv equal
+ * that call's bit for bit, with these differences:
+ *   - the device workspace is a static array of 256 MiB that the library holds
+ *     on every GPU it runs on until the process ends; a problem whose workspace
+ *     exceeds it fails with ISDF_EINVAL (FP64 fits up to about Ng = 96^3 with
+ *     rk = 2048; the mixed forms need Ng x rk x 4-6 bytes more, so they fit
+ *     only small problems);
+ *   - every stage runs in order on `stream` (no helper stream, no events);
+ *   - piv is copied straight into h_piv at the end;
+ *   - K-means point selection only (prob->select = ISDF_SELECT_KMEANS), and the
+ *     npt < rk branch (fewer weighted grid points than rk) fails with
+ *     ISDF_EINVAL;
+ *   - rep (may be NULL) gets rk, rk_kmeans, kmeans_iters, npt, potrf_info and
+ *     eps; the stage times are 0.
+ * All calls on one GPU share the array and it keeps the last call's contents;
+ * calls are serialised inside the library and return synchronised, so calls
+ * that overlapped on one GPU (from another copy of the library in the same
+ * process, for instance) would race.  Synchronous. */
+int isdf_hip_solve_noalloc(const isdf_problem* prob, const double* d_psi, double* d_theta,
+                           int* h_piv, void* stream, isdf_report* rep);
+
 /* rk the solve will use for this problem (the reference's rank rule). */
 int isdf_hip_rank(const isdf_problem* prob);
 

@@ -19,7 +19,8 @@ from ._lib import (ISDF_EHIP, ISDF_EINVAL, ISDF_ENOMEM, ISDF_ENOTSPD, ISDF_OK,
                    ISDF_PREC_BF16X3, ISDF_PREC_FP32, ISDF_PREC_FP64, ISDF_PREC_REFINE,
                    ISDF_SELECT_KMEANS, ISDF_SELECT_QRCP)
 
-__all__ = ["Domain", "ISDF", "ISDF_host", "isdf_rank", "buffer_sizes", "solve", "solve_host",
+__all__ = ["Domain", "ISDF", "ISDF_host", "isdf_rank", "buffer_sizes", "solve", "solve_noalloc",
+           "solve_host",
            "solve_batched",
            "check", "check_device", "fit_error", "ISDF_SELECT_KMEANS", "ISDF_SELECT_QRCP", "ISDF_PREC_FP64", "ISDF_PREC_FP32",
            "ISDF_PREC_BF16X3", "ISDF_PREC_REFINE"]
@@ -166,6 +167,30 @@ def solve(d_psi, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_theta, p
                           piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                           ctypes.c_void_p(_stream_ptr(stream)), ctypes.byref(rep),
                           ctypes.byref(dbg) if dbg is not None else None)
+    if rc != ISDF_OK:
+        _raise(rc)
+    return rep.as_dict()
+
+
+def solve_noalloc(d_psi, domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, d_theta, piv,
+                  max_iter, stream=None, precision=ISDF_PREC_FP64, ld_psi=0):
+    """isdf_hip_solve_noalloc: the solve of solve() with no device or host
+    allocation (workspace in a static 256 MiB device array, every stage on
+    `stream`, K-means selection only, no debug outputs).  Theta and piv equal
+    solve()'s bit for bit.  Returns the report as a dict (stage times 0)."""
+    L = _lib.lib()
+    p = _problem(domain, nocc, nstate, nv1, nc1, nv2, nc2, mu_points, max_iter, precision, ld_psi)
+    ng = domain.NumGridTotal()
+    _host_piv(piv, ng)
+    rk = isdf_rank(nv1, nc1, nv2, nc2, mu_points)
+    n_psi, n_theta = buffer_sizes(ng, nstate, rk, ld_psi)
+    psi_ptr = _dev_ptr(d_psi, n_psi, "d_psi")
+    theta_ptr = _dev_ptr(d_theta, n_theta, "d_theta")
+    rep = _lib.IsdfReport()
+    rc = L.isdf_hip_solve_noalloc(ctypes.byref(p), ctypes.c_void_p(psi_ptr),
+                                  ctypes.c_void_p(theta_ptr),
+                                  piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                  ctypes.c_void_p(_stream_ptr(stream)), ctypes.byref(rep))
     if rc != ISDF_OK:
         _raise(rc)
     return rep.as_dict()

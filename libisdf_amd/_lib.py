@@ -19,6 +19,7 @@ ISDF_SELECT_KMEANS, ISDF_SELECT_QRCP = 0, 1
 EXPORTS = (
     "isdf_hip_solve",
     "isdf_hip_solve_host",
+    "isdf_hip_solve_noalloc",
     "isdf_hip_rank",
     "isdf_hip_last_error",
     "isdf_hip_version",
@@ -123,6 +124,10 @@ def lib():
     L.isdf_hip_solve.argtypes = [ctypes.POINTER(IsdfProblem), ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.POINTER(ctypes.c_int), ctypes.c_void_p,
                                  ctypes.POINTER(IsdfReport), ctypes.POINTER(IsdfDebug)]
+    L.isdf_hip_solve_noalloc.restype = ctypes.c_int
+    L.isdf_hip_solve_noalloc.argtypes = [ctypes.POINTER(IsdfProblem), ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.c_void_p, ctypes.POINTER(IsdfReport)]
     L.isdf_hip_solve_host.restype = ctypes.c_int
     L.isdf_hip_solve_host.argtypes = [ctypes.POINTER(IsdfProblem), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double),

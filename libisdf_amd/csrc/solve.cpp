@@ -144,10 +144,14 @@ struct Aux {
   int* hpiv = nullptr;       // pinned, hpiv_cap ints
   size_t hpiv_cap = 0;
   long long* hscal = nullptr;  // pinned: [0] iters, [1] rku, [2] info, [3] eps bits
+  // The no-allocation path: s is the caller's stream, no events, and hpiv /
+  // hscal point at the caller's piv and a host array (no pinned staging).
+  bool fixed = false;
 };
 std::vector<Aux> g_aux;
 
 int aux_pinned(Aux* a, size_t ng) {
+  if (a->fixed) return 0;
   if (!a->hscal)
     ISDF_HIP_TRY(hipHostMalloc((void**)&a->hscal, 8 * sizeof(long long), hipHostMallocDefault));
   if (a->hpiv_cap < ng) {
@@ -157,6 +161,51 @@ int aux_pinned(Aux* a, size_t ng) {
     ISDF_HIP_TRY(hipHostMalloc((void**)&a->hpiv, sizeof(int) * ng, hipHostMallocDefault));
     a->hpiv_cap = ng;
   }
+  return 0;
+}
+
+// Scratch of the no-allocation entry point (isdf_hip_solve_noalloc): a
+// file-scope device array that every GPU the library runs on holds until the
+// process ends.  In host code `scratch` only names the symbol; its address on
+// the current device comes from hipGetSymbolAddress, once per device, and only
+// that address is handed to kernels and copies.  Every call on one GPU shares
+// the array (calls are serialised by the solve mutex and return synchronised).
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+void* g_scratch[kMaxDevices] = {};
+
+int scratch_area(size_t bytes, void** out) {
+  if (bytes > kScratchBytes)
+    return set_error(ISDF_EINVAL, "the problem needs " + std::to_string(bytes) +
+                                      " bytes of device workspace; isdf_hip_solve_noalloc has " +
+                                      std::to_string(kScratchBytes) +
+                                      " (use isdf_hip_solve, which allocates)");
+  int dev = 0;
+  ISDF_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(ISDF_EINVAL, "device index out of range");
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    ISDF_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_error(ISDF_EHIP, "hipGetSymbolAddress returned no address for scratch");
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+
+// Fork / join of the helper stream; no-ops when the helper is the caller's
+// stream itself (the no-allocation path runs every stage in order on it).
+int fork_helper(Aux* a, hipStream_t st) {
+  if (!a->fork) return 0;
+  ISDF_HIP_TRY(hipEventRecord(a->fork, st));
+  ISDF_HIP_TRY(hipStreamWaitEvent(a->s, a->fork, 0));
+  return 0;
+}
+int join_helper(Aux* a, hipStream_t st) {
+  if (!a->join) return 0;
+  ISDF_HIP_TRY(hipEventRecord(a->join, a->s));
+  ISDF_HIP_TRY(hipStreamWaitEvent(st, a->join, 0));
   return 0;
 }
 
@@ -269,8 +318,7 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
                 const isdf_debug* dbg, Events& ev, Aux* aux, hipStream_t st) {
   const bool dual = (p->nv1 != p->nv2 || p->nc1 != p->nc2);
   hipStream_t s2 = aux->s;
-  ISDF_HIP_TRY(hipEventRecord(aux->fork, st));
-  ISDF_HIP_TRY(hipStreamWaitEvent(s2, aux->fork, 0));
+  if (int rc = fork_helper(aux, st)) return rc;
   const int zv = env_variant("ISDF_LP_ZCT_TILE");  // the format of the packed rows follows it
   const int tv = mixed_theta_variant(mode, refine, dual);
   const bool zsplit = lp_theta_zt_split(mode, tv);
@@ -278,10 +326,11 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
   // queued before the helper branch (rk/64 Cholesky steps of three launches
   // each, then the inverse): queued after it, Z C^T would start only once the
   // host had queued the whole branch.
-  ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->pack, 0));
+  if (aux->pack) ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->pack, 0));
   if (int rc = launch_zct_lp(mode, ng, rk, ldz, m.A1, m.A2, kp1, dual ? kp2 : kp1, m.B1, m.B2,
                              dual, m.Zt, zv, st,
-                             reserve_cus(true, ng, rk, kp1 + (dual ? kp2 : 0)), zsplit))
+                             aux->fixed ? 0 : reserve_cus(true, ng, rk, kp1 + (dual ? kp2 : 0)),
+                             zsplit))
     return rc;
   if (int rc = ev.rec(5, st)) return rc;
   if (int rc = ev.rec(6, s2)) return rc;
@@ -306,8 +355,7 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
                               s2))
     return rc;
   if (int rc = ev.rec(8, s2)) return rc;
-  ISDF_HIP_TRY(hipEventRecord(aux->join, s2));
-  ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->join, 0));
+  if (int rc = join_helper(aux, st)) return rc;
   if (int rc = ev.rec(9, st)) return rc;
   if (int rc = launch_theta_lp(mode, ng, rk, ldz, m.Wf, m.Zt, d_theta, ng, false, tv, st))
     return rc;
@@ -320,8 +368,11 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
   return 0;
 }
 
+// noalloc: the isdf_hip_solve_noalloc form -- workspace carved from the static
+// scratch array, every stage in order on `st` (no helper stream, no events),
+// piv and the scalars copied straight to the caller's host memory at the end.
 int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, int* h_piv,
-                 hipStream_t st, isdf_report* rep, const isdf_debug* dbg) {
+                 hipStream_t st, isdf_report* rep, const isdf_debug* dbg, bool noalloc = false) {
   long long ng64 = 0;
   if (int rc = validate(p, &ng64)) return rc;
   if (!d_psi || !d_theta || !h_piv) return set_error(ISDF_EINVAL, "NULL buffer argument");
@@ -394,8 +445,10 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
       off += align256(sizeof(double) * (size_t)rk * rk);
     }
   }
+  if (noalloc && qrcp)
+    return set_error(ISDF_EINVAL, "isdf_hip_solve_noalloc selects points by K-means only");
   void* wsp = nullptr;
-  if (int rc = workspace(off, &wsp)) return rc;
+  if (int rc = noalloc ? scratch_area(off, &wsp) : workspace(off, &wsp)) return rc;
   char* base = (char*)wsp;
   KmeansBufs kb;
   kmeans_carve(&kb, base + off_km, ng, p->ngrid, rk, max_iter);
@@ -410,7 +463,17 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   double* eps_dev = (double*)(base + off_sc + 64);
 
   Aux* aux = nullptr;
-  if (int rc = aux_stream(&aux)) return rc;
+  Aux own;
+  long long own_scal[8] = {};
+  if (noalloc) {
+    own.s = st;
+    own.fixed = true;
+    own.hpiv = h_piv;
+    own.hscal = own_scal;
+    aux = &own;
+  } else if (int rc = aux_stream(&aux)) {
+    return rc;
+  }
   MixedBufs mb{};
   if (mixed) {
     mb.A1 = base + off_A1;
@@ -425,16 +488,15 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   }
 
   Events ev;
-  if (int rc = ev.create(rep != nullptr)) return rc;
+  if (int rc = ev.create(rep != nullptr && !noalloc)) return rc;
   if (int rc = ev.rec(0, st)) return rc;
   if (mixed) {
     // Operand rows of psi on the helper stream, beside weights and K-means.
-    ISDF_HIP_TRY(hipEventRecord(aux->fork, st));
-    ISDF_HIP_TRY(hipStreamWaitEvent(aux->s, aux->fork, 0));
+    if (int rc = fork_helper(aux, st)) return rc;
     if (int rc = launch_pack_psi(opm, d_psi, ld, ng, s1, K1, kp1, mb.A1, aux->s)) return rc;
     if (dual)
       if (int rc = launch_pack_psi(opm, d_psi, ld, ng, s2, K2, kp2, mb.A2, aux->s)) return rc;
-    ISDF_HIP_TRY(hipEventRecord(aux->pack, aux->s));
+    if (aux->pack) ISDF_HIP_TRY(hipEventRecord(aux->pack, aux->s));
   }
 
   // piv zeroed first (src/common/isdf.cpp:174-175).
@@ -469,17 +531,26 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   if (int rc = aux_pinned(aux, (size_t)ng)) return rc;
   if (int rc = kmeans_prepare(&kb, p->ngrid, p->length, 1e-8, &npt, st)) return rc;
   degenerate = npt < rk;
+  if (degenerate && noalloc)
+    return set_error(ISDF_EINVAL,
+                     "fewer grid points above the density threshold than rk (npt " +
+                         std::to_string(npt) + " < rk " + std::to_string(rk) +
+                         "): isdf_hip_solve_noalloc does not take that branch; use isdf_hip_solve");
   if (!degenerate) {
     if (int rc = kmeans_setup_points(&kb, p->ngrid, p->length, npt, rk, st)) return rc;
     if (int rc = kmeans_iterate(&kb, rk, max_iter, 1e-5, nullptr, st)) return rc;
     if (int rc = ev.rec(2, st)) return rc;
     if (int rc = kmeans_finish(&kb, piv_dev, st, true)) return rc;
-    ISDF_HIP_TRY(hipMemcpyAsync(aux->hpiv, piv_dev, sizeof(int) * (size_t)ng,
-                                hipMemcpyDeviceToHost, st));
-    ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[0], kb.scal + KS_ITERS, sizeof(int),
-                                hipMemcpyDeviceToHost, st));
-    ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[1], kb.scal + KS_RKU, sizeof(int),
-                                hipMemcpyDeviceToHost, st));
+    if (!noalloc) {
+      // Into pinned staging: stays asynchronous (the no-allocation form copies
+      // at the end instead, since a copy into pageable memory blocks the host).
+      ISDF_HIP_TRY(hipMemcpyAsync(aux->hpiv, piv_dev, sizeof(int) * (size_t)ng,
+                                  hipMemcpyDeviceToHost, st));
+      ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[0], kb.scal + KS_ITERS, sizeof(int),
+                                  hipMemcpyDeviceToHost, st));
+      ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[1], kb.scal + KS_RKU, sizeof(int),
+                                  hipMemcpyDeviceToHost, st));
+    }
   } else {
     // Fewer points than rk above the density threshold: the reference fills piv
     // from the threshold list and shuffles the tail (utility.cpp:667-681).
@@ -544,15 +615,14 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
     // contraction of the pivot rows, P(mu, nu) = Z C^T(piv[mu], nu), then the
     // Cholesky (:455) and the blocked-solve operands.
     hipStream_t s2s = aux->s;
-    ISDF_HIP_TRY(hipEventRecord(aux->fork, st));
-    ISDF_HIP_TRY(hipStreamWaitEvent(s2s, aux->fork, 0));
+    if (int rc = fork_helper(aux, st)) return rc;
     // Main stream: Z C^T into d_theta (getbasis_scalapack.cpp:297-369).  It is
     // queued before the helper branch (rk/64 Cholesky steps of three launches
     // each): queued after it, Z C^T would start only once the host had queued
     // the whole branch.
     const int zv = zct_pick_variant(ng64, env_variant("ISDF_ZCT_TILE"));
     if (int rc = launch_zct(d_psi, ld, ng, psimuT, rk, s1, K1, s2, K2, dual, d_theta, ng, zv, st,
-                            reserve_cus(false, ng64, rk, K1 + (dual ? K2 : 0))))
+                            aux->fixed ? 0 : reserve_cus(false, ng64, rk, K1 + (dual ? K2 : 0))))
       return rc;
     if (int rc = ev.rec(5, st)) return rc;
     if (int rc = ev.rec(6, s2s)) return rc;
@@ -568,8 +638,7 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
     if (int rc = launch_potrf(P, rk, Dinv, info_dev, s2s)) return rc;
     if (int rc = launch_make_fg(P, rk, Dinv, Dinv128, Ft, Gt, s2s)) return rc;
     if (int rc = ev.rec(8, s2s)) return rc;
-    ISDF_HIP_TRY(hipEventRecord(aux->join, s2s));
-    ISDF_HIP_TRY(hipStreamWaitEvent(st, aux->join, 0));
+    if (int rc = join_helper(aux, st)) return rc;
     if (int rc = ev.rec(9, st)) return rc;
     // Two right-side triangular solves in place (getbasis_scalapack.cpp:469-478).
     if (int rc = launch_potrs(d_theta, ng64, ng64, rk, Ft, Gt, env_variant("ISDF_POTRS_TILE"),
@@ -578,6 +647,15 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
     if (int rc = ev.rec(10, st)) return rc;
   }
 
+  if (noalloc && !degenerate && !qrcp) {
+    // Straight into the caller's host piv and a host array (pageable copies).
+    ISDF_HIP_TRY(hipMemcpyAsync(h_piv, piv_dev, sizeof(int) * (size_t)ng, hipMemcpyDeviceToHost,
+                                st));
+    ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[0], kb.scal + KS_ITERS, sizeof(int),
+                                hipMemcpyDeviceToHost, st));
+    ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[1], kb.scal + KS_RKU, sizeof(int),
+                                hipMemcpyDeviceToHost, st));
+  }
   ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[2], info_dev, sizeof(int), hipMemcpyDeviceToHost, st));
   ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[3], eps_dev, sizeof(double), hipMemcpyDeviceToHost, st));
   ISDF_HIP_TRY(hipStreamSynchronize(st));
@@ -585,7 +663,7 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   double eps = 0.0;
   std::memcpy(&eps, &aux->hscal[3], sizeof(double));
   if (!degenerate) {
-    std::memcpy(h_piv, aux->hpiv, sizeof(int) * (size_t)ng);
+    if (aux->hpiv != h_piv) std::memcpy(h_piv, aux->hpiv, sizeof(int) * (size_t)ng);
     iters = (int)aux->hscal[0];
     rku = (int)aux->hscal[1];
   }
@@ -675,6 +753,16 @@ int isdf_hip_solve(const isdf_problem* prob, const double* d_psi, double* d_thet
         isdf_hip::g_aux[dev].s)
       (void)hipStreamSynchronize(isdf_hip::g_aux[dev].s);
   }
+  return rc;
+}
+
+int isdf_hip_solve_noalloc(const isdf_problem* prob, const double* d_psi, double* d_theta,
+                           int* h_piv, void* stream, isdf_report* rep) {
+  std::lock_guard<std::mutex> lk(isdf_hip::g_mutex);
+  isdf_hip::set_error(0, "");
+  const int rc = isdf_hip::solve_locked(prob, d_psi, d_theta, h_piv, (hipStream_t)stream, rep,
+                                        nullptr, true);
+  if (rc == ISDF_EHIP) (void)hipStreamSynchronize((hipStream_t)stream);
   return rc;
 }
 
