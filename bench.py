@@ -133,6 +133,13 @@ def peak_probe():
             res.append({"workgroups": blocks, "waves_per_simd": blocks // 256,
                         "tflops": tf.value, "ms": ms.value, "frac_of_peak": tf.value / peak})
         out[name] = {"published_peak_tflops": peak, "runs": res}
+    # Latency of back-to-back dependent FP64 adds (the floor of a strictly
+    # sequential sum such as the K-means centroid sums): 256 single-wave groups.
+    tf, ms = ctypes.c_double(), ctypes.c_double()
+    rc = lib.isdf_probe_peak(3, 256, 200000, ctypes.byref(tf), ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError("add-chain probe failed: hipError %d" % rc)
+    out["v_add_f64_dependent_chain"] = {"ns_per_add": 1e3 / (tf.value * 1e6), "ms": ms.value}
     print(json.dumps({"peak_probe": out}))
 
 

@@ -63,8 +63,14 @@ struct KmeansBufs {
   int* porder;       // [nblk_pts * 256] point of each brick slot (-1: none)
   int* changed_blk;  // [nblk_pts] labels changed per point block
   int* bbox;         // [6 * rkmax] cluster bounding boxes in grid indices
+  int nsb;           // super-bricks (2 x 2 x 2 bricks) of the grid
+  int* slist;        // [nsb * kSuperCap] candidate centroids of each super-brick, ascending
+  int* scount;       // [nsb] their number (above kSuperCap: the list overflowed)
   int* scal;         // small device scalars, see KS_* below
 };
+
+// Capacity of a super-brick candidate list (k_cull_super).
+constexpr int kSuperCap = 1024;
 
 enum {
   KS_RK = 0,       // current number of centroids

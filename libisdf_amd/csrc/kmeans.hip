@@ -371,6 +371,173 @@ __device__ double block_min256(double v, double* red) {
 // rk grid points, one corner of the grid) leave hundreds of candidates per
 // brick.  A brick with more falls back to testing every centroid.
 constexpr int kCandCap = 256, kCandCapWide = 1024;
+// Lloyd iterations whose centroid sums use 1024-thread workgroups (the
+// initial centroids, the first rk grid points, leave clusters of up to ~1e5
+// points for the first few updates).
+constexpr int kWideSumsIters = 4;
+
+// Box holding the K-means points of the grid-index ranges [xa, xb] x [ya, yb]
+// x [za, zb]: their lattice box widened by 1e-10 of the cell length, which
+// covers the few ulps by which the reference's coordinate formula
+// (utility.cpp:711-713) departs from the lattice.
+__device__ __forceinline__ void brick_box(int xa, int xb, int ya, int yb, int za, int zb, int n0,
+                                          int n1, int n2, double dx0, double dx1, double dx2,
+                                          double lo[3], double hi[3]) {
+  const double m0 = 1e-10 * (dx0 * n0), m1 = 1e-10 * (dx1 * n1), m2 = 1e-10 * (dx2 * n2);
+  lo[0] = xa * dx0 - m0;
+  hi[0] = xb * dx0 + m0;
+  lo[1] = ya * dx1 - m1;
+  hi[1] = yb * dx1 + m1;
+  lo[2] = za * dx2 - m2;
+  hi[2] = zb * dx2 + m2;
+}
+
+// Minimum of one double per thread over a block of NT threads.
+template <int NT>
+__device__ double block_min_nt(double v, double* red) {
+  v = wave_min(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double r = red[0];
+#pragma unroll
+  for (int w = 1; w < NT / 64; w++) r = fmin(r, red[w]);
+  __syncthreads();
+  return r;
+}
+
+// Lane-strided culling of the centroids listed by `cid` (nsrc of them, in
+// ascending index order) against the box [lo, hi]: the kept ones, in the
+// same order, go to out[0 .. CAP) (and their coordinates to ox / oy / oz when
+// given); returns their number, which may exceed CAP.  A centroid is kept
+// when its lower bound on the distance to the box is <= U (1 + 1e-9), U the
+// smallest upper bound over the listed centroids.  Block-wide (NT threads);
+// uses red[NT / 64] and wcnt[32][NT / 64] in LDS and ends with a barrier.
+template <int NT, int CAP, typename CID>
+__device__ int cull_box(const double* __restrict__ cx, const double* __restrict__ cy,
+                        const double* __restrict__ cz, int nsrc, CID cid, const double lo[3],
+                        const double hi[3], int* out, double* ox, double* oy, double* oz,
+                        double* red, int (*wcnt)[NT / 64]) {
+  constexpr int NW = NT / 64;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int nchunk = (nsrc + NT - 1) / NT;
+  // Chunks of NT in groups of four whose loads are all issued before use, so
+  // a pass waits on memory once per group rather than once per chunk.
+  constexpr int G = 4;
+  double ub = 1e300;
+  for (int j0 = 0; j0 < nchunk; j0 += G) {
+    double vx[G], vy[G], vz[G];
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      const int idx = (j0 + u) * NT + t;
+      const bool ok = j0 + u < nchunk && idx < nsrc;
+      const int c = ok ? cid(idx) : 0;
+      vx[u] = ok ? cx[c] : lo[0];
+      vy[u] = ok ? cy[c] : lo[1];
+      vz[u] = ok ? cz[c] : lo[2];
+    }
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      const int idx = (j0 + u) * NT + t;
+      if (j0 + u < nchunk && idx < nsrc) {
+        const double ex = fmax(fabs(vx[u] - lo[0]), fabs(vx[u] - hi[0]));
+        const double ey = fmax(fabs(vy[u] - lo[1]), fabs(vy[u] - hi[1]));
+        const double ez = fmax(fabs(vz[u] - lo[2]), fabs(vz[u] - hi[2]));
+        ub = fmin(ub, ex * ex + ey * ey + ez * ez);
+      }
+    }
+  }
+  const double thr = block_min_nt<NT>(ub, red) * (1.0 + 1e-9);
+  unsigned keepbits = 0u;  // bit j: list entry NT j + t is kept (nchunk <= 32)
+  for (int j0 = 0; j0 < nchunk; j0 += G) {
+    double vx[G], vy[G], vz[G];
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      const int idx = (j0 + u) * NT + t;
+      const bool ok = j0 + u < nchunk && idx < nsrc;
+      const int c = ok ? cid(idx) : 0;
+      vx[u] = ok ? cx[c] : lo[0];
+      vy[u] = ok ? cy[c] : lo[1];
+      vz[u] = ok ? cz[c] : lo[2];
+    }
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      const int j = j0 + u;
+      if (j < nchunk) {  // uniform: every lane takes part in the ballot
+        const int idx = j * NT + t;
+        bool keep = false;
+        if (idx < nsrc) {
+          const double ex = fmax(0.0, fmax(lo[0] - vx[u], vx[u] - hi[0]));
+          const double ey = fmax(0.0, fmax(lo[1] - vy[u], vy[u] - hi[1]));
+          const double ez = fmax(0.0, fmax(lo[2] - vz[u], vz[u] - hi[2]));
+          keep = ex * ex + ey * ey + ez * ez <= thr;
+        }
+        const unsigned long long m = __ballot(keep);
+        if (keep) keepbits |= 1u << j;
+        if (lane == 0) wcnt[j][wave] = __popcll(m);
+      }
+    }
+  }
+  __syncthreads();
+  // Kept entries in list order: chunk j before chunk j + 1, inside a chunk by
+  // wave, then lane.
+  int nc = 0;
+  for (int j = 0; j < nchunk; j++) {
+    const bool keep = (keepbits >> j) & 1u;
+    const unsigned long long m = __ballot(keep);
+    int before = 0, tot = 0;
+#pragma unroll
+    for (int w2 = 0; w2 < NW; w2++) {
+      const int v = wcnt[j][w2];
+      before += (w2 < wave) ? v : 0;
+      tot += v;
+    }
+    if (keep) {
+      const int pos = nc + before + __popcll(m & ((1ull << lane) - 1ull));
+      if (pos < CAP) {
+        const int c = cid(j * NT + t);
+        out[pos] = c;
+        if (ox) {
+          ox[pos] = cx[c];
+          oy[pos] = cy[c];
+          oz[pos] = cz[c];
+        }
+      }
+    }
+    nc += tot;
+  }
+  __syncthreads();
+  return nc;
+}
+
+// Candidate lists of the super-bricks (2 x 2 x 2 bricks, 16 x 16 x 8 grid
+// points) against every centroid, for k_assign_culled to cull its brick from.
+// A centroid dropped here is, for every point of the super-brick, farther
+// than the nearest centroid by a relative 1e-9, so it is none of its bricks'
+// candidates either.  Grid mode (every grid point a K-means point) only.
+__global__ __launch_bounds__(1024) void k_cull_super(const double* __restrict__ cx,
+                                                     const double* __restrict__ cy,
+                                                     const double* __restrict__ cz, int n0,
+                                                     int n1, int n2, double dx0, double dx1,
+                                                     double dx2, int* __restrict__ slist,
+                                                     int* __restrict__ scount, const int* scal) {
+  __shared__ double red[16];
+  __shared__ int wcnt[32][16];
+  if (scal[KS_ACTIVE] == 0) return;
+  const int rk = scal[KS_RK];
+  const int t = threadIdx.x;
+  const int nsbx = (n0 + 15) / 16, nsby = (n1 + 15) / 16;
+  const int sb = blockIdx.x;
+  const int sx0 = (sb % nsbx) * 16, sy0 = ((sb / nsbx) % nsby) * 16, sz0 = (sb / (nsbx * nsby)) * 8;
+  double lo[3], hi[3];
+  brick_box(sx0, min(sx0 + 15, n0 - 1), sy0, min(sy0 + 15, n1 - 1), sz0, min(sz0 + 7, n2 - 1), n0,
+            n1, n2, dx0, dx1, dx2, lo, hi);
+  int* out = slist + (long long)sb * kSuperCap;
+  const int nc = cull_box<1024, kSuperCap>(cx, cy, cz, rk, [](int idx) { return idx; }, lo, hi,
+                                           out, (double*)nullptr, (double*)nullptr,
+                                           (double*)nullptr, red, wcnt);
+  if (t == 0) scount[sb] = nc;
+}
 
 // Fused pdist2 + findMin(Dim=1) (utility.cpp:598-615, :535-562) with
 // conservative culling.  A workgroup takes one block of up to 256 points,
@@ -388,8 +555,9 @@ __global__ __launch_bounds__(256) void k_assign_culled(
     const int* __restrict__ porder, const double* __restrict__ X, const double* __restrict__ Y,
     const double* __restrict__ Z, const double* __restrict__ cx, const double* __restrict__ cy,
     const double* __restrict__ cz, int* __restrict__ label, int* scal, int count_changes,
-    int* __restrict__ bbox, int rkmax, int grid_mode, int n0, int n1,
-    int* __restrict__ changed_blk, const KmJob* jobs) {
+    int* __restrict__ bbox, int rkmax, int grid_mode, int n0, int n1, int n2, double dx0,
+    double dx1, double dx2, int* __restrict__ changed_blk, const int* __restrict__ slist,
+    const int* __restrict__ scount, const KmJob* jobs) {
   __shared__ int cand[CAP];
   if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
@@ -408,7 +576,12 @@ __global__ __launch_bounds__(256) void k_assign_culled(
     grid_mode = 1;
     n0 = J.b.ngrid[0];
     n1 = J.b.ngrid[1];
+    n2 = J.b.ngrid[2];
+    dx0 = J.b.dx[0];
+    dx1 = J.b.dx[1];
+    dx2 = J.b.dx[2];
     changed_blk = J.b.changed_blk;
+    slist = nullptr;
   }
   __shared__ double ccx[CAP], ccy[CAP], ccz[CAP];
   __shared__ int cbox[6][CAP];
@@ -421,8 +594,19 @@ __global__ __launch_bounds__(256) void k_assign_culled(
   const bool valid = i >= 0;
   const double px = valid ? X[i] : 0.0, py = valid ? Y[i] : 0.0, pz = valid ? Z[i] : 0.0;
 
-  // Bounding box of the block's points.
-  {
+  double lo[3], hi[3];
+  if (grid_mode) {
+    // The box of the brick from its grid-index ranges: the coordinate formula
+    // (utility.cpp:711-713) puts every point within a few ulps of its lattice
+    // position, so the lattice box widened by 1e-10 L holds every point (a
+    // wider box only keeps more candidates).  A brick always holds a point.
+    const int nbx = (n0 + 7) / 8, nby = (n1 + 7) / 8;
+    const int b = blockIdx.x;
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    brick_box(bx * 8, min(bx * 8 + 7, n0 - 1), by * 8, min(by * 8 + 7, n1 - 1), bz * 4,
+              min(bz * 4 + 3, n2 - 1), n0, n1, n2, dx0, dx1, dx2, lo, hi);
+  } else {
+    // Bounding box of the block's points.
     double v[6] = {valid ? px : 1e300,  valid ? py : 1e300,  valid ? pz : 1e300,
                    valid ? -px : 1e300, valid ? -py : 1e300, valid ? -pz : 1e300};
 #pragma unroll
@@ -430,57 +614,38 @@ __global__ __launch_bounds__(256) void k_assign_culled(
       const double m = wave_min(v[k]);
       if (lane == 0) rbox[k][wave] = m;
     }
-  }
-  __syncthreads();
-  double lo[3], hi[3];
+    __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    lo[k] = fmin(fmin(rbox[k][0], rbox[k][1]), fmin(rbox[k][2], rbox[k][3]));
-    hi[k] = -fmin(fmin(rbox[k + 3][0], rbox[k + 3][1]), fmin(rbox[k + 3][2], rbox[k + 3][3]));
-  }
-  if (lo[0] > 1e299) {  // no point in this block (uniform)
-    if (count_changes && t == 0) changed_blk[blockIdx.x] = 0;
-    return;
+    for (int k = 0; k < 3; k++) {
+      lo[k] = fmin(fmin(rbox[k][0], rbox[k][1]), fmin(rbox[k][2], rbox[k][3]));
+      hi[k] = -fmin(fmin(rbox[k + 3][0], rbox[k + 3][1]), fmin(rbox[k + 3][2], rbox[k + 3][3]));
+    }
+    if (lo[0] > 1e299) {  // no point in this block (uniform)
+      if (count_changes && t == 0) changed_blk[blockIdx.x] = 0;
+      return;
+    }
   }
 
-  // Thread t bounds the contiguous centroid range [t*q, t*q + q): one pass for
-  // the smallest upper bound, one for the lower-bound test, one scan, so the
-  // candidate list comes out in ascending centroid order.
-  const int q = (rk + 255) / 256;
-  const int c0 = min(rk, t * q), c1 = min(rk, c0 + q);
-  double ub = 1e300;
-  for (int c = c0; c < c1; c++) {
-    const double ex = fmax(fabs(cx[c] - lo[0]), fabs(cx[c] - hi[0]));
-    const double ey = fmax(fabs(cy[c] - lo[1]), fabs(cy[c] - hi[1]));
-    const double ez = fmax(fabs(cz[c] - lo[2]), fabs(cz[c] - hi[2]));
-    ub = fmin(ub, ex * ex + ey * ey + ez * ez);
-  }
-  const double thr = block_min256(ub, red) * (1.0 + 1e-9);
-  unsigned long long keepmask = 0ull;  // q <= 32 (rk <= 8192)
-  int mycnt = 0;
-  for (int c = c0; c < c1; c++) {
-    const double ex = fmax(0.0, fmax(lo[0] - cx[c], cx[c] - hi[0]));
-    const double ey = fmax(0.0, fmax(lo[1] - cy[c], cy[c] - hi[1]));
-    const double ez = fmax(0.0, fmax(lo[2] - cz[c], cz[c] - hi[2]));
-    if (ex * ex + ey * ey + ez * ez <= thr) {
-      keepmask |= 1ull << (c - c0);
-      mycnt++;
+  // The centroids to cull: all of them, or (one instance, grid mode) the list
+  // of the brick's super-brick (2 x 2 x 2 bricks), when it did not overflow.
+  const int* sl = nullptr;
+  int nsrc = rk;
+  if (slist) {
+    const int nbx = (n0 + 7) / 8, nby = (n1 + 7) / 8;
+    const int b = blockIdx.x;
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int nsbx = (n0 + 15) / 16, nsby = (n1 + 15) / 16;
+    const int sb = (bx >> 1) + nsbx * ((by >> 1) + nsby * (bz >> 1));
+    const int ns = scount[sb];
+    if (ns <= kSuperCap) {
+      sl = slist + (long long)sb * kSuperCap;
+      nsrc = ns;
     }
   }
-  int nc;
-  {
-    int pos = block_excl_scan(mycnt, &nc);
-    for (int c = c0; c < c1; c++) {
-      if (!((keepmask >> (c - c0)) & 1ull)) continue;
-      if (pos < CAP) {
-        cand[pos] = c;
-        ccx[pos] = cx[c];
-        ccy[pos] = cy[c];
-        ccz[pos] = cz[c];
-      }
-      pos++;
-    }
-  }
+  __shared__ int wcnt[32][4];
+  const int nc = cull_box<256, CAP>(
+      cx, cy, cz, nsrc, [sl](int idx) { return sl ? sl[idx] : idx; }, lo, hi, cand, ccx, ccy,
+      ccz, red, wcnt);
   const bool culled = nc <= CAP;
   if (culled && grid_mode) {
     for (int k = t; k < nc; k += 256) {
@@ -533,25 +698,42 @@ __global__ __launch_bounds__(256) void k_assign_culled(
   }
   if (count_changes) {
     // Per-block count; the loop control sums the blocks (no contended atomics).
-    int tot;
-    block_excl_scan(changed ? 1 : 0, &tot);
-    if (t == 0) changed_blk[blockIdx.x] = tot;
+    const unsigned long long m = __ballot(changed);
+    if (lane == 0) wcnt[0][wave] = __popcll(m);
+    __syncthreads();
+    if (t == 0) changed_blk[blockIdx.x] = wcnt[0][0] + wcnt[0][1] + wcnt[0][2] + wcnt[0][3];
   }
   if (!grid_mode) return;
-  int gx = 0, gy = 0, gz = 0;
-  if (valid) {
-    gx = i % n0;
-    gy = (i / n0) % n1;
-    gz = i / (n0 * n1);
-  }
   if (culled) {
-    if (valid) {
-      atomicMin(&cbox[0][bk], gx);
-      atomicMin(&cbox[1][bk], gy);
-      atomicMin(&cbox[2][bk], gz);
-      atomicMax(&cbox[3][bk], gx);
-      atomicMax(&cbox[4][bk], gy);
-      atomicMax(&cbox[5][bk], gz);
+    // Boxes of the clusters, per wave: lane l holds brick point (x, y) =
+    // (l & 7, l >> 3) of the wave's z plane (k_build_blocks), so the ballot of
+    // one label's lanes gives its x and y extent by bit folding (uniform,
+    // scalar work) and one lane folds it into the block's box of that
+    // candidate: no contended LDS atomics.
+    const int nbx = (n0 + 7) / 8, nby = (n1 + 7) / 8;
+    const int b = blockIdx.x;
+    const int bx0 = (b % nbx) * 8, by0 = ((b / nbx) % nby) * 8, zw = (b / (nbx * nby)) * 4 + wave;
+    unsigned long long todo = __ballot(valid);
+    while (todo) {
+      const int leader = __ffsll((long long)todo) - 1;
+      const int kk = __shfl(bk, leader);
+      const unsigned long long m = __ballot(valid && bk == kk);
+      todo &= ~m;
+      unsigned cols = 0u, rows = 0u;
+#pragma unroll
+      for (int yy = 0; yy < 8; yy++) {
+        const unsigned r8 = (unsigned)(m >> (8 * yy)) & 0xffu;
+        cols |= r8;
+        rows |= (r8 ? 1u : 0u) << yy;
+      }
+      if (lane == leader) {
+        atomicMin(&cbox[0][kk], bx0 + __ffs(cols) - 1);
+        atomicMin(&cbox[1][kk], by0 + __ffs(rows) - 1);
+        atomicMin(&cbox[2][kk], zw);
+        atomicMax(&cbox[3][kk], bx0 + 31 - __clz(cols));
+        atomicMax(&cbox[4][kk], by0 + 31 - __clz(rows));
+        atomicMax(&cbox[5][kk], zw);
+      }
     }
     __syncthreads();
     for (int k = t; k < nc; k += 256) {
@@ -565,6 +747,7 @@ __global__ __launch_bounds__(256) void k_assign_culled(
       atomicMax(&bbox[5 * rkmax + c], cbox[5][k]);
     }
   } else if (valid) {
+    const int gx = i % n0, gy = (i / n0) % n1, gz = i / (n0 * n1);
     atomicMin(&bbox[bi], gx);
     atomicMin(&bbox[rkmax + bi], gy);
     atomicMin(&bbox[2 * rkmax + bi], gz);
@@ -590,9 +773,10 @@ __global__ void k_reset_boxes(int* bbox, int rkmax, const KmJob* jobs) {
 // Thread t tests positions 4t..4t+3 of the round, members are compacted in
 // position order through a block scan, and their products w*x (one rounding
 // each, as in the reference) land in LDS; threads 0..3 then carry the four
-// sequential sums over the round's members in that order.
-template <bool BATCH>
-__global__ __launch_bounds__(256) void k_centroid_sums_box(
+// sequential sums over the round's members in that order, while the labels
+// and terms of the next rounds load.
+template <bool BATCH, int NT = 256>
+__global__ __launch_bounds__(NT) void k_centroid_sums_box(
     int n0, int n1, const int* __restrict__ label, const double* __restrict__ WL,
     const double* __restrict__ X, const double* __restrict__ Y, const double* __restrict__ Z,
     const int* __restrict__ bbox, int rkmax, double* __restrict__ sw, double* __restrict__ sx,
@@ -602,8 +786,14 @@ __global__ __launch_bounds__(256) void k_centroid_sums_box(
   // workgroups fit a CU (the sequential sums wait on memory and on the
   // previous add; more resident workgroups hide it): 24 uneven instances
   // 1468 -> 1506 solves/s.  One instance (rk up to a few per CU) keeps 1024.
-  constexpr int CH = BATCH ? 512 : 1024, PT = CH / 256;
-  __shared__ double buf[4][CH];
+  // NT = 1024 (4096 positions per pass, 128 KB of LDS) serves the first Lloyd
+  // iterations, whose clusters still hold tens of thousands of points: their
+  // boxes need a quarter of the passes, and each pass costs a scan and two
+  // barriers whatever its size.
+  constexpr int CH = BATCH ? 512 : 4 * NT, PT = CH / NT;
+  // Rows padded by 32 bytes: the four adder lanes read four rows at once, which
+  // otherwise start in the same LDS bank.
+  __shared__ __align__(16) double buf[4][CH + 4];
   if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     n0 = J.b.ngrid[0];
@@ -631,56 +821,132 @@ __global__ __launch_bounds__(256) void k_centroid_sums_box(
   double acc = 0.0;
   if (x1 >= x0) {
     const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
-    const long long plane = (long long)bw * bh;
-    const long long total = plane * (z1 - z0 + 1);
-    for (long long j0 = 0; j0 < total; j0 += CH) {
-      int g[PT];
-      bool mem[PT];
-      int cnt = 0;
+    const int plane = bw * bh;  // box positions < Ng < 2^31
+    const int total = plane * (z1 - z0 + 1);
+    const int nround = (total + CH - 1) / CH;
+    // Labels of round r: thread t takes box positions r CH + PT t .. + PT - 1
+    // in grid order ((x, y, z) of the first from one division pair, then
+    // stepping); g = -1 past the box.  The loads stay in flight until used.
+    auto load_labels = [&](int r, int* g, int* l) {
+      const int j = r * CH + PT * t;
+      int z = j / plane, rr = j - z * plane;
+      int y = rr / bw, x = rr - y * bw;
 #pragma unroll
       for (int q = 0; q < PT; q++) {
-        const long long j = j0 + PT * t + q;
-        mem[q] = false;
-        g[q] = 0;
-        if (j < total) {
-          const int z = z0 + (int)(j / plane);
-          const long long r = j % plane;
-          const int y = y0 + (int)(r / bw);
-          const int x = x0 + (int)(r % bw);
-          g[q] = x + n0 * (y + n1 * z);
-          mem[q] = label[g[q]] == c;
-          cnt += mem[q] ? 1 : 0;
+        g[q] = -1;
+        l[q] = -1;
+        if (j + q < total) {
+          g[q] = (x0 + x) + n0 * ((y0 + y) + n1 * (z0 + z));
+          l[q] = label[g[q]];
+        }
+        if (++x == bw) {
+          x = 0;
+          if (++y == bh) {
+            y = 0;
+            ++z;
+          }
         }
       }
-      int tot;
-      int pos = block_excl_scan(cnt, &tot);
+    };
+    // Members of a round and the loads of their terms (w, x, y, z).
+    double tw[PT], tx[PT], ty[PT], tz[PT];
+    bool mem[PT];
+    auto gather = [&](const int* g, const int* l) {
+      int n = 0;
+#pragma unroll
+      for (int q = 0; q < PT; q++) {
+        mem[q] = l[q] == c;
+        if (mem[q]) {
+          tw[q] = WL[g[q]];
+          tx[q] = X[g[q]];
+          ty[q] = Y[g[q]];
+          tz[q] = Z[g[q]];
+          n++;
+        }
+      }
+      return n;
+    };
+    // Members to LDS in position order; products w*x (one rounding each, as
+    // the reference forms them).
+    auto store = [&](int pos) {
 #pragma unroll
       for (int q = 0; q < PT; q++) {
         if (mem[q]) {
-          const int gg = g[q];
-          const double wv = WL[gg];
-          buf[0][pos] = wv;
-          buf[1][pos] = wv * X[gg];
-          buf[2][pos] = wv * Y[gg];
-          buf[3][pos] = wv * Z[gg];
+          buf[0][pos] = tw[q];
+          buf[1][pos] = tw[q] * tx[q];
+          buf[2][pos] = tw[q] * ty[q];
+          buf[3][pos] = tw[q] * tz[q];
           pos++;
         }
       }
-      __syncthreads();
-      if (t < 4) {
-        // Eight terms fetched ahead of the adds, which stay in sequence.
-        const double* qv = buf[t];
-        int e = 0;
-        for (; e + 7 < tot; e += 8) {
-          double v[8];
+    };
+    // Software pipeline over rounds: labels two rounds ahead, member terms one
+    // round ahead, so threads 0..3 add round r while the loads of rounds r+1
+    // and r+2 are in flight.  The scan of round r+1 waits for those adds (its
+    // barrier) before round r+1 overwrites the buffer.
+    int gn[PT], ln[PT], gf[PT], lf[PT];  // round r+1 (next), round r+2 (far)
+    load_labels(0, gn, ln);
+    if (nround > 1) load_labels(1, gf, lf);
+    int tot;
+    store(block_excl_scan(gather(gn, ln), &tot));
+    __syncthreads();
+    for (int r = 0; r < nround; r++) {
+      const int tot_r = tot;
 #pragma unroll
-          for (int u = 0; u < 8; u++) v[u] = qv[e + u];
-#pragma unroll
-          for (int u = 0; u < 8; u++) acc = acc + v[u];
-        }
-        for (; e < tot; e++) acc = acc + qv[e];
+      for (int q = 0; q < PT; q++) {
+        gn[q] = gf[q];
+        ln[q] = lf[q];
       }
-      __syncthreads();
+      if (r + 2 < nround) load_labels(r + 2, gf, lf);
+      const int cnt = (r + 1 < nround) ? gather(gn, ln) : 0;
+      if (t < 4) {
+        // The sum stays strictly sequential; only the LDS reads run ahead:
+        // 16 terms in flight as 8 two-term reads.
+        const double* qv = buf[t];
+        const int nfull = tot_r & ~15;
+        int e = 0;
+        if (nfull) {
+          // Ping-pong between two register sets; the read of the next 16 terms
+          // is issued before the adds of the current 16 (its index clamped
+          // inside the buffer: past nfull it is read and never added).
+          double2 a[8], b[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) a[u] = *(const double2*)(qv + 2 * u);
+          while (true) {
+            {
+              const int en = min(e + 16, CH - 16);
+#pragma unroll
+              for (int u = 0; u < 8; u++) b[u] = *(const double2*)(qv + en + 2 * u);
+              __asm__ volatile("" ::: "memory");  // keep the reads ahead of the adds
+#pragma unroll
+              for (int u = 0; u < 8; u++) {
+                acc = acc + a[u].x;
+                acc = acc + a[u].y;
+              }
+              e += 16;
+              if (e >= nfull) break;
+            }
+            {
+              const int en = min(e + 16, CH - 16);
+#pragma unroll
+              for (int u = 0; u < 8; u++) a[u] = *(const double2*)(qv + en + 2 * u);
+              __asm__ volatile("" ::: "memory");  // keep the reads ahead of the adds
+#pragma unroll
+              for (int u = 0; u < 8; u++) {
+                acc = acc + b[u].x;
+                acc = acc + b[u].y;
+              }
+              e += 16;
+              if (e >= nfull) break;
+            }
+          }
+        }
+        for (; e < tot_r; e++) acc = acc + qv[e];
+      }
+      if (r + 1 < nround) {
+        store(block_excl_scan(cnt, &tot));
+        __syncthreads();
+      }
     }
   }
   if (t == 0) sw[c] = acc;
@@ -1267,6 +1533,9 @@ void kmeans_carve(KmeansBufs* b, void* basep, int n, const int ngrid[3], int rkm
     b->changed_blk = (int*)take(sizeof(int) * (size_t)blocks);
   }
   b->bbox = (int*)take(sizeof(int) * 6 * (size_t)rkmax);
+  b->nsb = ((ngrid[0] + 15) / 16) * ((ngrid[1] + 15) / 16) * ((ngrid[2] + 7) / 8);
+  b->slist = (int*)take(sizeof(int) * (size_t)b->nsb * kSuperCap);
+  b->scount = (int*)take(sizeof(int) * (size_t)b->nsb);
   b->scal = (int*)take(sizeof(int) * KS_COUNT);
 }
 
@@ -1349,7 +1618,9 @@ int kmeans_setup_points(KmeansBufs* b, const int ngrid[3], const double length[3
   hipLaunchKernelGGL((k_assign_culled<false, kCandCapWide>), dim3(b->nblk_pts), dim3(256), 0, st,
                      b->porder, b->X, b->Y,
                      b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 0, b->bbox, b->rkmax,
-                     b->grid_mode ? 1 : 0, ngrid[0], ngrid[1], b->changed_blk, (const KmJob*)nullptr);
+                     b->grid_mode ? 1 : 0, ngrid[0], ngrid[1], ngrid[2], b->dx[0], b->dx[1], b->dx[2],
+                     b->changed_blk, (const int*)nullptr,
+                     (const int*)nullptr, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_assign_culled");
   return 0;
 }
@@ -1374,7 +1645,11 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
   // centroid update of iteration s (one launch), then the assignment.
   int last = 0;
   for (int s = 0; s < max_iter; s++) {
-    if (b->grid_mode) {
+    if (b->grid_mode && s < kWideSumsIters) {
+      hipLaunchKernelGGL((k_centroid_sums_box<false, 1024>), dim3(rk), dim3(1024), 0, st,
+                         b->ngrid[0], b->ngrid[1], b->label, b->WL, b->X, b->Y, b->Z, b->bbox,
+                         b->rkmax, b->sw, b->sx, b->sy, b->sz, b->scal, (const KmJob*)nullptr);
+    } else if (b->grid_mode) {
       hipLaunchKernelGGL(k_centroid_sums_box<false>, dim3(rk), dim3(256), 0, st, b->ngrid[0],
                          b->ngrid[1], b->label, b->WL, b->X, b->Y, b->Z, b->bbox, b->rkmax, b->sw,
                          b->sx, b->sy, b->sz, b->scal, (const KmJob*)nullptr);
@@ -1401,14 +1676,30 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
                          b->cy, b->cz, b->map, b->scal, b->grid_mode ? b->bbox : (int*)nullptr,
                          b->rkmax);
     last = s;
+    // Grid mode: the super-brick candidate lists first (the bricks cull from
+    // them).  The first iterations keep the wide candidate list; their
+    // centroids still crowd the corner of the initial ones.
+    const int* sl = nullptr;
+    const int* sc = nullptr;
+    if (b->grid_mode && s >= 3) {
+      hipLaunchKernelGGL(k_cull_super, dim3(b->nsb), dim3(1024), 0, st, b->cx, b->cy, b->cz,
+                         b->ngrid[0], b->ngrid[1], b->ngrid[2], b->dx[0], b->dx[1], b->dx[2],
+                         b->slist, b->scount, b->scal);
+      sl = b->slist;
+      sc = b->scount;
+    }
     if (s < 3)
       hipLaunchKernelGGL((k_assign_culled<false, kCandCapWide>), dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X,
                          b->Y, b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 1, b->bbox, b->rkmax,
-                         b->grid_mode ? 1 : 0, b->ngrid[0], b->ngrid[1], b->changed_blk, (const KmJob*)nullptr);
+                         b->grid_mode ? 1 : 0, b->ngrid[0], b->ngrid[1], b->ngrid[2], b->dx[0], b->dx[1],
+                         b->dx[2], b->changed_blk, sl, sc,
+                         (const KmJob*)nullptr);
     else
       hipLaunchKernelGGL((k_assign_culled<false, kCandCap>), dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X,
                          b->Y, b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 1, b->bbox, b->rkmax,
-                         b->grid_mode ? 1 : 0, b->ngrid[0], b->ngrid[1], b->changed_blk, (const KmJob*)nullptr);
+                         b->grid_mode ? 1 : 0, b->ngrid[0], b->ngrid[1], b->ngrid[2], b->dx[0], b->dx[1],
+                         b->dx[2], b->changed_blk, sl, sc,
+                         (const KmJob*)nullptr);
     ISDF_HIP_LAUNCHED("k-means iteration");
     if (poll && (s + 1) % check_every == 0 && s + 1 < max_iter) {
       int act = 1;
@@ -1507,7 +1798,8 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
                      (const int*)nullptr,
                      (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                      (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
-                     (int*)nullptr, (int*)nullptr, 0, (int*)nullptr, 0, 1, 0, 0, (int*)nullptr, J);
+                     (int*)nullptr, (int*)nullptr, 0, (int*)nullptr, 0, 1, 0, 0, 0, 0.0, 0.0, 0.0, (int*)nullptr,
+                     (const int*)nullptr, (const int*)nullptr, J);
   hipLaunchKernelGGL(k_km_batch_active, dim3(1, nb), dim3(64), 0, st, J);
   ISDF_HIP_LAUNCHED("k-means setup (batched)");
   // Lloyd iterations (utility.cpp:812-922); converged instances return at once.
@@ -1525,14 +1817,14 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
       hipLaunchKernelGGL((k_assign_culled<true, kCandCapWide>), dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
-                         (int*)nullptr, (int*)nullptr, 1, (int*)nullptr, 0, 1, 0, 0, (int*)nullptr,
-                         J);
+                         (int*)nullptr, (int*)nullptr, 1, (int*)nullptr, 0, 1, 0, 0, 0, 0.0, 0.0, 0.0, (int*)nullptr,
+                         (const int*)nullptr, (const int*)nullptr, J);
     else
       hipLaunchKernelGGL((k_assign_culled<true, kCandCap>), dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
-                         (int*)nullptr, (int*)nullptr, 1, (int*)nullptr, 0, 1, 0, 0, (int*)nullptr,
-                         J);
+                         (int*)nullptr, (int*)nullptr, 1, (int*)nullptr, 0, 1, 0, 0, 0, 0.0, 0.0, 0.0, (int*)nullptr,
+                         (const int*)nullptr, (const int*)nullptr, J);
     hipLaunchKernelGGL(k_control<true>, dim3(1, nb), dim3(1024), 0, st, s, 0, 0.0, 0,
                        (const int*)nullptr, (int*)nullptr, (int*)nullptr, J);
     ISDF_HIP_LAUNCHED("k-means iteration (batched)");

@@ -57,11 +57,24 @@ __global__ __launch_bounds__(256) void k_fma_f64(double a0, double b0, int iters
   out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// One dependent FP64 add chain per lane, terms from registers: the latency of
+// v_add_f64 back to back (the bound of any strictly sequential FP64 sum).
+__global__ __launch_bounds__(64) void k_add_chain(double a0, int iters, double* out) {
+  double acc = threadIdx.x * 1e-3;
+  const double a = a0 + threadIdx.x * 1e-12, b = a0 * 0.5;
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int u = 0; u < 16; u++) acc = acc + ((u & 1) ? a : b);
+  }
+  out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
 }  // namespace
 
 extern "C" {
 
-// which: 0 v_mfma_f64_16x16x4f64, 1 v_fma_f64, 2 v_mfma_f32_16x16x4f32.
+// which: 0 v_mfma_f64_16x16x4f64, 1 v_fma_f64, 2 v_mfma_f32_16x16x4f32,
+// 3 dependent v_add_f64 chain (rate of one lane, in 1e12 adds/s).
 // Launches `blocks` workgroups of 256 threads (4 waves), each running `iters`
 // rounds, twice (the first run warms the clocks up), and returns the second
 // run's rate in TFLOP/s through *tflops and its time through *ms.
@@ -83,9 +96,14 @@ int isdf_probe_peak(int which, int blocks, int iters, double* tflops, double* ms
     } else if (which == 1) {
       k_fma_f64<<<blocks, 256>>>(0.999, 1e-3, iters, out);
       flops_per_thread_iter = 2.0 * 16;
-    } else {
+    } else if (which == 2) {
       k_mfma_f32<<<blocks, 256>>>(1.0f, 0.5f, iters, out);
       flops_per_thread_iter = 32.0 * kChains;
+    } else {
+      // One wave per workgroup, `blocks` of them; *tflops then reports the
+      // dependent adds per second of one lane (1e12 units), *ms the time.
+      k_add_chain<<<blocks, 64>>>(1.0, iters, out);
+      flops_per_thread_iter = 16.0;
     }
     (void)hipEventRecord(t1, nullptr);
   }
@@ -98,7 +116,8 @@ int isdf_probe_peak(int which, int blocks, int iters, double* tflops, double* ms
   (void)hipFree(out);
   if (e != hipSuccess) return (int)e;
   *ms = t;
-  *tflops = flops_per_thread_iter * (double)iters * blocks * 256 / (t * 1e-3) / 1e12;
+  *tflops = which == 3 ? flops_per_thread_iter * (double)iters / (t * 1e-3) / 1e12
+                      : flops_per_thread_iter * (double)iters * blocks * 256 / (t * 1e-3) / 1e12;
   return 0;
 }
 
