@@ -133,6 +133,17 @@ def peak_probe():
             res.append({"workgroups": blocks, "waves_per_simd": blocks // 256,
                         "tflops": tf.value, "ms": ms.value, "frac_of_peak": tf.value / peak})
         out[name] = {"published_peak_tflops": peak, "runs": res}
+    # Both FP64 pipes at once: half the waves on MFMA, half on vector FMA.
+    res = []
+    for blocks in (256 * 4, 256 * 8):
+        tf, ms = ctypes.c_double(), ctypes.c_double()
+        rc = lib.isdf_probe_peak(4, blocks, 2000, ctypes.byref(tf), ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError("mixed-pipe probe failed: hipError %d" % rc)
+        res.append({"workgroups": blocks, "tflops": tf.value, "ms": ms.value,
+                    "frac_of_peak": tf.value / FP64_PEAK_TFLOPS})
+    out["fp64_mfma_and_fma_waves_together"] = {"published_peak_tflops": FP64_PEAK_TFLOPS,
+                                               "runs": res}
     # Latency of back-to-back dependent FP64 adds (the floor of a strictly
     # sequential sum such as the K-means centroid sums): 256 single-wave groups.
     tf, ms = ctypes.c_double(), ctypes.c_double()

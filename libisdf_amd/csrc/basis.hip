@@ -501,7 +501,6 @@ __global__ __launch_bounds__(256) void k_potrf_syrk(double* __restrict__ A, int 
 // owns a strip of rows and walks the block columns in order.  Block inverses of
 // 128 x 128 come from the Cholesky's 64 x 64 ones by the 2 x 2 block formula.
 // ---------------------------------------------------------------------------
-constexpr int kSB = 128;
 
 // Dinv128[J] = inv(L_JJ) for the 128-blocks: [[Da, 0], [-Db L_ba Da, Db]].
 // Both 64 x 64 products are register-blocked, 4 x 4 outputs per thread, with
@@ -775,6 +774,8 @@ typedef MfmaTile<128, kSB, 16, 2, 2> PotrsTile2;  // 4 waves of 64 x 64
 typedef ValuTile<256, kSB, 16, 1, 8> PotrsTile3;  // vector FMA, 8 waves of 256 x 16
 typedef MfmaTile<32, kSB, 16, 2, 2> PotrsTile4;   // 4 waves of 16 x 64: few rows, many workgroups
 typedef MfmaTile<16, kSB, 16, 1, 4> PotrsTile5;   // 4 waves of 16 x 32
+typedef ScalarBTile<256, kSB, 8> PotrsTile7;  // vector FMA, B in scalar registers, no LDS
+typedef ScalarBTile<128, kSB, 8> PotrsTile8;  // the same on 128-row strips
 
 }  // namespace
 
@@ -996,7 +997,7 @@ int launch_linv_transposed(const double* L, int n, const double* Dinv128, double
   return 0;
 }
 
-int potrs_variants() { return 7; }
+int potrs_variants() { return 9; }
 
 int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* Ft, const double* Gt,
                  int variant, hipStream_t st, bool unit_lower) {
@@ -1021,6 +1022,8 @@ int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* F
                          dim3(PotrsTile0::NT), 0, st, T, ng, ldt, rk, Ft, Gt, (int)unit_lower,
                          (const CholJob*)nullptr);
       break;
+    ISDF_POTRS_CASE(7)
+    ISDF_POTRS_CASE(8)
 #undef ISDF_POTRS_CASE
     default:
       return set_error(1, "unknown solve tile variant");

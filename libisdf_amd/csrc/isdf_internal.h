@@ -157,7 +157,8 @@ int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* F
 // diagonal (one block).
 int launch_trace_eps(double* P, int rk, double* eps_dev, hipStream_t st);
 
-constexpr int kNB = 64;  // block size of the Cholesky (the solves use 128-blocks)
+constexpr int kNB = 64;   // block size of the Cholesky
+constexpr int kSB = 128;  // block-column width of the triangular solves
 
 // One instance of a batched Cholesky / solve-operand chain (device array; the
 // batched launches take blockIdx.z as the instance).  Sizes of the buffers as

@@ -375,6 +375,10 @@ constexpr int kCandCap = 256, kCandCapWide = 1024;
 // initial centroids, the first rk grid points, leave clusters of up to ~1e5
 // points for the first few updates).
 constexpr int kWideSumsIters = 4;
+// Smallest rk for which the bricks cull from super-brick lists: below it a
+// brick's own pass over every centroid costs less than the extra launch
+// (64^3, rk 1024: 17.5 + 5.8 us per iteration with the lists).
+constexpr int kSuperMinRk = 1536;
 
 // Box holding the K-means points of the grid-index ranges [xa, xb] x [ya, yb]
 // x [za, zb]: their lattice box widened by 1e-10 of the cell length, which
@@ -1681,7 +1685,7 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
     // centroids still crowd the corner of the initial ones.
     const int* sl = nullptr;
     const int* sc = nullptr;
-    if (b->grid_mode && s >= 3) {
+    if (b->grid_mode && s >= 3 && rk >= kSuperMinRk) {
       hipLaunchKernelGGL(k_cull_super, dim3(b->nsb), dim3(1024), 0, st, b->cx, b->cy, b->cz,
                          b->ngrid[0], b->ngrid[1], b->ngrid[2], b->dx[0], b->dx[1], b->dx[2],
                          b->slist, b->scount, b->scal);

@@ -191,7 +191,7 @@ struct ValuTile {
   static constexpr int APT = BM * KB / NT;
   static constexpr int BPT = BN * KB / NT;
   static constexpr int LDS_DOUBLES = 2 * KB * (BM + BN);
-  static_assert(TM >= 1 && TN >= 1 && BM % (64 * WGM) == 0 && BN % WGN == 0, "tile shape");
+  static_assert(TM >= 1 && TN % 2 == 0 && BM % (64 * WGM) == 0 && BN % WGN == 0, "tile shape");
   static_assert(BM * KB % NT == 0 && BN * KB % NT == 0, "chunk must split evenly");
   typedef MfmaTile<BM, BN, KB, WGM, 1> Stager;  // same global->LDS staging (B k-major)
   typedef double Acc[TM][TN];
@@ -246,11 +246,14 @@ struct ValuTile {
       double a[TM];
 #pragma unroll
       for (int i = 0; i < TM; i++) a[i] = Aw[kk * BM + 64 * i];
+      // B as pairs: one 16-byte LDS broadcast feeds two columns.
 #pragma unroll
-      for (int j = 0; j < TN; j++) {
-        const double b = Bw[kk * BN + j];
+      for (int j = 0; j < TN; j += 2) {
+        const double2 b = *(const double2*)(Bw + kk * BN + j);
 #pragma unroll
-        for (int i = 0; i < TM; i++) acc[i][j] = __builtin_fma(a[i], b, acc[i][j]);
+        for (int i = 0; i < TM; i++) acc[i][j] = __builtin_fma(a[i], b.x, acc[i][j]);
+#pragma unroll
+        for (int i = 0; i < TM; i++) acc[i][j + 1] = __builtin_fma(a[i], b.y, acc[i][j + 1]);
       }
     }
   }
@@ -294,6 +297,111 @@ struct ValuTile {
     for (int j = 0; j < TN; j++)
 #pragma unroll
       for (int i = 0; i < TM; i++) f(row(i), col(j), a1[i][j], a2[i][j]);
+  }
+};
+
+// Vector-FMA tile with the B operand in scalar registers and no LDS.  NW
+// waves; lane l of wave w owns rows 64 i + l (i < TM = BM / 64) and the
+// columns w TN + j (j < TN = BN / NW), the same for every lane of the wave:
+// the B values of a k step are wave-uniform, so they come through scalar loads
+// (constant address space) and feed v_fmac_f64 as its scalar operand, and the
+// A values of a lane come from global memory through vector loads (the waves
+// of a workgroup read the same A columns, shared in L1).  Scalar loads return
+// out of order, so a use waits for all of them: the loads of step k + 1 are
+// issued only after the first FMA of step k has taken its B value (an empty
+// asm ties their address to that FMA's result), and they land during the rest
+// of step k.  A is prefetched two steps ahead (vector loads return in order).
+// B: row k at B[k * ldb], B_KMAJOR layout; the TN values of a row that a wave
+// reads must lie in readable memory even past nlim (their results are never
+// stored), so the caller's B allocation extends BN doubles past its end.
+template <int BM_, int BN_, int NW_>
+struct ScalarBTile {
+  static constexpr int BM = BM_, BN = BN_, NW = NW_;
+  static constexpr int NT = 64 * NW;
+  static constexpr int TM = BM / 64;
+  static constexpr int TN = BN / NW;
+  static constexpr int LDS_DOUBLES = 1;  // no LDS staging
+  static constexpr int WGN = NW;
+  static_assert(BM % 64 == 0 && BN % NW == 0, "tile shape");
+  typedef double Acc[TM][TN];
+  static_assert(TN % 8 == 0, "B rows go as vectors of 8 doubles");
+  typedef double D8 __attribute__((ext_vector_type(8)));
+  typedef const __attribute__((address_space(4))) D8* CPtr8;
+
+  __device__ static void zero(Acc& acc) {
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+      for (int j = 0; j < TN; j++) acc[i][j] = 0.0;
+  }
+
+  // The wave's TN values of row `off / ldb` of B (scalar loads).
+  __device__ static void load_b(const double* __restrict__ Bw, long long off, D8 (&b)[TN / 8]) {
+#pragma unroll
+    for (int h = 0; h < TN / 8; h++) b[h] = *(CPtr8)(Bw + off + 8 * h);
+  }
+
+  // One k step on register set X: the FMAs of column 0, then (their result
+  // gating the address) the loads of step k + 1 into set Y, then the other
+  // columns while those loads land.  Steps past K re-read step K - 1 (valid
+  // memory; never used).
+  template <int X, int Y>
+  __device__ static void step(const double* __restrict__ A, const int (&ro)[TM], long long lda,
+                              const double* __restrict__ Bw, long long ldb, int k, int K,
+                              double (&a)[2][TM], D8 (&b)[2][TN / 8], Acc& acc) {
+#pragma unroll
+    for (int i = 0; i < TM; i++) acc[i][0] = __builtin_fma(a[X][i], b[X][0][0], acc[i][0]);
+    long long kn = k + 1 < K ? k + 1 : K - 1;
+    asm volatile("" : "+s"(kn) : "v"(acc[0][0]));
+    // Volatile: left alone, the compiler sinks these loads to their use in
+    // the next step and the two register sets collapse into one.
+#pragma unroll
+    for (int h = 0; h < TN / 8; h++) b[Y][h] = *(volatile CPtr8)(Bw + kn * ldb + 8 * h);
+#pragma unroll
+    for (int i = 0; i < TM; i++) a[Y][i] = A[kn * lda + ro[i]];
+#pragma unroll
+    for (int j = 1; j < TN; j++)
+#pragma unroll
+      for (int i = 0; i < TM; i++) acc[i][j] = __builtin_fma(a[X][i], b[X][j / 8][j % 8], acc[i][j]);
+  }
+
+  // acc += A[0:mlim, 0:K] * B[0:K, 0:nlim].  Ends with a workgroup barrier.
+  // Rows at or past mlim read row mlim - 1 (their results are never stored).
+  __device__ static void run(const double* __restrict__ A, long long lda,
+                             const double* __restrict__ B, long long ldb, int K, int mlim,
+                             int nlim, double* lds, Acc& acc) {
+    (void)lds;
+    (void)nlim;
+    if (K > 0) {
+      const int lane = threadIdx.x & 63;
+      const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+      int ro[TM];  // the lane's rows, clamped into the strip
+#pragma unroll
+      for (int i = 0; i < TM; i++) ro[i] = min(lane + 64 * i, mlim - 1);
+      const double* Bw = B + wave * TN;
+      double a[2][TM];
+      D8 b[2][TN / 8];
+#pragma unroll
+      for (int h = 0; h < TN / 8; h++) b[0][h] = *(CPtr8)(Bw + 8 * h);
+#pragma unroll
+      for (int i = 0; i < TM; i++) a[0][i] = A[ro[i]];
+      for (int k = 0; k < K; k += 2) {
+        step<0, 1>(A, ro, lda, Bw, ldb, k, K, a, b, acc);
+        if (k + 1 >= K) break;
+        step<1, 0>(A, ro, lda, Bw, ldb, k + 1, K, a, b, acc);
+      }
+    }
+    __syncthreads();
+  }
+
+  __device__ static int row(int i) { return 64 * i + (threadIdx.x & 63); }
+  __device__ static int col(int j) { return (threadIdx.x >> 6) * TN + j; }
+  template <class F>
+  __device__ static void epilogue(const Acc& acc, F f) {
+#pragma unroll
+    for (int j = 0; j < TN; j++)
+#pragma unroll
+      for (int i = 0; i < TM; i++) f(row(i), col(j), acc[i][j]);
   }
 };
 

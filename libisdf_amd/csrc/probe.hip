@@ -69,12 +69,40 @@ __global__ __launch_bounds__(64) void k_add_chain(double a0, int iters, double* 
   out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
+// Even waves of a workgroup run the MFMA chains of k_mfma_f64, odd waves the
+// FMA chains of k_fma_f64: whether the two FP64 pipes add up when both are busy.
+__global__ __launch_bounds__(256) void k_mix_f64(double a0, double b0, int iters_mfma,
+                                                  int iters_fma, double* out) {
+  double s = 0.0;
+  if (((threadIdx.x >> 6) & 1) == 0) {
+    d4 acc[kChains];
+    for (int c = 0; c < kChains; c++) acc[c] = (d4){0.0, 0.0, 0.0, 0.0};
+    const double a = a0 + threadIdx.x * 1e-9, b = b0 - threadIdx.x * 1e-9;
+    for (int it = 0; it < iters_mfma; it++) {
+#pragma unroll
+      for (int c = 0; c < kChains; c++) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
+    }
+    for (int c = 0; c < kChains; c++) s += acc[c][0] + acc[c][1] + acc[c][2] + acc[c][3];
+  } else {
+    double acc[16];
+    for (int c = 0; c < 16; c++) acc[c] = c * 1e-3;
+    const double a = 0.999 + threadIdx.x * 1e-12, b = 1e-3;
+    for (int it = 0; it < iters_fma; it++) {
+#pragma unroll
+      for (int c = 0; c < 16; c++) acc[c] = __builtin_fma(acc[c], a, b);
+    }
+    for (int c = 0; c < 16; c++) s += acc[c];
+  }
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 }  // namespace
 
 extern "C" {
 
 // which: 0 v_mfma_f64_16x16x4f64, 1 v_fma_f64, 2 v_mfma_f32_16x16x4f32,
-// 3 dependent v_add_f64 chain (rate of one lane, in 1e12 adds/s).
+// 3 dependent v_add_f64 chain (rate of one lane, in 1e12 adds/s), 4 half the
+// waves on v_mfma_f64_16x16x4f64 and half on v_fma_f64 (combined TFLOP/s).
 // Launches `blocks` workgroups of 256 threads (4 waves), each running `iters`
 // rounds, twice (the first run warms the clocks up), and returns the second
 // run's rate in TFLOP/s through *tflops and its time through *ms.
@@ -99,6 +127,11 @@ int isdf_probe_peak(int which, int blocks, int iters, double* tflops, double* ms
     } else if (which == 2) {
       k_mfma_f32<<<blocks, 256>>>(1.0f, 0.5f, iters, out);
       flops_per_thread_iter = 32.0 * kChains;
+    } else if (which == 4) {
+      // Half MFMA waves, half FMA waves; iters is the MFMA loop count and the
+      // FMA loop runs 4x as many rounds (about the same time on each pipe).
+      k_mix_f64<<<blocks, 256>>>(1.0, 0.5, iters, 4 * iters, out);
+      flops_per_thread_iter = 0.5 * (32.0 * kChains) + 0.5 * (2.0 * 16 * 4);
     } else {
       // One wave per workgroup, `blocks` of them; *tflops then reports the
       // dependent adds per second of one lane (1e12 units), *ms the time.

@@ -408,10 +408,13 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   off += align256(sizeof(double) * (size_t)nJ * kNB * kNB);
   const size_t off_D128 = off;
   off += align256(sizeof(double) * (size_t)((rk + 127) / 128) * 128 * 128);
+  // The solve operands F and G, each followed by one block width of readable
+  // padding: the scalar-operand solve tile reads whole rows of a block column
+  // past its last valid column.
   const size_t off_Ft = off;
-  off += align256(sizeof(double) * (size_t)rk * rk);
+  off += align256(sizeof(double) * ((size_t)rk * rk + kSB));
   const size_t off_Gt = off;
-  off += align256(sizeof(double) * (size_t)rk * rk);
+  off += align256(sizeof(double) * ((size_t)rk * rk + kSB));
   const size_t off_sc = off;
   off += 256;
   const bool qrcp = p->select == ISDF_SELECT_QRCP;
