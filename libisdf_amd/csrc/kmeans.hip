@@ -771,6 +771,55 @@ __global__ void k_reset_boxes(int* bbox, int rkmax, const KmJob* jobs) {
     bbox[e] = (e < 3 * rkmax) ? 0x7fffffff : -1;
 }
 
+// Position of this thread's first member among a round's members, taken in
+// position order (thread-major, slots q = 0..PT-1 of a thread in order), and
+// the round's total, with one barrier: ballots rank the members inside a wave,
+// LDS (wsum[NT / 64]) carries the wave totals.  Needs a barrier between the
+// reads of wsum here and the next call's writes.
+template <int NT, int PT>
+__device__ int rank_members(const bool* mem, int* total, int* wsum) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int below = 0, wtot = 0;
+#pragma unroll
+  for (int q = 0; q < PT; q++) {
+    const unsigned long long m = __ballot(mem[q]);
+    below += __popcll(m & lanemask_lt());
+    wtot += __popcll(m);
+  }
+  if (lane == 0) wsum[wave] = wtot;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; w++) {
+    const int v = wsum[w];
+    off += (w < wave) ? v : 0;
+    tot += v;
+  }
+  *total = tot;
+  return off + below;
+}
+
+// 32 steps of four sequential FP64 sums in one wave: row q (lanes 16q..16q+15)
+// carries sum q, and lane 16q + k holds its terms 2k (lo) and 2k+1 (hi) of
+// this batch.  Step m broadcasts lane m/2 of every row (DPP row_newbcast) into
+// acc = fma(term, 1.0, acc), which rounds exactly as acc + term: every lane of
+// a row ends with its sum, one instruction per step for all four sums.  The
+// leading s_nop covers the DPP hazards (a VALU write of a term or of EXEC just
+// before the block).
+__device__ __forceinline__ void add32_rows(double& acc, double lo, double hi, double one) {
+#define ISDF_BCAST2(K)                                                      \
+  "v_fmac_f64_dpp %0, %1, %3 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n" \
+  "v_fmac_f64_dpp %0, %2, %3 row_newbcast:" #K " row_mask:0xf bank_mask:0xf\n"
+  __asm__ volatile("s_nop 4\n" ISDF_BCAST2(0) ISDF_BCAST2(1) ISDF_BCAST2(2) ISDF_BCAST2(3)
+                   ISDF_BCAST2(4) ISDF_BCAST2(5) ISDF_BCAST2(6) ISDF_BCAST2(7) ISDF_BCAST2(8)
+                   ISDF_BCAST2(9) ISDF_BCAST2(10) ISDF_BCAST2(11) ISDF_BCAST2(12)
+                   ISDF_BCAST2(13) ISDF_BCAST2(14) ISDF_BCAST2(15)
+                   : "+v"(acc)
+                   : "v"(lo), "v"(hi), "v"(one));
+#undef ISDF_BCAST2
+}
+
 // Centroid sums, utility.cpp:816-822, when every grid point is a K-means point
 // (local index == grid index): one workgroup per cluster walks the cluster's
 // bounding box in ascending grid order (z, y, x), 1024 positions per round.
@@ -795,9 +844,11 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
   // boxes need a quarter of the passes, and each pass costs a scan and two
   // barriers whatever its size.
   constexpr int CH = BATCH ? 512 : 4 * NT, PT = CH / NT;
-  // Rows padded by 32 bytes: the four adder lanes read four rows at once, which
-  // otherwise start in the same LDS bank.
-  __shared__ __align__(16) double buf[4][CH + 4];
+  static_assert(CH % 32 == 0, "rounds are whole batches of 32 terms");
+  // Row q: term q (w, w x, w y, w z) of the round's members in grid order, then
+  // zeros up to the next multiple of 32 (the adder takes whole batches).
+  __shared__ __align__(16) double buf[4][CH];
+  __shared__ __align__(16) int wsum[NT / 64];
   if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     n0 = J.b.ngrid[0];
@@ -819,9 +870,10 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
   const int rk = scal[KS_RK];
   const int c = blockIdx.x;
   if (c >= rk) return;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int x0 = bbox[c], y0 = bbox[rkmax + c], z0 = bbox[2 * rkmax + c];
   const int x1 = bbox[3 * rkmax + c], y1 = bbox[4 * rkmax + c], z1 = bbox[5 * rkmax + c];
+  // Wave 0 carries the four sums, one per row of 16 lanes (add32_rows).
   double acc = 0.0;
   if (x1 >= x0) {
     const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
@@ -856,7 +908,6 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
     double tw[PT], tx[PT], ty[PT], tz[PT];
     bool mem[PT];
     auto gather = [&](const int* g, const int* l) {
-      int n = 0;
 #pragma unroll
       for (int q = 0; q < PT; q++) {
         mem[q] = l[q] == c;
@@ -865,14 +916,14 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
           tx[q] = X[g[q]];
           ty[q] = Y[g[q]];
           tz[q] = Z[g[q]];
-          n++;
         }
       }
-      return n;
     };
     // Members to LDS in position order; products w*x (one rounding each, as
-    // the reference forms them).
-    auto store = [&](int pos) {
+    // the reference forms them).  Then +0.0 up to the next multiple of 32 (at
+    // most CH): a sum that starts at +0.0 never becomes -0.0, so adding +0.0
+    // leaves it exactly as it is.
+    auto store = [&](int pos, int tot_round) {
 #pragma unroll
       for (int q = 0; q < PT; q++) {
         if (mem[q]) {
@@ -883,16 +934,45 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
           pos++;
         }
       }
+      if (t < 128) {
+        const int i = tot_round + (t & 31);
+        if (i < ((tot_round + 31) & ~31)) buf[t >> 5][i] = 0.0;
+      }
+    };
+    // The sums over one round's members in order, by wave 0: batches of 32
+    // terms, each one LDS read per lane (two terms of its row) and 32 DPP
+    // steps; the reads run two batches ahead.
+    const double one = 1.0;
+    auto add_round = [&](int tot_round) {
+      const int nb = (tot_round + 31) >> 5;
+      if (nb == 0) return;
+      const double* row = &buf[lane >> 4][2 * (lane & 15)];
+      // Batch j (clamped: past the last batch it is read again, not added).
+      auto rd = [&](int j) { return *(const double2*)(row + 32 * min(j, nb - 1)); };
+      // Three register sets in rotation (unrolled, so no copies between them).
+      double2 a = rd(0), b = rd(1), c;
+      for (int j = 0;; j += 3) {
+        c = rd(j + 2);
+        add32_rows(acc, a.x, a.y, one);
+        if (j + 1 >= nb) break;
+        a = rd(j + 3);
+        add32_rows(acc, b.x, b.y, one);
+        if (j + 2 >= nb) break;
+        b = rd(j + 4);
+        add32_rows(acc, c.x, c.y, one);
+        if (j + 3 >= nb) break;
+      }
     };
     // Software pipeline over rounds: labels two rounds ahead, member terms one
-    // round ahead, so threads 0..3 add round r while the loads of rounds r+1
-    // and r+2 are in flight.  The scan of round r+1 waits for those adds (its
+    // round ahead, so wave 0 adds round r while the loads of rounds r+1 and
+    // r+2 are in flight.  The scan of round r+1 waits for those adds (its
     // barrier) before round r+1 overwrites the buffer.
     int gn[PT], ln[PT], gf[PT], lf[PT];  // round r+1 (next), round r+2 (far)
     load_labels(0, gn, ln);
     if (nround > 1) load_labels(1, gf, lf);
     int tot;
-    store(block_excl_scan(gather(gn, ln), &tot));
+    gather(gn, ln);
+    store(rank_members<NT, PT>(mem, &tot, wsum), tot);
     __syncthreads();
     for (int r = 0; r < nround; r++) {
       const int tot_r = tot;
@@ -902,61 +982,18 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
         ln[q] = lf[q];
       }
       if (r + 2 < nround) load_labels(r + 2, gf, lf);
-      const int cnt = (r + 1 < nround) ? gather(gn, ln) : 0;
-      if (t < 4) {
-        // The sum stays strictly sequential; only the LDS reads run ahead:
-        // 16 terms in flight as 8 two-term reads.
-        const double* qv = buf[t];
-        const int nfull = tot_r & ~15;
-        int e = 0;
-        if (nfull) {
-          // Ping-pong between two register sets; the read of the next 16 terms
-          // is issued before the adds of the current 16 (its index clamped
-          // inside the buffer: past nfull it is read and never added).
-          double2 a[8], b[8];
-#pragma unroll
-          for (int u = 0; u < 8; u++) a[u] = *(const double2*)(qv + 2 * u);
-          while (true) {
-            {
-              const int en = min(e + 16, CH - 16);
-#pragma unroll
-              for (int u = 0; u < 8; u++) b[u] = *(const double2*)(qv + en + 2 * u);
-              __asm__ volatile("" ::: "memory");  // keep the reads ahead of the adds
-#pragma unroll
-              for (int u = 0; u < 8; u++) {
-                acc = acc + a[u].x;
-                acc = acc + a[u].y;
-              }
-              e += 16;
-              if (e >= nfull) break;
-            }
-            {
-              const int en = min(e + 16, CH - 16);
-#pragma unroll
-              for (int u = 0; u < 8; u++) a[u] = *(const double2*)(qv + en + 2 * u);
-              __asm__ volatile("" ::: "memory");  // keep the reads ahead of the adds
-#pragma unroll
-              for (int u = 0; u < 8; u++) {
-                acc = acc + b[u].x;
-                acc = acc + b[u].y;
-              }
-              e += 16;
-              if (e >= nfull) break;
-            }
-          }
-        }
-        for (; e < tot_r; e++) acc = acc + qv[e];
-      }
+      if (r + 1 < nround) gather(gn, ln);
+      if (wave == 0) add_round(tot_r);
       if (r + 1 < nround) {
-        store(block_excl_scan(cnt, &tot));
+        store(rank_members<NT, PT>(mem, &tot, wsum), tot);
         __syncthreads();
       }
     }
   }
   if (t == 0) sw[c] = acc;
-  if (t == 1) sx[c] = acc;
-  if (t == 2) sy[c] = acc;
-  if (t == 3) sz[c] = acc;
+  if (t == 16) sx[c] = acc;
+  if (t == 32) sy[c] = acc;
+  if (t == 48) sz[c] = acc;
 }
 
 // findMin(Dim=0) (utility.cpp:926-945) when every grid point is a K-means
