@@ -65,6 +65,7 @@ struct KmeansBufs {
   int* bbox;         // [6 * rkmax] cluster bounding boxes in grid indices
   int nsb;           // super-bricks (2 x 2 x 2 bricks) of the grid
   int* slist;        // [nsb * kSuperCap] candidate centroids of each super-brick, ascending
+  double* slxyz;     // [3 * nsb * kSuperCap] their coordinates (x, y, z planes)
   int* scount;       // [nsb] their number (above kSuperCap: the list overflowed)
   int* scal;         // small device scalars, see KS_* below
 };

@@ -350,6 +350,14 @@ __global__ __launch_bounds__(256) void k_build_blocks(int n, int grid_mode, int 
   porder[(long long)b * 256 + s] = v;
 }
 
+// A value every lane of the wave holds alike, moved to scalar registers.
+__device__ __forceinline__ double uniform_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ double wave_min(double v) {
   for (int d = 32; d > 0; d >>= 1) v = fmin(v, __shfl_xor(v, d));
   return v;
@@ -367,10 +375,14 @@ __device__ double block_min256(double v, double* red) {
 }
 
 // Candidate-list capacity of the assignment: 256 once the centroids have
-// spread; 1024 for the first iterations, when the initial centroids (the first
-// rk grid points, one corner of the grid) leave hundreds of candidates per
-// brick.  A brick with more falls back to testing every centroid.
+// spread; 1024 for the first two assignments, when the initial centroids (the
+// first rk grid points, one corner of the grid) leave hundreds of candidates
+// per brick.  A brick with more takes its candidates in windows of that size.
 constexpr int kCandCap = 256, kCandCapWide = 1024;
+// Grid points per thread of a grid-mode assignment brick (8 x 8 x 4 kBrickPZ
+// points per workgroup of 256): one cull then serves twice as many points,
+// and a 96^3 grid fits the chip in one round of workgroups.
+constexpr int kBrickPZ = 1;
 // Lloyd iterations whose centroid sums use 1024-thread workgroups (the
 // initial centroids, the first rk grid points, leave clusters of up to ~1e5
 // points for the first few updates).
@@ -410,112 +422,148 @@ __device__ double block_min_nt(double v, double* red) {
   return r;
 }
 
-// Lane-strided culling of the centroids listed by `cid` (nsrc of them, in
-// ascending index order) against the box [lo, hi]: the kept ones, in the
-// same order, go to out[0 .. CAP) (and their coordinates to ox / oy / oz when
-// given); returns their number, which may exceed CAP.  A centroid is kept
-// when its lower bound on the distance to the box is <= U (1 + 1e-9), U the
-// smallest upper bound over the listed centroids.  Block-wide (NT threads);
-// uses red[NT / 64] and wcnt[32][NT / 64] in LDS and ends with a barrier.
-template <int NT, int CAP, typename CID>
-__device__ int cull_box(const double* __restrict__ cx, const double* __restrict__ cy,
-                        const double* __restrict__ cz, int nsrc, CID cid, const double lo[3],
-                        const double hi[3], int* out, double* ox, double* oy, double* oz,
-                        double* red, int (*wcnt)[NT / 64]) {
+// Lane-strided culling of the nsrc centroids given by src (list entry idx ->
+// centroid index c and its coordinates, in ascending index order) against the
+// box [lo, hi]: the kept ones, in the same order, go to out[] (and their
+// coordinates to ox / oy / oz when given) in windows of CAP entries, and
+// consume(first, count) runs on each window (all threads, after a barrier;
+// WINDOWS false: the first window only).  Returns the number kept, which may
+// exceed CAP.  A centroid is kept when its lower bound on the distance to
+// the box is <= U (1 + 1e-9), U the smallest upper bound over the listed
+// centroids.  The coordinates of the first GK chunks of NT entries are read
+// once and kept in registers for both passes; later chunks are read again, and
+// so are the kept entries for the output (GK is what the caller's occupancy
+// affords).  Block-wide (NT
+// threads); uses red[NT / 64] and wcnt[32][NT / 64] in LDS (read until the
+// last window) and ends with a barrier.
+template <int NT, int CAP, int GK, bool WINDOWS, typename SRC, typename CONSUME>
+__device__ int cull_box(int nsrc, SRC src, const double lo[3], const double hi[3], int* out,
+                        double* ox, double* oy, double* oz, double* red, int (*wcnt)[NT / 64],
+                        CONSUME consume) {
   constexpr int NW = NT / 64;
+  constexpr int G = 4;
+  static_assert(GK >= 1 && GK <= G, "kept chunks");
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int nchunk = (nsrc + NT - 1) / NT;
-  // Chunks of NT in groups of four whose loads are all issued before use, so
-  // a pass waits on memory once per group rather than once per chunk.
-  constexpr int G = 4;
+  auto fetch = [&](int idx, int& c, double& x, double& y, double& z) {
+    if (idx < nsrc) {
+      src(idx, c, x, y, z);
+    } else {
+      c = 0;
+      x = lo[0];
+      y = lo[1];
+      z = lo[2];
+    }
+  };
+  auto upper = [&](double x, double y, double z) {
+    const double ex = fmax(fabs(x - lo[0]), fabs(x - hi[0]));
+    const double ey = fmax(fabs(y - lo[1]), fabs(y - hi[1]));
+    const double ez = fmax(fabs(z - lo[2]), fabs(z - hi[2]));
+    return ex * ex + ey * ey + ez * ez;
+  };
+  auto lower = [&](double x, double y, double z) {
+    const double ex = fmax(0.0, fmax(lo[0] - x, x - hi[0]));
+    const double ey = fmax(0.0, fmax(lo[1] - y, y - hi[1]));
+    const double ez = fmax(0.0, fmax(lo[2] - z, z - hi[2]));
+    return ex * ex + ey * ey + ez * ez;
+  };
+  double x0[GK], y0[GK], z0[GK];
   double ub = 1e300;
-  for (int j0 = 0; j0 < nchunk; j0 += G) {
-    double vx[G], vy[G], vz[G];
+  {
+    double x[G], y[G], z[G];
 #pragma unroll
     for (int u = 0; u < G; u++) {
-      const int idx = (j0 + u) * NT + t;
-      const bool ok = j0 + u < nchunk && idx < nsrc;
-      const int c = ok ? cid(idx) : 0;
-      vx[u] = ok ? cx[c] : lo[0];
-      vy[u] = ok ? cy[c] : lo[1];
-      vz[u] = ok ? cz[c] : lo[2];
+      int c;
+      fetch(u * NT + t, c, x[u], y[u], z[u]);
     }
 #pragma unroll
-    for (int u = 0; u < G; u++) {
-      const int idx = (j0 + u) * NT + t;
-      if (j0 + u < nchunk && idx < nsrc) {
-        const double ex = fmax(fabs(vx[u] - lo[0]), fabs(vx[u] - hi[0]));
-        const double ey = fmax(fabs(vy[u] - lo[1]), fabs(vy[u] - hi[1]));
-        const double ez = fmax(fabs(vz[u] - lo[2]), fabs(vz[u] - hi[2]));
-        ub = fmin(ub, ex * ex + ey * ey + ez * ez);
-      }
+    for (int u = 0; u < G; u++)
+      if (u * NT + t < nsrc) ub = fmin(ub, upper(x[u], y[u], z[u]));
+#pragma unroll
+    for (int u = 0; u < GK; u++) {
+      x0[u] = x[u];
+      y0[u] = y[u];
+      z0[u] = z[u];
     }
+  }
+  for (int j0 = G; j0 < nchunk; j0 += G) {
+    int c[G];
+    double x[G], y[G], z[G];
+#pragma unroll
+    for (int u = 0; u < G; u++) fetch((j0 + u) * NT + t, c[u], x[u], y[u], z[u]);
+#pragma unroll
+    for (int u = 0; u < G; u++)
+      if ((j0 + u) * NT + t < nsrc) ub = fmin(ub, upper(x[u], y[u], z[u]));
   }
   const double thr = block_min_nt<NT>(ub, red) * (1.0 + 1e-9);
   unsigned keepbits = 0u;  // bit j: list entry NT j + t is kept (nchunk <= 32)
-  for (int j0 = 0; j0 < nchunk; j0 += G) {
-    double vx[G], vy[G], vz[G];
-#pragma unroll
-    for (int u = 0; u < G; u++) {
-      const int idx = (j0 + u) * NT + t;
-      const bool ok = j0 + u < nchunk && idx < nsrc;
-      const int c = ok ? cid(idx) : 0;
-      vx[u] = ok ? cx[c] : lo[0];
-      vy[u] = ok ? cy[c] : lo[1];
-      vz[u] = ok ? cz[c] : lo[2];
-    }
-#pragma unroll
-    for (int u = 0; u < G; u++) {
-      const int j = j0 + u;
-      if (j < nchunk) {  // uniform: every lane takes part in the ballot
-        const int idx = j * NT + t;
-        bool keep = false;
-        if (idx < nsrc) {
-          const double ex = fmax(0.0, fmax(lo[0] - vx[u], vx[u] - hi[0]));
-          const double ey = fmax(0.0, fmax(lo[1] - vy[u], vy[u] - hi[1]));
-          const double ez = fmax(0.0, fmax(lo[2] - vz[u], vz[u] - hi[2]));
-          keep = ex * ex + ey * ey + ez * ez <= thr;
-        }
-        const unsigned long long m = __ballot(keep);
-        if (keep) keepbits |= 1u << j;
-        if (lane == 0) wcnt[j][wave] = __popcll(m);
-      }
-    }
-  }
-  __syncthreads();
-  // Kept entries in list order: chunk j before chunk j + 1, inside a chunk by
-  // wave, then lane.
-  int nc = 0;
-  for (int j = 0; j < nchunk; j++) {
-    const bool keep = (keepbits >> j) & 1u;
+  auto test = [&](int j, double x, double y, double z) {  // j < nchunk (uniform)
+    const bool keep = j * NT + t < nsrc && lower(x, y, z) <= thr;
     const unsigned long long m = __ballot(keep);
-    int before = 0, tot = 0;
+    if (keep) keepbits |= 1u << j;
+    if (lane == 0) wcnt[j][wave] = __popcll(m);
+  };
 #pragma unroll
-    for (int w2 = 0; w2 < NW; w2++) {
-      const int v = wcnt[j][w2];
-      before += (w2 < wave) ? v : 0;
-      tot += v;
-    }
-    if (keep) {
-      const int pos = nc + before + __popcll(m & ((1ull << lane) - 1ull));
-      if (pos < CAP) {
-        const int c = cid(j * NT + t);
-        out[pos] = c;
-        if (ox) {
-          ox[pos] = cx[c];
-          oy[pos] = cy[c];
-          oz[pos] = cz[c];
-        }
-      }
-    }
-    nc += tot;
+  for (int u = 0; u < GK; u++)
+    if (u < nchunk) test(u, x0[u], y0[u], z0[u]);
+  for (int j0 = GK; j0 < nchunk; j0 += G) {
+    int c[G];
+    double x[G], y[G], z[G];
+#pragma unroll
+    for (int u = 0; u < G; u++) fetch((j0 + u) * NT + t, c[u], x[u], y[u], z[u]);
+#pragma unroll
+    for (int u = 0; u < G; u++)
+      if (j0 + u < nchunk) test(j0 + u, x[u], y[u], z[u]);
   }
   __syncthreads();
-  return nc;
+  int total = 0;
+  for (int j = 0; j < nchunk; j++) {
+#pragma unroll
+    for (int w2 = 0; w2 < NW; w2++) total += wcnt[j][w2];
+  }
+  // Kept entries in list order: chunk j before chunk j + 1, inside a chunk by
+  // wave, then lane; window [ws, ws + CAP) of them at a time.
+  const int nwin = WINDOWS ? (total + CAP - 1) / CAP : 1;
+  for (int ws = 0, win = 0; win < nwin; win++, ws += CAP) {
+    int nc = 0;
+    auto emit = [&](int j) {  // j < nchunk (uniform)
+      const bool keep = (keepbits >> j) & 1u;
+      const unsigned long long m = __ballot(keep);
+      int before = 0, tot = 0;
+#pragma unroll
+      for (int w2 = 0; w2 < NW; w2++) {
+        const int v = wcnt[j][w2];
+        before += (w2 < wave) ? v : 0;
+        tot += v;
+      }
+      if (keep) {
+        const int pos = nc + before + __popcll(m & lanemask_lt()) - ws;
+        if (pos >= 0 && pos < CAP) {
+          int c;
+          double x, y, z;
+          src(j * NT + t, c, x, y, z);
+          out[pos] = c;
+          if (ox) {
+            ox[pos] = x;
+            oy[pos] = y;
+            oz[pos] = z;
+          }
+        }
+      }
+      nc += tot;
+    };
+    for (int j = 0; j < nchunk; j++) emit(j);
+    __syncthreads();
+    consume(ws, min(CAP, total - ws));
+    if (win + 1 < nwin) __syncthreads();  // the window is read before the next is written
+  }
+  return total;
 }
 
 // Candidate lists of the super-bricks (2 x 2 x 2 bricks, 16 x 16 x 8 grid
-// points) against every centroid, for k_assign_culled to cull its brick from.
+// points) against every centroid, for k_assign_culled to cull its brick from:
+// the centroid indices (slist) and, in the same order, their coordinates
+// (slxyz: x, y, z planes of nsb * kSuperCap each).
 // A centroid dropped here is, for every point of the super-brick, farther
 // than the nearest centroid by a relative 1e-9, so it is none of its bricks'
 // candidates either.  Grid mode (every grid point a K-means point) only.
@@ -524,6 +572,7 @@ __global__ __launch_bounds__(1024) void k_cull_super(const double* __restrict__ 
                                                      const double* __restrict__ cz, int n0,
                                                      int n1, int n2, double dx0, double dx1,
                                                      double dx2, int* __restrict__ slist,
+                                                     double* __restrict__ slxyz,
                                                      int* __restrict__ scount, const int* scal) {
   __shared__ double red[16];
   __shared__ int wcnt[32][16];
@@ -536,32 +585,44 @@ __global__ __launch_bounds__(1024) void k_cull_super(const double* __restrict__ 
   double lo[3], hi[3];
   brick_box(sx0, min(sx0 + 15, n0 - 1), sy0, min(sy0 + 15, n1 - 1), sz0, min(sz0 + 7, n2 - 1), n0,
             n1, n2, dx0, dx1, dx2, lo, hi);
-  int* out = slist + (long long)sb * kSuperCap;
-  const int nc = cull_box<1024, kSuperCap>(cx, cy, cz, rk, [](int idx) { return idx; }, lo, hi,
-                                           out, (double*)nullptr, (double*)nullptr,
-                                           (double*)nullptr, red, wcnt);
+  const long long o = (long long)sb * kSuperCap, plane = (long long)gridDim.x * kSuperCap;
+  auto all = [=](int idx, int& c, double& x, double& y, double& z) {
+    c = idx;
+    x = cx[idx];
+    y = cy[idx];
+    z = cz[idx];
+  };
+  // One window: a list that overflows is not used (its bricks cull from all).
+  const int nc = cull_box<1024, kSuperCap, 2, false>(rk, all, lo, hi, slist + o, slxyz + o,
+                                                  slxyz + plane + o, slxyz + 2 * plane + o, red,
+                                                  wcnt, [](int, int) {});
   if (t == 0) scount[sb] = nc;
 }
 
 // Fused pdist2 + findMin(Dim=1) (utility.cpp:598-615, :535-562) with
-// conservative culling.  A workgroup takes one block of up to 256 points,
-// bounds their coordinates by a box, and keeps only the centroids whose lower
-// bound LB on the distance to the box is <= U (1 + 1e-9), U the smallest upper
-// bound over all centroids.  A centroid left out is, for every point of the
-// block, farther than the nearest centroid by a relative 1e-9, far beyond the
-// few ulps of rounding in the distance, so it can neither win nor tie; the
-// first minimum over the candidates, kept in ascending order, is therefore
-// the reference's first minimum over all centroids.
-// In grid mode the workgroup also folds its points into the bounding boxes
-// (grid indices) of their clusters, which the centroid sums walk.
-template <bool BATCH, int CAP>
+// conservative culling.  A workgroup takes one block of points, bounds their
+// coordinates by a box, and keeps only the centroids whose lower bound LB on
+// the distance to the box is <= U (1 + 1e-9), U the smallest upper bound over
+// all centroids.  A centroid left out is, for every point of the block,
+// farther than the nearest centroid by a relative 1e-9, far beyond the few
+// ulps of rounding in the distance, so it can neither win nor tie; the first
+// minimum over the candidates, kept in ascending order, is therefore the
+// reference's first minimum over all centroids.
+// Blocks: in grid mode a brick of 8 x 8 x 4 PZ grid points, thread t holding
+// points (t & 7, (t >> 3) & 7, (t >> 6) + 4 p) of it for p < PZ (one cull
+// serves PZ x 256 points); otherwise (PZ = 1) a run of 256 entries of the
+// point order.  In grid mode the workgroup also folds its points into the
+// bounding boxes (grid indices) of their clusters, which the centroid sums
+// walk.
+template <bool BATCH, int CAP, int PZ>
 __global__ __launch_bounds__(256) void k_assign_culled(
     const int* __restrict__ porder, const double* __restrict__ X, const double* __restrict__ Y,
     const double* __restrict__ Z, const double* __restrict__ cx, const double* __restrict__ cy,
     const double* __restrict__ cz, int* __restrict__ label, int* scal, int count_changes,
     int* __restrict__ bbox, int rkmax, int grid_mode, int n0, int n1, int n2, double dx0,
     double dx1, double dx2, int* __restrict__ changed_blk, const int* __restrict__ slist,
-    const int* __restrict__ scount, const KmJob* jobs) {
+    const double* __restrict__ slxyz, const int* __restrict__ scount, int nsb,
+    const KmJob* jobs) {
   __shared__ int cand[CAP];
   if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
@@ -587,16 +648,43 @@ __global__ __launch_bounds__(256) void k_assign_culled(
     changed_blk = J.b.changed_blk;
     slist = nullptr;
   }
-  __shared__ double ccx[CAP], ccy[CAP], ccz[CAP];
-  __shared__ int cbox[6][CAP];
+  // Candidate coordinates; once the distances are done the same LDS holds the
+  // block's box of each candidate's points (6 ints per candidate, cbox).
+  __shared__ __align__(16) double ccs[3 * CAP];
+  double* const ccx = ccs;
+  double* const ccy = ccs + CAP;
+  double* const ccz = ccs + 2 * CAP;
+  int(*const cbox)[CAP] = reinterpret_cast<int(*)[CAP]>(ccs);
   __shared__ double red[8];
   __shared__ double rbox[6][4];
   if (count_changes && scal[KS_ACTIVE] == 0) return;
   const int rk = scal[KS_RK];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int i = porder[(long long)blockIdx.x * 256 + t];
-  const bool valid = i >= 0;
-  const double px = valid ? X[i] : 0.0, py = valid ? Y[i] : 0.0, pz = valid ? Z[i] : 0.0;
+  const int nbx = (n0 + 7) / 8, nby = (n1 + 7) / 8;
+  const int b = blockIdx.x;
+  const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);  // grid mode
+  int i[PZ];
+  if (grid_mode) {
+    const int x = bx * 8 + (t & 7), y = by * 8 + ((t >> 3) & 7);
+#pragma unroll
+    for (int p = 0; p < PZ; p++) {
+      const int z = bz * 4 * PZ + (t >> 6) + 4 * p;
+      i[p] = (x < n0 && y < n1 && z < n2) ? x + n0 * (y + n1 * z) : -1;
+    }
+  } else {
+    i[0] = porder[(long long)b * 256 + t];  // PZ == 1 here
+  }
+  bool valid[PZ];
+  double px[PZ], py[PZ], pz[PZ];
+  int old[PZ];
+#pragma unroll
+  for (int p = 0; p < PZ; p++) {
+    valid[p] = i[p] >= 0;
+    px[p] = valid[p] ? X[i[p]] : 0.0;
+    py[p] = valid[p] ? Y[i[p]] : 0.0;
+    pz[p] = valid[p] ? Z[i[p]] : 0.0;
+    old[p] = (count_changes && valid[p]) ? label[i[p]] : -1;
+  }
 
   double lo[3], hi[3];
   if (grid_mode) {
@@ -604,15 +692,13 @@ __global__ __launch_bounds__(256) void k_assign_culled(
     // (utility.cpp:711-713) puts every point within a few ulps of its lattice
     // position, so the lattice box widened by 1e-10 L holds every point (a
     // wider box only keeps more candidates).  A brick always holds a point.
-    const int nbx = (n0 + 7) / 8, nby = (n1 + 7) / 8;
-    const int b = blockIdx.x;
-    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
-    brick_box(bx * 8, min(bx * 8 + 7, n0 - 1), by * 8, min(by * 8 + 7, n1 - 1), bz * 4,
-              min(bz * 4 + 3, n2 - 1), n0, n1, n2, dx0, dx1, dx2, lo, hi);
+    brick_box(bx * 8, min(bx * 8 + 7, n0 - 1), by * 8, min(by * 8 + 7, n1 - 1), bz * 4 * PZ,
+              min(bz * 4 * PZ + 4 * PZ - 1, n2 - 1), n0, n1, n2, dx0, dx1, dx2, lo, hi);
   } else {
     // Bounding box of the block's points.
-    double v[6] = {valid ? px : 1e300,  valid ? py : 1e300,  valid ? pz : 1e300,
-                   valid ? -px : 1e300, valid ? -py : 1e300, valid ? -pz : 1e300};
+    double v[6] = {valid[0] ? px[0] : 1e300,  valid[0] ? py[0] : 1e300,
+                   valid[0] ? pz[0] : 1e300,  valid[0] ? -px[0] : 1e300,
+                   valid[0] ? -py[0] : 1e300, valid[0] ? -pz[0] : 1e300};
 #pragma unroll
     for (int k = 0; k < 6; k++) {
       const double m = wave_min(v[k]);
@@ -625,32 +711,99 @@ __global__ __launch_bounds__(256) void k_assign_culled(
       hi[k] = -fmin(fmin(rbox[k + 3][0], rbox[k + 3][1]), fmin(rbox[k + 3][2], rbox[k + 3][3]));
     }
     if (lo[0] > 1e299) {  // no point in this block (uniform)
-      if (count_changes && t == 0) changed_blk[blockIdx.x] = 0;
+      if (count_changes && t == 0) changed_blk[b] = 0;
       return;
     }
   }
 
+  // The box is uniform over the workgroup: scalar registers.
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    lo[k] = uniform_f64(lo[k]);
+    hi[k] = uniform_f64(hi[k]);
+  }
+
   // The centroids to cull: all of them, or (one instance, grid mode) the list
-  // of the brick's super-brick (2 x 2 x 2 bricks), when it did not overflow.
+  // of the brick's super-brick (16 x 16 x 8 grid points), when it did not
+  // overflow.
   const int* sl = nullptr;
+  const double *slx = nullptr, *sly = nullptr, *slz = nullptr;
   int nsrc = rk;
   if (slist) {
-    const int nbx = (n0 + 7) / 8, nby = (n1 + 7) / 8;
-    const int b = blockIdx.x;
-    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
     const int nsbx = (n0 + 15) / 16, nsby = (n1 + 15) / 16;
-    const int sb = (bx >> 1) + nsbx * ((by >> 1) + nsby * (bz >> 1));
+    const int sb = (bx >> 1) + nsbx * ((by >> 1) + nsby * ((bz * PZ) >> 1));
     const int ns = scount[sb];
     if (ns <= kSuperCap) {
-      sl = slist + (long long)sb * kSuperCap;
+      const long long o = (long long)sb * kSuperCap, plane = (long long)nsb * kSuperCap;
+      sl = slist + o;
+      slx = slxyz + o;
+      sly = slxyz + plane + o;
+      slz = slxyz + 2 * plane + o;
       nsrc = ns;
     }
   }
   __shared__ int wcnt[32][4];
-  const int nc = cull_box<256, CAP>(
-      cx, cy, cz, nsrc, [sl](int idx) { return sl ? sl[idx] : idx; }, lo, hi, cand, ccx, ccy,
-      ccz, red, wcnt);
+  auto src = [=](int idx, int& c, double& x, double& y, double& z) {
+    if (sl) {
+      c = sl[idx];
+      x = slx[idx];
+      y = sly[idx];
+      z = slz[idx];
+    } else {
+      c = idx;
+      x = cx[idx];
+      y = cy[idx];
+      z = cz[idx];
+    }
+  };
+  // First minimum over the candidates (ascending), window by window, for the
+  // PZ points at once: bi the centroid, bk its place in the candidate list.
+  double best[PZ];
+  int bk[PZ], bi[PZ];
+#pragma unroll
+  for (int p = 0; p < PZ; p++) {
+    best[p] = __builtin_inf();
+    bk[p] = 0;
+    bi[p] = 0;
+  }
+  auto consume = [&](int ws, int n) {
+    int kw[PZ];
+#pragma unroll
+    for (int p = 0; p < PZ; p++) kw[p] = -1;
+    int k = 0;
+    for (; k + 1 < n; k += 2) {
+      const double qx0 = ccx[k], qy0 = ccy[k], qz0 = ccz[k];
+      const double qx1 = ccx[k + 1], qy1 = ccy[k + 1], qz1 = ccz[k + 1];
+#pragma unroll
+      for (int p = 0; p < PZ; p++) {
+        const double d0 = dist2(px[p], py[p], pz[p], qx0, qy0, qz0);
+        const double d1 = dist2(px[p], py[p], pz[p], qx1, qy1, qz1);
+        if (d0 < best[p]) { best[p] = d0; kw[p] = k; }
+        if (d1 < best[p]) { best[p] = d1; kw[p] = k + 1; }
+      }
+    }
+    if (k < n) {
+      const double qx = ccx[k], qy = ccy[k], qz = ccz[k];
+#pragma unroll
+      for (int p = 0; p < PZ; p++) {
+        const double d = dist2(px[p], py[p], pz[p], qx, qy, qz);
+        if (d < best[p]) { best[p] = d; kw[p] = k; }
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < PZ; p++) {
+      if (kw[p] >= 0) {
+        bk[p] = ws + kw[p];
+        bi[p] = cand[kw[p]];
+      }
+    }
+  };
+  const int nc = cull_box<256, CAP, (PZ == 1 ? 2 : 1), true>(nsrc, src, lo, hi, cand, ccx, ccy,
+                                                             ccz, red, wcnt, consume);
+  // One window: the block's cluster boxes fold through LDS (cbox, by place in
+  // the candidate list); several: straight into the global boxes.
   const bool culled = nc <= CAP;
+  __syncthreads();  // every thread is done with the candidates' coordinates
   if (culled && grid_mode) {
     for (int k = t; k < nc; k += 256) {
       cbox[0][k] = 0x7fffffff;
@@ -662,81 +815,54 @@ __global__ __launch_bounds__(256) void k_assign_culled(
     }
   }
   __syncthreads();
-
-  int bk = 0, bi = 0;
-  if (culled) {
-    double best = dist2(px, py, pz, ccx[0], ccy[0], ccz[0]);
-    int k = 1;
-    for (; k + 3 < nc; k += 4) {
-      const double d0 = dist2(px, py, pz, ccx[k], ccy[k], ccz[k]);
-      const double d1 = dist2(px, py, pz, ccx[k + 1], ccy[k + 1], ccz[k + 1]);
-      const double d2 = dist2(px, py, pz, ccx[k + 2], ccy[k + 2], ccz[k + 2]);
-      const double d3 = dist2(px, py, pz, ccx[k + 3], ccy[k + 3], ccz[k + 3]);
-      if (d0 < best) { best = d0; bk = k; }
-      if (d1 < best) { best = d1; bk = k + 1; }
-      if (d2 < best) { best = d2; bk = k + 2; }
-      if (d3 < best) { best = d3; bk = k + 3; }
+  int nchanged = 0;
+#pragma unroll
+  for (int p = 0; p < PZ; p++) {
+    if (valid[p]) {
+      if (count_changes) nchanged += (old[p] != bi[p]) ? 1 : 0;
+      label[i[p]] = bi[p];
     }
-    for (; k < nc; k++) {
-      const double d = dist2(px, py, pz, ccx[k], ccy[k], ccz[k]);
-      if (d < best) {
-        best = d;
-        bk = k;
-      }
-    }
-    bi = cand[bk];
-  } else {
-    double best = dist2(px, py, pz, cx[0], cy[0], cz[0]);
-    for (int c = 1; c < rk; c++) {
-      const double d = dist2(px, py, pz, cx[c], cy[c], cz[c]);
-      if (d < best) {
-        best = d;
-        bi = c;
-      }
-    }
-  }
-  bool changed = false;
-  if (valid) {
-    if (count_changes) changed = (label[i] != bi);
-    label[i] = bi;
   }
   if (count_changes) {
     // Per-block count; the loop control sums the blocks (no contended atomics).
-    const unsigned long long m = __ballot(changed);
-    if (lane == 0) wcnt[0][wave] = __popcll(m);
+    int v = nchanged;
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+    if (lane == 0) wcnt[0][wave] = v;
     __syncthreads();
-    if (t == 0) changed_blk[blockIdx.x] = wcnt[0][0] + wcnt[0][1] + wcnt[0][2] + wcnt[0][3];
+    if (t == 0) changed_blk[b] = wcnt[0][0] + wcnt[0][1] + wcnt[0][2] + wcnt[0][3];
   }
   if (!grid_mode) return;
   if (culled) {
-    // Boxes of the clusters, per wave: lane l holds brick point (x, y) =
-    // (l & 7, l >> 3) of the wave's z plane (k_build_blocks), so the ballot of
-    // one label's lanes gives its x and y extent by bit folding (uniform,
-    // scalar work) and one lane folds it into the block's box of that
-    // candidate: no contended LDS atomics.
-    const int nbx = (n0 + 7) / 8, nby = (n1 + 7) / 8;
-    const int b = blockIdx.x;
-    const int bx0 = (b % nbx) * 8, by0 = ((b / nbx) % nby) * 8, zw = (b / (nbx * nby)) * 4 + wave;
-    unsigned long long todo = __ballot(valid);
-    while (todo) {
-      const int leader = __ffsll((long long)todo) - 1;
-      const int kk = __shfl(bk, leader);
-      const unsigned long long m = __ballot(valid && bk == kk);
-      todo &= ~m;
-      unsigned cols = 0u, rows = 0u;
+    // Boxes of the clusters, per wave and z plane: lane l holds brick point
+    // (x, y) = (l & 7, l >> 3) of the plane, so the ballot of one label's lanes
+    // gives its x and y extent by bit folding (uniform, scalar work) and one
+    // lane folds it into the block's box of that candidate: no contended LDS
+    // atomics.
+    const int bx0 = bx * 8, by0 = by * 8;
 #pragma unroll
-      for (int yy = 0; yy < 8; yy++) {
-        const unsigned r8 = (unsigned)(m >> (8 * yy)) & 0xffu;
-        cols |= r8;
-        rows |= (r8 ? 1u : 0u) << yy;
-      }
-      if (lane == leader) {
-        atomicMin(&cbox[0][kk], bx0 + __ffs(cols) - 1);
-        atomicMin(&cbox[1][kk], by0 + __ffs(rows) - 1);
-        atomicMin(&cbox[2][kk], zw);
-        atomicMax(&cbox[3][kk], bx0 + 31 - __clz(cols));
-        atomicMax(&cbox[4][kk], by0 + 31 - __clz(rows));
-        atomicMax(&cbox[5][kk], zw);
+    for (int p = 0; p < PZ; p++) {
+      const int zw = bz * 4 * PZ + wave + 4 * p;
+      unsigned long long todo = __ballot(valid[p]);
+      while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int kk = __shfl(bk[p], leader);
+        const unsigned long long m = __ballot(valid[p] && bk[p] == kk);
+        todo &= ~m;
+        unsigned cols = 0u, rows = 0u;
+#pragma unroll
+        for (int yy = 0; yy < 8; yy++) {
+          const unsigned r8 = (unsigned)(m >> (8 * yy)) & 0xffu;
+          cols |= r8;
+          rows |= (r8 ? 1u : 0u) << yy;
+        }
+        if (lane == leader) {
+          atomicMin(&cbox[0][kk], bx0 + __ffs(cols) - 1);
+          atomicMin(&cbox[1][kk], by0 + __ffs(rows) - 1);
+          atomicMin(&cbox[2][kk], zw);
+          atomicMax(&cbox[3][kk], bx0 + 31 - __clz(cols));
+          atomicMax(&cbox[4][kk], by0 + 31 - __clz(rows));
+          atomicMax(&cbox[5][kk], zw);
+        }
       }
     }
     __syncthreads();
@@ -750,14 +876,19 @@ __global__ __launch_bounds__(256) void k_assign_culled(
       atomicMax(&bbox[4 * rkmax + c], cbox[4][k]);
       atomicMax(&bbox[5 * rkmax + c], cbox[5][k]);
     }
-  } else if (valid) {
-    const int gx = i % n0, gy = (i / n0) % n1, gz = i / (n0 * n1);
-    atomicMin(&bbox[bi], gx);
-    atomicMin(&bbox[rkmax + bi], gy);
-    atomicMin(&bbox[2 * rkmax + bi], gz);
-    atomicMax(&bbox[3 * rkmax + bi], gx);
-    atomicMax(&bbox[4 * rkmax + bi], gy);
-    atomicMax(&bbox[5 * rkmax + bi], gz);
+  } else {
+#pragma unroll
+    for (int p = 0; p < PZ; p++) {
+      if (!valid[p]) continue;
+      const int q = bi[p];
+      const int gx = i[p] % n0, gy = (i[p] / n0) % n1, gz = i[p] / (n0 * n1);
+      atomicMin(&bbox[q], gx);
+      atomicMin(&bbox[rkmax + q], gy);
+      atomicMin(&bbox[2 * rkmax + q], gz);
+      atomicMax(&bbox[3 * rkmax + q], gx);
+      atomicMax(&bbox[4 * rkmax + q], gy);
+      atomicMax(&bbox[5 * rkmax + q], gz);
+    }
   }
 }
 
@@ -1497,6 +1628,25 @@ __global__ void k_km_batch_clear_flags(const KmJob* __restrict__ jobs) {
     J.b.flags[i] = 0;
 }
 
+// One assignment of a single instance: bricks of kBrickPZ planes per thread in
+// grid mode, runs of 256 points otherwise; sl / sc: super-brick lists or null.
+template <int CAP>
+void launch_assign(KmeansBufs* b, int count_changes, const int* sl, const int* sc, hipStream_t st) {
+  if (b->grid_mode)
+    hipLaunchKernelGGL((k_assign_culled<false, CAP, kBrickPZ>), dim3(b->nblk_pts), dim3(256), 0, st,
+                       (const int*)nullptr, b->X, b->Y, b->Z, b->cx, b->cy, b->cz, b->label,
+                       b->scal, count_changes, b->bbox, b->rkmax, 1, b->ngrid[0], b->ngrid[1],
+                       b->ngrid[2], b->dx[0], b->dx[1], b->dx[2], b->changed_blk, sl, b->slxyz,
+                       sc, b->nsb, (const KmJob*)nullptr);
+  else
+    hipLaunchKernelGGL((k_assign_culled<false, CAP, 1>), dim3(b->nblk_pts), dim3(256), 0, st,
+                       b->porder, b->X, b->Y, b->Z, b->cx, b->cy, b->cz, b->label, b->scal,
+                       count_changes, b->bbox, b->rkmax, 0, b->ngrid[0], b->ngrid[1],
+                       b->ngrid[2], b->dx[0], b->dx[1], b->dx[2], b->changed_blk,
+                       (const int*)nullptr, (const double*)nullptr, (const int*)nullptr, 0,
+                       (const KmJob*)nullptr);
+}
+
 int compact_flags(KmeansBufs* b, int want, int base, int* out, hipStream_t st,
                   const int* base_dev = nullptr) {
   const int nblk = b->nblk_flag;
@@ -1575,7 +1725,10 @@ void kmeans_carve(KmeansBufs* b, void* basep, int n, const int ngrid[3], int rkm
   }
   b->bbox = (int*)take(sizeof(int) * 6 * (size_t)rkmax);
   b->nsb = ((ngrid[0] + 15) / 16) * ((ngrid[1] + 15) / 16) * ((ngrid[2] + 7) / 8);
-  b->slist = (int*)take(sizeof(int) * (size_t)b->nsb * kSuperCap);
+  // Super-brick candidate lists, used from rk = kSuperMinRk on.
+  const bool super = rkmax >= kSuperMinRk;
+  b->slist = super ? (int*)take(sizeof(int) * (size_t)b->nsb * kSuperCap) : nullptr;
+  b->slxyz = super ? (double*)take(sizeof(double) * 3 * (size_t)b->nsb * kSuperCap) : nullptr;
   b->scount = (int*)take(sizeof(int) * (size_t)b->nsb);
   b->scal = (int*)take(sizeof(int) * KS_COUNT);
 }
@@ -1632,7 +1785,8 @@ void kmeans_geometry(KmeansBufs* b, const int ngrid[3], const double length[3], 
   }
   b->grid_mode = (npt == b->n);
   if (b->grid_mode) {
-    const int nbx = (ngrid[0] + 7) / 8, nby = (ngrid[1] + 7) / 8, nbz = (ngrid[2] + 3) / 4;
+    const int nbx = (ngrid[0] + 7) / 8, nby = (ngrid[1] + 7) / 8;
+    const int nbz = (ngrid[2] + 4 * kBrickPZ - 1) / (4 * kBrickPZ);
     b->nblk_pts = nbx * nby * nbz;
   } else {
     b->nblk_pts = (int)cdiv(b->n, 256);
@@ -1650,18 +1804,15 @@ int kmeans_setup_points(KmeansBufs* b, const int ngrid[3], const double length[3
   hipLaunchKernelGGL(k_init_centroids<false>, dim3(cdiv(rk, 256)), dim3(256), 0, st, rk, n, npt, b->index,
                      ngrid[0], ngrid[1], dx0, dx1, dx2, b->cx, b->cy, b->cz, b->scal, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_init_centroids");
-  hipLaunchKernelGGL(k_build_blocks<false>, dim3(b->nblk_pts), dim3(256), 0, st, n, b->grid_mode ? 1 : 0,
-                     ngrid[0], ngrid[1], ngrid[2], b->porder, (const KmJob*)nullptr);
-  ISDF_HIP_LAUNCHED("k_build_blocks");
+  if (!b->grid_mode) {  // grid mode: the assignment numbers the bricks itself
+    hipLaunchKernelGGL(k_build_blocks<false>, dim3(b->nblk_pts), dim3(256), 0, st, n, 0, ngrid[0],
+                       ngrid[1], ngrid[2], b->porder, (const KmJob*)nullptr);
+    ISDF_HIP_LAUNCHED("k_build_blocks");
+  }
   hipLaunchKernelGGL(k_reset_boxes<false>, dim3(cdiv(6 * (long long)b->rkmax, 256)), dim3(256), 0, st,
                      b->bbox, b->rkmax, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_reset_boxes");
-  hipLaunchKernelGGL((k_assign_culled<false, kCandCapWide>), dim3(b->nblk_pts), dim3(256), 0, st,
-                     b->porder, b->X, b->Y,
-                     b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 0, b->bbox, b->rkmax,
-                     b->grid_mode ? 1 : 0, ngrid[0], ngrid[1], ngrid[2], b->dx[0], b->dx[1], b->dx[2],
-                     b->changed_blk, (const int*)nullptr,
-                     (const int*)nullptr, (const KmJob*)nullptr);
+  launch_assign<kCandCapWide>(b, 0, nullptr, nullptr, st);
   ISDF_HIP_LAUNCHED("k_assign_culled");
   return 0;
 }
@@ -1725,22 +1876,14 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
     if (b->grid_mode && s >= 3 && rk >= kSuperMinRk) {
       hipLaunchKernelGGL(k_cull_super, dim3(b->nsb), dim3(1024), 0, st, b->cx, b->cy, b->cz,
                          b->ngrid[0], b->ngrid[1], b->ngrid[2], b->dx[0], b->dx[1], b->dx[2],
-                         b->slist, b->scount, b->scal);
+                         b->slist, b->slxyz, b->scount, b->scal);
       sl = b->slist;
       sc = b->scount;
     }
-    if (s < 3)
-      hipLaunchKernelGGL((k_assign_culled<false, kCandCapWide>), dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X,
-                         b->Y, b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 1, b->bbox, b->rkmax,
-                         b->grid_mode ? 1 : 0, b->ngrid[0], b->ngrid[1], b->ngrid[2], b->dx[0], b->dx[1],
-                         b->dx[2], b->changed_blk, sl, sc,
-                         (const KmJob*)nullptr);
+    if (s == 0)
+      launch_assign<kCandCapWide>(b, 1, sl, sc, st);
     else
-      hipLaunchKernelGGL((k_assign_culled<false, kCandCap>), dim3(b->nblk_pts), dim3(256), 0, st, b->porder, b->X,
-                         b->Y, b->Z, b->cx, b->cy, b->cz, b->label, b->scal, 1, b->bbox, b->rkmax,
-                         b->grid_mode ? 1 : 0, b->ngrid[0], b->ngrid[1], b->ngrid[2], b->dx[0], b->dx[1],
-                         b->dx[2], b->changed_blk, sl, sc,
-                         (const KmJob*)nullptr);
+      launch_assign<kCandCap>(b, 1, sl, sc, st);
     ISDF_HIP_LAUNCHED("k-means iteration");
     if (poll && (s + 1) % check_every == 0 && s + 1 < max_iter) {
       int act = 1;
@@ -1831,16 +1974,14 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
   hipLaunchKernelGGL(k_init_centroids<true>, dim3(cdiv(maxrk, 256), nb), dim3(256), 0, st, 0, 0, 0,
                      (const int*)nullptr, 0, 0, 0.0, 0.0, 0.0, (double*)nullptr, (double*)nullptr,
                      (double*)nullptr, (int*)nullptr, J);
-  hipLaunchKernelGGL(k_build_blocks<true>, dim3(maxpts, nb), dim3(256), 0, st, 0, 1, 0, 0, 0,
-                     (int*)nullptr, J);
   hipLaunchKernelGGL(k_reset_boxes<true>, dim3(cdiv(6LL * maxrk, 256), nb), dim3(256), 0, st,
                      (int*)nullptr, 0, J);
-  hipLaunchKernelGGL((k_assign_culled<true, kCandCapWide>), dim3(maxpts, nb), dim3(256), 0, st,
+  hipLaunchKernelGGL((k_assign_culled<true, kCandCapWide, kBrickPZ>), dim3(maxpts, nb), dim3(256), 0, st,
                      (const int*)nullptr,
                      (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                      (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                      (int*)nullptr, (int*)nullptr, 0, (int*)nullptr, 0, 1, 0, 0, 0, 0.0, 0.0, 0.0, (int*)nullptr,
-                     (const int*)nullptr, (const int*)nullptr, J);
+                     (const int*)nullptr, (const double*)nullptr, (const int*)nullptr, 0, J);
   hipLaunchKernelGGL(k_km_batch_active, dim3(1, nb), dim3(64), 0, st, J);
   ISDF_HIP_LAUNCHED("k-means setup (batched)");
   // Lloyd iterations (utility.cpp:812-922); converged instances return at once.
@@ -1854,18 +1995,18 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
                        (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                        (double*)nullptr, (double*)nullptr, (double*)nullptr, (int*)nullptr,
                        (int*)nullptr, (int*)nullptr, 0, J);
-    if (s < 3)
-      hipLaunchKernelGGL((k_assign_culled<true, kCandCapWide>), dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
+    if (s == 0)
+      hipLaunchKernelGGL((k_assign_culled<true, kCandCapWide, kBrickPZ>), dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                          (int*)nullptr, (int*)nullptr, 1, (int*)nullptr, 0, 1, 0, 0, 0, 0.0, 0.0, 0.0, (int*)nullptr,
-                         (const int*)nullptr, (const int*)nullptr, J);
+                         (const int*)nullptr, (const double*)nullptr, (const int*)nullptr, 0, J);
     else
-      hipLaunchKernelGGL((k_assign_culled<true, kCandCap>), dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
+      hipLaunchKernelGGL((k_assign_culled<true, kCandCap, kBrickPZ>), dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                          (int*)nullptr, (int*)nullptr, 1, (int*)nullptr, 0, 1, 0, 0, 0, 0.0, 0.0, 0.0, (int*)nullptr,
-                         (const int*)nullptr, (const int*)nullptr, J);
+                         (const int*)nullptr, (const double*)nullptr, (const int*)nullptr, 0, J);
     hipLaunchKernelGGL(k_control<true>, dim3(1, nb), dim3(1024), 0, st, s, 0, 0.0, 0,
                        (const int*)nullptr, (int*)nullptr, (int*)nullptr, J);
     ISDF_HIP_LAUNCHED("k-means iteration (batched)");
