@@ -117,12 +117,21 @@ __global__ __launch_bounds__(256) void k_weights(const double* __restrict__ psi,
   if (r >= ng) return;
   double sI = 0.0, sJ = 0.0;  // sI over S2, sJ over S1 (reference names)
   const double* p = psi + r;
-#pragma unroll 4
-  for (int i = lo; i < hi; i++) {
-    double v = p[(long long)i * ld];
-    double q = v * v;
-    if (i >= s2lo && i < s2hi) sI = sI + q;
-    if (i >= s1lo && i < s1hi) sJ = sJ + q;
+  // 16 states' loads in flight per lane (HBM latency), then the sums in order.
+  constexpr int U = 16;
+  for (int i0 = lo; i0 < hi; i0 += U) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = (i0 + u < hi) ? p[(long long)(i0 + u) * ld] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u;
+      if (i < hi) {
+        const double q = v[u] * v[u];
+        if (i >= s2lo && i < s2hi) sI = sI + q;
+        if (i >= s1lo && i < s1hi) sJ = sJ + q;
+      }
+    }
   }
   double acc = 0.0;
   acc = acc + sI * sJ;
@@ -130,10 +139,13 @@ __global__ __launch_bounds__(256) void k_weights(const double* __restrict__ psi,
 }
 
 // findMax (numvec_impl.hpp:259-266) as an atomic max over IEEE bit patterns:
-// all weights are >= +0, where the bit order is the numeric order.
+// all weights are >= +0, where the bit order is the numeric order.  One
+// atomic per workgroup (same-address atomics serialise; the launch keeps to a
+// few hundred workgroups).
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_maxw(const double* __restrict__ w, int n,
                                               unsigned long long* maxbits, const KmJob* jobs) {
+  __shared__ unsigned long long wm[4];
   if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     w = J.b.w;
@@ -151,26 +163,47 @@ __global__ __launch_bounds__(256) void k_maxw(const double* __restrict__ w, int 
     unsigned long long o = __shfl_xor(m, d);
     m = o > m ? o : m;
   }
-  if ((threadIdx.x & 63) == 0) atomicMax(maxbits, m);
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; k++) m = wm[k] > m ? wm[k] : m;
+    atomicMax(maxbits, m);
+  }
 }
 
-// flags[i] = w[i] > df_tol * maxW (utility.cpp:650-659).
+// flags[i] = w[i] > df_tol * maxW (utility.cpp:650-659; every flag set when
+// df_tol <= 1e-16), and the number set per tile of kFlagTile into bcount.
 template <bool BATCH>
-__global__ __launch_bounds__(256) void k_thresh(const double* __restrict__ w, int n,
-                                                const unsigned long long* maxbits, double df_tol,
-                                                uint8_t* __restrict__ flags, const KmJob* jobs) {
+__global__ __launch_bounds__(256) void k_thresh_count(const double* __restrict__ w, int n,
+                                                      const unsigned long long* maxbits,
+                                                      double df_tol, uint8_t* __restrict__ flags,
+                                                      int* __restrict__ bcount,
+                                                      const KmJob* jobs) {
   if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
+    if ((int)blockIdx.x >= J.b.nblk_flag) return;
     w = J.b.w;
     n = J.b.n;
     maxbits = (const unsigned long long*)(J.b.scal + KS_MAXW);
     flags = J.b.flags;
+    bcount = J.b.bcount;
   }
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double maxW = __longlong_as_double((long long)*maxbits);
-  double thr = df_tol * maxW;
-  flags[i] = (w[i] > thr) ? 1 : 0;
+  const bool all = !(df_tol > 1e-16);
+  const double thr = df_tol * __longlong_as_double((long long)*maxbits);
+  const long long i0 = (long long)blockIdx.x * kFlagTile + threadIdx.x * 8;
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const long long i = i0 + k;
+    if (i < n) {
+      const bool f = all || w[i] > thr;
+      flags[i] = f ? 1 : 0;
+      c += f ? 1 : 0;
+    }
+  }
+  int tot;
+  block_excl_scan(c, &tot);
+  if (threadIdx.x == 0) bcount[blockIdx.x] = tot;
 }
 
 // Ordered compaction: count, scan, scatter of indices whose flag == want.
@@ -196,9 +229,11 @@ __global__ __launch_bounds__(256) void k_flag_count(const uint8_t* __restrict__ 
   if (threadIdx.x == 0) bcount[blockIdx.x] = tot;
 }
 
-// In-place exclusive scan of a[0..m) by one block; a[m] receives the total.
+// In-place exclusive scan of a[0..m) by one block; a[m] receives the total,
+// and so does *total_out when given.
 template <bool BATCH>
-__global__ __launch_bounds__(1024) void k_scan_single(int* a, int m, const KmJob* jobs) {
+__global__ __launch_bounds__(1024) void k_scan_single(int* a, int m, int* total_out,
+                                                      const KmJob* jobs) {
   __shared__ int carry;
   if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
@@ -217,7 +252,10 @@ __global__ __launch_bounds__(1024) void k_scan_single(int* a, int m, const KmJob
     if (threadIdx.x == 0) carry += tot;
     __syncthreads();
   }
-  if (threadIdx.x == 0) a[m] = carry;
+  if (threadIdx.x == 0) {
+    a[m] = carry;
+    if (total_out) *total_out = carry;
+  }
 }
 
 // base_dev, when given, replaces base (an offset still on the device).
@@ -1652,7 +1690,8 @@ int compact_flags(KmeansBufs* b, int want, int base, int* out, hipStream_t st,
   const int nblk = b->nblk_flag;
   hipLaunchKernelGGL(k_flag_count<false>, dim3(nblk), dim3(256), 0, st, b->flags, b->n, want, b->bcount, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_flag_count");
-  hipLaunchKernelGGL(k_scan_single<false>, dim3(1), dim3(1024), 0, st, b->bcount, nblk, (const KmJob*)nullptr);
+  hipLaunchKernelGGL(k_scan_single<false>, dim3(1), dim3(1024), 0, st, b->bcount, nblk, (int*)nullptr,
+                     (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_scan_single");
   if (out) {
     hipLaunchKernelGGL(k_flag_scatter<false>, dim3(nblk), dim3(256), 0, st, b->flags, b->n, want,
@@ -1750,20 +1789,18 @@ int kmeans_prepare(KmeansBufs* b, const int ngrid[3], const double length[3], do
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, b->scal, (long long)KS_COUNT, 0);
   ISDF_HIP_LAUNCHED("k_fill_i32");
   unsigned long long* maxbits = (unsigned long long*)(b->scal + KS_MAXW);
-  hipLaunchKernelGGL(k_maxw<false>, dim3(std::min(cdiv(n, 256), 2048u)), dim3(256), 0, st, b->w, n,
+  hipLaunchKernelGGL(k_maxw<false>, dim3(std::min(cdiv(n, 256), 256u)), dim3(256), 0, st, b->w, n,
                      maxbits, (const KmJob*)nullptr);
   ISDF_HIP_LAUNCHED("k_maxw");
-  if (df_tol > 1e-16) {
-    hipLaunchKernelGGL(k_thresh<false>, dim3(cdiv(n, 256)), dim3(256), 0, st, b->w, n, maxbits, df_tol,
-                       b->flags, (const KmJob*)nullptr);
-  } else {
-    hipLaunchKernelGGL(k_fill_u8, dim3(std::min(cdiv(n, 256), 4096u)), dim3(256), 0, st,
-                       b->flags, (long long)n, (uint8_t)1);
-  }
-  ISDF_HIP_LAUNCHED("k_thresh");
-  if (int rc = compact_flags(b, 1, 0, nullptr, st)) return rc;
+  hipLaunchKernelGGL(k_thresh_count<false>, dim3(b->nblk_flag), dim3(256), 0, st, b->w, n,
+                     (const unsigned long long*)maxbits, df_tol, b->flags, b->bcount,
+                     (const KmJob*)nullptr);
+  ISDF_HIP_LAUNCHED("k_thresh_count");
+  hipLaunchKernelGGL(k_scan_single<false>, dim3(1), dim3(1024), 0, st, b->bcount, b->nblk_flag,
+                     b->scal + KS_NPT, (const KmJob*)nullptr);
+  ISDF_HIP_LAUNCHED("k_scan_single");
   // Without npt_host the caller assumes every grid point is a K-means point and
-  // checks npt (left in bcount[nblk_flag]) afterwards.
+  // checks npt (scal[KS_NPT]) afterwards.
   if (!npt_host) return 0;
   int npt = 0;
   ISDF_HIP_TRY(hipMemcpyAsync(&npt, b->bcount + b->nblk_flag, sizeof(int), hipMemcpyDeviceToHost,
@@ -1959,13 +1996,13 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
   ISDF_HIP_LAUNCHED("k_weights (batched)");
   // Threshold and npt (utility.cpp:646-662); npt stays in bcount[nblk_flag].
   hipLaunchKernelGGL(k_km_batch_init, dim3(1, nb), dim3(64), 0, st, J);
-  hipLaunchKernelGGL(k_maxw<true>, dim3(std::min(gn, 2048u), nb), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_maxw<true>, dim3(std::min(gn, 256u), nb), dim3(256), 0, st,
                      (const double*)nullptr, 0, (unsigned long long*)nullptr, J);
-  hipLaunchKernelGGL(k_thresh<true>, dim3(gn, nb), dim3(256), 0, st, (const double*)nullptr, 0,
-                     (const unsigned long long*)nullptr, 1e-8, (uint8_t*)nullptr, J);
-  hipLaunchKernelGGL(k_flag_count<true>, dim3(maxflag, nb), dim3(256), 0, st, (const uint8_t*)nullptr, 0,
-                     1, (int*)nullptr, J);
-  hipLaunchKernelGGL(k_scan_single<true>, dim3(1, nb), dim3(1024), 0, st, (int*)nullptr, 0, J);
+  hipLaunchKernelGGL(k_thresh_count<true>, dim3(maxflag, nb), dim3(256), 0, st,
+                     (const double*)nullptr, 0, (const unsigned long long*)nullptr, 1e-8,
+                     (uint8_t*)nullptr, (int*)nullptr, J);
+  hipLaunchKernelGGL(k_scan_single<true>, dim3(1, nb), dim3(1024), 0, st, (int*)nullptr, 0,
+                     (int*)nullptr, J);
   ISDF_HIP_LAUNCHED("threshold (batched)");
   // Points, initial centroids, brick order, first assignment (utility.cpp:682-812).
   hipLaunchKernelGGL(k_setup_points<true>, dim3(gn, nb), dim3(256), 0, st, 0, 0, (const int*)nullptr,
@@ -2025,7 +2062,8 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
                      (const int*)nullptr, (uint8_t*)nullptr, (int*)nullptr, (const int*)nullptr, J);
   hipLaunchKernelGGL(k_flag_count<true>, dim3(maxflag, nb), dim3(256), 0, st, (const uint8_t*)nullptr, 0,
                      0, (int*)nullptr, J);
-  hipLaunchKernelGGL(k_scan_single<true>, dim3(1, nb), dim3(1024), 0, st, (int*)nullptr, 0, J);
+  hipLaunchKernelGGL(k_scan_single<true>, dim3(1, nb), dim3(1024), 0, st, (int*)nullptr, 0,
+                     (int*)nullptr, J);
   hipLaunchKernelGGL(k_flag_scatter<true>, dim3(maxflag, nb), dim3(256), 0, st, (const uint8_t*)nullptr,
                      0, 0, (const int*)nullptr, 0, (int*)nullptr, (const int*)nullptr, J);
   ISDF_HIP_LAUNCHED("k-means finish (batched)");

@@ -143,7 +143,7 @@ struct Aux {
   hipEvent_t fork = nullptr, join = nullptr, pack = nullptr;
   int* hpiv = nullptr;       // pinned, hpiv_cap ints
   size_t hpiv_cap = 0;
-  long long* hscal = nullptr;  // pinned: [0] iters, [1] rku, [2] info, [3] eps bits
+  long long* hscal = nullptr;  // pinned: [0] iters, [1] rku, [2] info, [3] eps bits, [4] npt
   // The no-allocation path: s is the caller's stream, no events, and hpiv /
   // hscal point at the caller's piv and a host array (no pinned staging).
   bool fixed = false;
@@ -372,7 +372,8 @@ int basis_mixed(const isdf_problem* p, int ng, int rk, int mode, bool refine, co
 // scratch array, every stage in order on `st` (no helper stream, no events),
 // piv and the scalars copied straight to the caller's host memory at the end.
 int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, int* h_piv,
-                 hipStream_t st, isdf_report* rep, const isdf_debug* dbg, bool noalloc = false) {
+                 hipStream_t st, isdf_report* rep, const isdf_debug* dbg, bool noalloc = false,
+                 bool exact_npt = false) {
   long long ng64 = 0;
   if (int rc = validate(p, &ng64)) return rc;
   if (!d_psi || !d_theta || !h_piv) return set_error(ISDF_EINVAL, "NULL buffer argument");
@@ -506,7 +507,7 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   std::memset(h_piv, 0, sizeof(int) * (size_t)ng);
 
   int npt = 0, iters = 0, rku = rk;
-  bool degenerate = false;
+  bool degenerate = false, speculate = false;
   if (qrcp) {
     // QRCP point selection (src/common/isdf.cpp:105-109, getpoints_QRCP.cpp).
     if (int rc = ev.rec(1, st)) return rc;
@@ -528,11 +529,18 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   if (int rc = ev.rec(1, st)) return rc;
 
   // 2. K-means (utility.cpp:631-988).
-  // npt is read back (one wait: the degenerate case is handled on the host);
-  // after that nothing is read until the end of the solve.  All max_iter
-  // Lloyd iterations are queued; after convergence they return at once.
+  // npt, the number of points above the density threshold, picks the branch.
+  // It is Ng whenever no weight falls below 1e-8 of the largest: the solve
+  // assumes so and queues everything without a wait, reads npt back with its
+  // last results, and runs again with npt read first (one wait; the
+  // degenerate case is handled on the host) when the assumption was wrong.
+  // All max_iter Lloyd iterations are queued; after convergence they return
+  // at once.
   if (int rc = aux_pinned(aux, (size_t)ng)) return rc;
-  if (int rc = kmeans_prepare(&kb, p->ngrid, p->length, 1e-8, &npt, st)) return rc;
+  speculate = !exact_npt;
+  if (int rc = kmeans_prepare(&kb, p->ngrid, p->length, 1e-8, speculate ? nullptr : &npt, st))
+    return rc;
+  if (speculate) npt = ng;
   degenerate = npt < rk;
   if (degenerate && noalloc)
     return set_error(ISDF_EINVAL,
@@ -661,7 +669,14 @@ int solve_locked(const isdf_problem* p, const double* d_psi, double* d_theta, in
   }
   ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[2], info_dev, sizeof(int), hipMemcpyDeviceToHost, st));
   ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[3], eps_dev, sizeof(double), hipMemcpyDeviceToHost, st));
+  if (speculate)
+    ISDF_HIP_TRY(hipMemcpyAsync(&aux->hscal[4], kb.scal + KS_NPT, sizeof(int),
+                                hipMemcpyDeviceToHost, st));
   ISDF_HIP_TRY(hipStreamSynchronize(st));
+  // Some weights below the threshold: everything above was computed for a
+  // grid-mode K-means that does not apply.  Solve again, npt first.
+  if (speculate && (int)aux->hscal[4] != ng)
+    return solve_locked(p, d_psi, d_theta, h_piv, st, rep, dbg, noalloc, true);
   const int info = (int)aux->hscal[2];
   double eps = 0.0;
   std::memcpy(&eps, &aux->hscal[3], sizeof(double));
