@@ -63,6 +63,8 @@ struct KmeansBufs {
   int* porder;       // [nblk_pts * 256] point of each brick slot (-1: none)
   int* changed_blk;  // [nblk_pts] labels changed per point block
   int* bbox;         // [6 * rkmax] cluster bounding boxes in grid indices
+  int* cost;         // [rkmax] cost of each cluster's centroid sums (box positions + 2 members)
+  int* order;        // [rkmax] clusters of the next centroid sums, costliest first
   int nsb;           // super-bricks (2 x 2 x 2 bricks) of the grid
   int* slist;        // [nsb * kSuperCap] candidate centroids of each super-brick, ascending
   double* slxyz;     // [3 * nsb * kSuperCap] their coordinates (x, y, z planes)

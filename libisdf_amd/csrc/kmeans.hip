@@ -1002,7 +1002,8 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
     int n0, int n1, const int* __restrict__ label, const double* __restrict__ WL,
     const double* __restrict__ X, const double* __restrict__ Y, const double* __restrict__ Z,
     const int* __restrict__ bbox, int rkmax, double* __restrict__ sw, double* __restrict__ sx,
-    double* __restrict__ sy, double* __restrict__ sz, const int* scal, const KmJob* jobs) {
+    double* __restrict__ sy, double* __restrict__ sz, const int* scal, int* __restrict__ cost,
+    const int* __restrict__ order, const KmJob* jobs) {
   // Box positions per pass.  The batched launch has many more clusters than
   // CUs: 512 (2 per thread) keeps the member buffer at 16 KB, so eight
   // workgroups fit a CU (the sequential sums wait on memory and on the
@@ -1034,21 +1035,28 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
     sy = J.b.sy;
     sz = J.b.sz;
     scal = J.b.scal;
+    cost = nullptr;
+    order = nullptr;
   }
   if (scal[KS_ACTIVE] == 0) return;
   const int rk = scal[KS_RK];
-  const int c = blockIdx.x;
-  if (c >= rk) return;
+  if ((int)blockIdx.x >= rk) return;
+  // Workgroup -> cluster: costliest first when an order is given.
+  const int c = order ? order[blockIdx.x] : blockIdx.x;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int x0 = bbox[c], y0 = bbox[rkmax + c], z0 = bbox[2 * rkmax + c];
   const int x1 = bbox[3 * rkmax + c], y1 = bbox[4 * rkmax + c], z1 = bbox[5 * rkmax + c];
   // Wave 0 carries the four sums, one per row of 16 lanes (add32_rows).
   double acc = 0.0;
+  // Cost of this cluster for the dispatch order of the next sums: box
+  // positions plus twice the members (a member costs about two positions).
+  int box_cost = 0;
   if (x1 >= x0) {
     const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
     const int plane = bw * bh;  // box positions < Ng < 2^31
     const int total = plane * (z1 - z0 + 1);
     const int nround = (total + CH - 1) / CH;
+    box_cost = total;
     // Labels of round r: thread t takes box positions r CH + PT t .. + PT - 1
     // in grid order ((x, y, z) of the first from one division pair, then
     // stepping); g = -1 past the box.  The loads stay in flight until used.
@@ -1145,6 +1153,7 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
     __syncthreads();
     for (int r = 0; r < nround; r++) {
       const int tot_r = tot;
+      box_cost += 2 * tot_r;
 #pragma unroll
       for (int q = 0; q < PT; q++) {
         gn[q] = gf[q];
@@ -1159,6 +1168,7 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
       }
     }
   }
+  if (t == 0 && cost) cost[c] = box_cost;
   if (t == 0) sw[c] = acc;
   if (t == 16) sx[c] = acc;
   if (t == 32) sy[c] = acc;
@@ -1376,11 +1386,45 @@ __global__ __launch_bounds__(256) void k_centroid_sums(const int* __restrict__ s
 
 // Division and empty-cluster compaction, utility.cpp:831-854 (swap-with-last,
 // re-testing the slot that received the last cluster), by one block.
+// Order in which the next centroid sums take their clusters: costliest first
+// (cost as the last sums measured it, cost[map[c]] for cluster c after the
+// compaction; map null: no compaction), by a 32-bucket histogram of log2 of
+// the cost.  Any order gives the same sums; it only keeps the workgroups of
+// the largest boxes from starting last.  Block-wide; ends with a barrier.
+__device__ void order_by_cost(const int* __restrict__ cost, const int* __restrict__ map, int r,
+                              int* __restrict__ order) {
+  __shared__ int hist[32];
+  if (threadIdx.x < 32) hist[threadIdx.x] = 0;
+  __syncthreads();
+  auto bucket = [&](int c) {
+    const int v = cost[map ? map[c] : c];
+    return v > 0 ? 31 - __clz(v) : 0;
+  };
+  for (int c = threadIdx.x; c < r; c += blockDim.x) atomicAdd(&hist[bucket(c)], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {  // exclusive offsets, bucket 31 first
+    int o = 0;
+    for (int k = 31; k >= 0; k--) {
+      const int v = hist[k];
+      hist[k] = o;
+      o += v;
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < r; c += blockDim.x) order[atomicAdd(&hist[bucket(c)], 1)] = c;
+  __syncthreads();
+}
+
+// Centroid update with the empty-cluster compaction (utility.cpp:824-856):
+// new centroids from the sums, clusters that lost every point removed by
+// swap-with-last (map: new index -> old).  Also resets the cluster boxes for
+// the next assignment and, when order is given, orders the next sums.
 __device__ void update_centroids(const double* __restrict__ sw, const double* __restrict__ sx,
                                  const double* __restrict__ sy, const double* __restrict__ sz,
                                  double* __restrict__ cx, double* __restrict__ cy,
                                  double* __restrict__ cz, int* __restrict__ map, int* scal,
-                                 int* __restrict__ bbox, int rkmax) {
+                                 int* __restrict__ bbox, int rkmax, const int* __restrict__ cost,
+                                 int* __restrict__ order) {
   __shared__ int nempty, rknew;
   const int rk = scal[KS_RK];
   if (bbox)  // the next assignment rebuilds the cluster boxes
@@ -1400,6 +1444,7 @@ __device__ void update_centroids(const double* __restrict__ sw, const double* __
       cz[c] = sz[c] / d;
     }
     if (threadIdx.x == 0) scal[KS_CHANGED] = 0;
+    if (order) order_by_cost(cost, nullptr, rk, order);
     return;
   }
   if (threadIdx.x == 0) {
@@ -1429,6 +1474,7 @@ __device__ void update_centroids(const double* __restrict__ sw, const double* __
     scal[KS_RK] = r;
     scal[KS_CHANGED] = 0;
   }
+  if (order) order_by_cost(cost, map, r, order);
 }
 
 template <bool BATCH>
@@ -1439,7 +1485,8 @@ __global__ __launch_bounds__(1024) void k_update(const double* __restrict__ sw,
                                                  double* __restrict__ cx, double* __restrict__ cy,
                                                  double* __restrict__ cz, int* __restrict__ map,
                                                  int* scal, int* __restrict__ bbox, int rkmax,
-                                                 const KmJob* jobs) {
+                                                 const int* __restrict__ cost,
+                                                 int* __restrict__ order, const KmJob* jobs) {
   if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     sw = J.b.sw;
@@ -1453,9 +1500,10 @@ __global__ __launch_bounds__(1024) void k_update(const double* __restrict__ sw,
     scal = J.b.scal;
     bbox = J.b.bbox;
     rkmax = J.b.rkmax;
+    order = nullptr;
   }
   if (scal[KS_ACTIVE] == 0) return;
-  update_centroids(sw, sx, sy, sz, cx, cy, cz, map, scal, bbox, rkmax);
+  update_centroids(sw, sx, sy, sz, cx, cy, cz, map, scal, bbox, rkmax, cost, order);
 }
 
 // Loop control, utility.cpp:813 and :906-921, by one block: the changed-label
@@ -1499,14 +1547,14 @@ __global__ __launch_bounds__(1024) void k_control_update(
     const double* __restrict__ sw, const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ sz, double* __restrict__ cx, double* __restrict__ cy,
     double* __restrict__ cz, int* __restrict__ map, int* scal, int* __restrict__ bbox,
-    int rkmax) {
+    int rkmax, const int* __restrict__ cost, int* __restrict__ order) {
   __shared__ int go;
   if (scal[KS_ACTIVE] == 0) return;
   loop_control(s - 1, max_iter, maxF, nblk, changed_blk, trace, scal);
   if (threadIdx.x == 0) go = scal[KS_ACTIVE];
   __syncthreads();
   if (!go) return;
-  update_centroids(sw, sx, sy, sz, cx, cy, cz, map, scal, bbox, rkmax);
+  update_centroids(sw, sx, sy, sz, cx, cy, cz, map, scal, bbox, rkmax, cost, order);
 }
 
 // findMin(Dim=0), utility.cpp:926-945: per centroid, the first point at the
@@ -1763,6 +1811,8 @@ void kmeans_carve(KmeansBufs* b, void* basep, int n, const int ngrid[3], int rkm
     b->changed_blk = (int*)take(sizeof(int) * (size_t)blocks);
   }
   b->bbox = (int*)take(sizeof(int) * 6 * (size_t)rkmax);
+  b->cost = (int*)take(sizeof(int) * (size_t)rkmax);
+  b->order = (int*)take(sizeof(int) * (size_t)rkmax);
   b->nsb = ((ngrid[0] + 15) / 16) * ((ngrid[1] + 15) / 16) * ((ngrid[2] + 7) / 8);
   // Super-brick candidate lists, used from rk = kSuperMinRk on.
   const bool super = rkmax >= kSuperMinRk;
@@ -1873,15 +1923,20 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
   // Per iteration s: the sums, then the loop control of iteration s-1 with the
   // centroid update of iteration s (one launch), then the assignment.
   int last = 0;
+  // Grid mode: each centroid-sum launch records its clusters' costs, and the
+  // update after it orders the next launch's clusters by them.
+  int* const cost = b->grid_mode ? b->cost : nullptr;
   for (int s = 0; s < max_iter; s++) {
+    const int* order = (b->grid_mode && s > 0) ? b->order : nullptr;
     if (b->grid_mode && s < kWideSumsIters) {
       hipLaunchKernelGGL((k_centroid_sums_box<false, 1024>), dim3(rk), dim3(1024), 0, st,
                          b->ngrid[0], b->ngrid[1], b->label, b->WL, b->X, b->Y, b->Z, b->bbox,
-                         b->rkmax, b->sw, b->sx, b->sy, b->sz, b->scal, (const KmJob*)nullptr);
+                         b->rkmax, b->sw, b->sx, b->sy, b->sz, b->scal, cost, order,
+                         (const KmJob*)nullptr);
     } else if (b->grid_mode) {
       hipLaunchKernelGGL(k_centroid_sums_box<false>, dim3(rk), dim3(256), 0, st, b->ngrid[0],
                          b->ngrid[1], b->label, b->WL, b->X, b->Y, b->Z, b->bbox, b->rkmax, b->sw,
-                         b->sx, b->sy, b->sz, b->scal, (const KmJob*)nullptr);
+                         b->sx, b->sy, b->sz, b->scal, cost, order, (const KmJob*)nullptr);
     } else {
       hipLaunchKernelGGL(k_sort_hist, dim3(b->nchunk_sort), dim3(256), lds_rk, st, n, b->label,
                          b->hist, b->scal);
@@ -1898,12 +1953,13 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
     if (s == 0)
       hipLaunchKernelGGL(k_update<false>, dim3(1), dim3(1024), 0, st, b->sw, b->sx, b->sy, b->sz,
                          b->cx, b->cy, b->cz, b->map, b->scal,
-                         b->grid_mode ? b->bbox : (int*)nullptr, b->rkmax, (const KmJob*)nullptr);
+                         b->grid_mode ? b->bbox : (int*)nullptr, b->rkmax, cost,
+                         cost ? b->order : (int*)nullptr, (const KmJob*)nullptr);
     else
       hipLaunchKernelGGL(k_control_update, dim3(1), dim3(1024), 0, st, s, max_iter, maxF,
                          b->nblk_pts, b->changed_blk, b->trace, b->sw, b->sx, b->sy, b->sz, b->cx,
                          b->cy, b->cz, b->map, b->scal, b->grid_mode ? b->bbox : (int*)nullptr,
-                         b->rkmax);
+                         b->rkmax, cost, cost ? b->order : (int*)nullptr);
     last = s;
     // Grid mode: the super-brick candidate lists first (the bricks cull from
     // them).  The first iterations keep the wide candidate list; their
@@ -2027,11 +2083,11 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
                        (const int*)nullptr, (const double*)nullptr, (const double*)nullptr,
                        (const double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0,
                        (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr,
-                       (const int*)nullptr, J);
+                       (const int*)nullptr, (int*)nullptr, (const int*)nullptr, J);
     hipLaunchKernelGGL(k_update<true>, dim3(1, nb), dim3(1024), 0, st, (const double*)nullptr,
                        (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                        (double*)nullptr, (double*)nullptr, (double*)nullptr, (int*)nullptr,
-                       (int*)nullptr, (int*)nullptr, 0, J);
+                       (int*)nullptr, (int*)nullptr, 0, (const int*)nullptr, (int*)nullptr, J);
     if (s == 0)
       hipLaunchKernelGGL((k_assign_culled<true, kCandCapWide, kBrickPZ>), dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
