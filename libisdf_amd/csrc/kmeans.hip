@@ -418,12 +418,14 @@ __device__ double block_min256(double v, double* red) {
 // per brick.  A brick with more takes its candidates in windows of that size.
 constexpr int kCandCap = 256, kCandCapWide = 1024;
 // Grid points per thread of a grid-mode assignment brick (8 x 8 x 4 kBrickPZ
-// points per workgroup of 256): one cull then serves twice as many points,
-// and a 96^3 grid fits the chip in one round of workgroups.
+// points per workgroup of 256).  Two (one cull for twice the points, a 96^3
+// grid in one round of workgroups) measured slower at 64^3 and 96^3: the
+// deeper bricks keep more candidates.
 constexpr int kBrickPZ = 1;
-// Lloyd iterations whose centroid sums use 1024-thread workgroups (the
-// initial centroids, the first rk grid points, leave clusters of up to ~1e5
-// points for the first few updates).
+// Lloyd iterations whose centroid sums use 1024-thread workgroups, 15 producer
+// waves (the initial centroids, the first rk grid points, leave clusters of up
+// to ~1e5 points for the first few updates); later ones use 320 threads, 4
+// producer waves.
 constexpr int kWideSumsIters = 4;
 // Smallest rk for which the bricks cull from super-brick lists: below it a
 // brick's own pass over every centroid costs less than the extra launch
@@ -991,34 +993,28 @@ __device__ __forceinline__ void add32_rows(double& acc, double lo, double hi, do
 
 // Centroid sums, utility.cpp:816-822, when every grid point is a K-means point
 // (local index == grid index): one workgroup per cluster walks the cluster's
-// bounding box in ascending grid order (z, y, x), 1024 positions per round.
-// Thread t tests positions 4t..4t+3 of the round, members are compacted in
-// position order through a block scan, and their products w*x (one rounding
-// each, as in the reference) land in LDS; threads 0..3 then carry the four
-// sequential sums over the round's members in that order, while the labels
-// and terms of the next rounds load.
-template <bool BATCH, int NT = 256>
-__global__ __launch_bounds__(NT) void k_centroid_sums_box(
+// bounding box in ascending grid order (z, y, x) and adds the terms w, w x,
+// w y, w z of the points carrying its label in that order: the reference's
+// sequence of roundings (it adds in point order) without sorting.
+// Wave 0 is the adder (add32_rows: the four sums in its four rows); waves
+// 1..NP produce: a round of CH = NP 64 PT box positions, producer wave pw
+// taking positions pw 64 PT .. of it, PT consecutive ones per lane.  While the
+// adder adds round r from LDS, the producers load and rank round r + 1 (labels
+// two rounds ahead, member terms one round ahead); two barriers per round
+// hand the buffer over.
+template <bool BATCH, int NP, int PT>
+__global__ __launch_bounds__(64 * (NP + 1)) void k_centroid_sums_box(
     int n0, int n1, const int* __restrict__ label, const double* __restrict__ WL,
     const double* __restrict__ X, const double* __restrict__ Y, const double* __restrict__ Z,
     const int* __restrict__ bbox, int rkmax, double* __restrict__ sw, double* __restrict__ sx,
     double* __restrict__ sy, double* __restrict__ sz, const int* scal, int* __restrict__ cost,
     const int* __restrict__ order, const KmJob* jobs) {
-  // Box positions per pass.  The batched launch has many more clusters than
-  // CUs: 512 (2 per thread) keeps the member buffer at 16 KB, so eight
-  // workgroups fit a CU (the sequential sums wait on memory and on the
-  // previous add; more resident workgroups hide it): 24 uneven instances
-  // 1468 -> 1506 solves/s.  One instance (rk up to a few per CU) keeps 1024.
-  // NT = 1024 (4096 positions per pass, 128 KB of LDS) serves the first Lloyd
-  // iterations, whose clusters still hold tens of thousands of points: their
-  // boxes need a quarter of the passes, and each pass costs a scan and two
-  // barriers whatever its size.
-  constexpr int CH = BATCH ? 512 : 4 * NT, PT = CH / NT;
+  constexpr int CH = NP * 64 * PT;
   static_assert(CH % 32 == 0, "rounds are whole batches of 32 terms");
   // Row q: term q (w, w x, w y, w z) of the round's members in grid order, then
   // zeros up to the next multiple of 32 (the adder takes whole batches).
   __shared__ __align__(16) double buf[4][CH];
-  __shared__ __align__(16) int wsum[NT / 64];
+  __shared__ __align__(16) int wsum[NP];
   if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     n0 = J.b.ngrid[0];
@@ -1044,10 +1040,12 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
   // Workgroup -> cluster: costliest first when an order is given.
   const int c = order ? order[blockIdx.x] : blockIdx.x;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const bool adder = wave == 0;
+  const int pw = wave - 1;              // producer wave (adder: -1)
+  const int pt = t - 64;                // producer thread (adder: < 0)
   const int x0 = bbox[c], y0 = bbox[rkmax + c], z0 = bbox[2 * rkmax + c];
   const int x1 = bbox[3 * rkmax + c], y1 = bbox[4 * rkmax + c], z1 = bbox[5 * rkmax + c];
-  // Wave 0 carries the four sums, one per row of 16 lanes (add32_rows).
-  double acc = 0.0;
+  double acc = 0.0;  // the adder's sums, one per row
   // Cost of this cluster for the dispatch order of the next sums: box
   // positions plus twice the members (a member costs about two positions).
   int box_cost = 0;
@@ -1057,11 +1055,11 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
     const int total = plane * (z1 - z0 + 1);
     const int nround = (total + CH - 1) / CH;
     box_cost = total;
-    // Labels of round r: thread t takes box positions r CH + PT t .. + PT - 1
-    // in grid order ((x, y, z) of the first from one division pair, then
+    // Labels of round r (producers): positions r CH + PT pt .. + PT - 1 in
+    // grid order ((x, y, z) of the first from one division pair, then
     // stepping); g = -1 past the box.  The loads stay in flight until used.
     auto load_labels = [&](int r, int* g, int* l) {
-      const int j = r * CH + PT * t;
+      const int j = r * CH + PT * pt;
       int z = j / plane, rr = j - z * plane;
       int y = rr / bw, x = rr - y * bw;
 #pragma unroll
@@ -1096,6 +1094,31 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
         }
       }
     };
+    // Rank inside the producer wave (ballots; slots in position order) and
+    // the wave's count to LDS; the offset over waves is read after a barrier.
+    int below = 0;
+    auto rank = [&]() {
+      int wtot = 0;
+      below = 0;
+#pragma unroll
+      for (int q = 0; q < PT; q++) {
+        const unsigned long long m = __ballot(mem[q]);
+        below += __popcll(m & lanemask_lt());
+        wtot += __popcll(m);
+      }
+      if (lane == 0) wsum[pw] = wtot;
+    };
+    auto round_total = [&](int* off) {  // after the barrier that follows rank()
+      int o = 0, tot = 0;
+#pragma unroll
+      for (int w = 0; w < NP; w++) {
+        const int v = wsum[w];
+        o += (w < pw) ? v : 0;
+        tot += v;
+      }
+      *off = o;
+      return tot;
+    };
     // Members to LDS in position order; products w*x (one rounding each, as
     // the reference forms them).  Then +0.0 up to the next multiple of 32 (at
     // most CH): a sum that starts at +0.0 never becomes -0.0, so adding +0.0
@@ -1111,14 +1134,14 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
           pos++;
         }
       }
-      if (t < 128) {
-        const int i = tot_round + (t & 31);
-        if (i < ((tot_round + 31) & ~31)) buf[t >> 5][i] = 0.0;
+      if (pt < 128) {
+        const int i = tot_round + (pt & 31);
+        if (i < ((tot_round + 31) & ~31)) buf[pt >> 5][i] = 0.0;
       }
     };
-    // The sums over one round's members in order, by wave 0: batches of 32
-    // terms, each one LDS read per lane (two terms of its row) and 32 DPP
-    // steps; the reads run two batches ahead.
+    // The adder: one round's members in order, batches of 32 terms, each one
+    // LDS read per lane (two terms of its row) and 32 DPP steps; the reads run
+    // two batches ahead.
     const double one = 1.0;
     auto add_round = [&](int tot_round) {
       const int nb = (tot_round + 31) >> 5;
@@ -1127,44 +1150,55 @@ __global__ __launch_bounds__(NT) void k_centroid_sums_box(
       // Batch j (clamped: past the last batch it is read again, not added).
       auto rd = [&](int j) { return *(const double2*)(row + 32 * min(j, nb - 1)); };
       // Three register sets in rotation (unrolled, so no copies between them).
-      double2 a = rd(0), b = rd(1), c;
+      double2 a = rd(0), b = rd(1), cc;
       for (int j = 0;; j += 3) {
-        c = rd(j + 2);
+        cc = rd(j + 2);
         add32_rows(acc, a.x, a.y, one);
         if (j + 1 >= nb) break;
         a = rd(j + 3);
         add32_rows(acc, b.x, b.y, one);
         if (j + 2 >= nb) break;
         b = rd(j + 4);
-        add32_rows(acc, c.x, c.y, one);
+        add32_rows(acc, cc.x, cc.y, one);
         if (j + 3 >= nb) break;
       }
     };
-    // Software pipeline over rounds: labels two rounds ahead, member terms one
-    // round ahead, so wave 0 adds round r while the loads of rounds r+1 and
-    // r+2 are in flight.  The scan of round r+1 waits for those adds (its
-    // barrier) before round r+1 overwrites the buffer.
-    int gn[PT], ln[PT], gf[PT], lf[PT];  // round r+1 (next), round r+2 (far)
-    load_labels(0, gn, ln);
-    if (nround > 1) load_labels(1, gf, lf);
-    int tot;
-    gather(gn, ln);
-    store(rank_members<NT, PT>(mem, &tot, wsum), tot);
+    int gn[PT], ln[PT], gf[PT], lf[PT];  // producers: round r+1 (next), round r+2 (far)
+    int tot = 0;
+    if (!adder) {
+      load_labels(0, gn, ln);
+      if (nround > 1) load_labels(1, gf, lf);
+      gather(gn, ln);
+      rank();
+    }
+    __syncthreads();
+    {
+      int off;
+      tot = round_total(&off);
+      if (!adder) store(off + below, tot);
+    }
     __syncthreads();
     for (int r = 0; r < nround; r++) {
       const int tot_r = tot;
       box_cost += 2 * tot_r;
+      if (adder) {
+        add_round(tot_r);
+      } else if (r + 1 < nround) {
 #pragma unroll
-      for (int q = 0; q < PT; q++) {
-        gn[q] = gf[q];
-        ln[q] = lf[q];
+        for (int q = 0; q < PT; q++) {
+          gn[q] = gf[q];
+          ln[q] = lf[q];
+        }
+        if (r + 2 < nround) load_labels(r + 2, gf, lf);
+        gather(gn, ln);
+        rank();
       }
-      if (r + 2 < nround) load_labels(r + 2, gf, lf);
-      if (r + 1 < nround) gather(gn, ln);
-      if (wave == 0) add_round(tot_r);
+      __syncthreads();  // the adder is done with round r; round r+1 is ranked
       if (r + 1 < nround) {
-        store(rank_members<NT, PT>(mem, &tot, wsum), tot);
-        __syncthreads();
+        int off;
+        tot = round_total(&off);
+        if (!adder) store(off + below, tot);
+        __syncthreads();  // round r+1 is in the buffer
       }
     }
   }
@@ -1929,12 +1963,12 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
   for (int s = 0; s < max_iter; s++) {
     const int* order = (b->grid_mode && s > 0) ? b->order : nullptr;
     if (b->grid_mode && s < kWideSumsIters) {
-      hipLaunchKernelGGL((k_centroid_sums_box<false, 1024>), dim3(rk), dim3(1024), 0, st,
+      hipLaunchKernelGGL((k_centroid_sums_box<false, 15, 4>), dim3(rk), dim3(1024), 0, st,
                          b->ngrid[0], b->ngrid[1], b->label, b->WL, b->X, b->Y, b->Z, b->bbox,
                          b->rkmax, b->sw, b->sx, b->sy, b->sz, b->scal, cost, order,
                          (const KmJob*)nullptr);
     } else if (b->grid_mode) {
-      hipLaunchKernelGGL(k_centroid_sums_box<false>, dim3(rk), dim3(256), 0, st, b->ngrid[0],
+      hipLaunchKernelGGL((k_centroid_sums_box<false, 4, 4>), dim3(rk), dim3(320), 0, st, b->ngrid[0],
                          b->ngrid[1], b->label, b->WL, b->X, b->Y, b->Z, b->bbox, b->rkmax, b->sw,
                          b->sx, b->sy, b->sz, b->scal, cost, order, (const KmJob*)nullptr);
     } else {
@@ -2079,7 +2113,7 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
   ISDF_HIP_LAUNCHED("k-means setup (batched)");
   // Lloyd iterations (utility.cpp:812-922); converged instances return at once.
   for (int s = 0; s < maxit; s++) {
-    hipLaunchKernelGGL(k_centroid_sums_box<true>, dim3(maxrk, nb), dim3(256), 0, st, 0, 0,
+    hipLaunchKernelGGL((k_centroid_sums_box<true, 4, 2>), dim3(maxrk, nb), dim3(320), 0, st, 0, 0,
                        (const int*)nullptr, (const double*)nullptr, (const double*)nullptr,
                        (const double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0,
                        (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr,
