@@ -13,8 +13,10 @@
 // per grid point, Rt[g][c] = sqrt(V) Re, Rt[Ng + g][c] = sqrt(V) Im, so that
 // both Gram matrices are plain FP64 MFMA products Rt^T Rt over K = 2 Ng (split
 // over K into slices summed in a fixed order).  The DFT is separable: one pass
-// per axis, each output a direct sum over its line with a twiddle table (the
-// check is a validator; its cost is the (K1 K2)^2 Ng Gram, not the DFT).
+// per axis, a mixed-radix Stockham FFT of whole lines in LDS (k_fft_axis;
+// an axis whose length has a prime factor above 13, or whose lines would need
+// more than 64 KB of LDS, falls back to a direct sum over its line,
+// k_dft_axis), with the host validator's twiddle table.
 // Every sum has a fixed order, so the result is deterministic.
 
 #include <hip/hip_runtime.h>
@@ -24,6 +26,7 @@
 #include <cstdlib>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/isdf_hip.h"
@@ -125,6 +128,203 @@ __global__ __launch_bounds__(256) void k_dft_axis(const double2* __restrict__ in
     out[e] = make_double2(re, im);
   }
 }
+
+// Radices of a mixed-radix FFT of length n: fours, a two, then odd primes up to
+// kMaxRadix (n with a larger prime factor: count = -1, the direct DFT above
+// serves it).
+constexpr int kMaxRadix = 13, kMaxPass = 16;
+// LDS of one FFT workgroup at most (the default dynamic LDS limit); a longer
+// axis takes the direct DFT.
+constexpr size_t kFftMaxLds = 65536;
+struct FftPlan {
+  int count;
+  int radix[kMaxPass];
+};
+
+// One Stockham pass of radix R over TL lines of length n held in LDS rows of
+// ld entries (A -> B); Ns is the product of the earlier radices.  Butterfly j
+// of a line takes a[j + r n/R] times w^{(j mod Ns) r n/(Ns R)} (w = T[1] =
+// e^{-2 pi i/n}), transforms the R values (radix 2 and 4 by their closed
+// forms, odd R as a direct length-R DFT with w_R = w^{n/R}) and writes them to
+// b[(j / Ns) Ns R + j mod Ns + r Ns].  R is a template argument, so the R
+// values stay in registers.
+template <int R>
+__device__ __forceinline__ void fft_pass(const double2* A, double2* B, const double2* T, int TL,
+                                         int ld, int n, int Ns) {
+  const int nR = n / R, step = n / (Ns * R);
+  for (int w = threadIdx.x; w < TL * nR; w += 256) {
+    const int li = w / nR, j = w - li * nR, jj = j % Ns;
+    const double2* a = A + li * ld;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const double2 x = a[j + r * nR];
+      const double2 c = T[(jj * r * step) % n];
+      v[r] = make_double2(x.x * c.x - x.y * c.y, x.x * c.y + x.y * c.x);
+    }
+    double2 o[R];
+    if constexpr (R == 2) {
+      o[0] = make_double2(v[0].x + v[1].x, v[0].y + v[1].y);
+      o[1] = make_double2(v[0].x - v[1].x, v[0].y - v[1].y);
+    } else if constexpr (R == 4) {  // w_4 = -i
+      const double2 s0 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
+      const double2 d0 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
+      const double2 s1 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
+      const double2 d1 = make_double2(v[1].y - v[3].y, v[3].x - v[1].x);  // -i (v1 - v3)
+      o[0] = make_double2(s0.x + s1.x, s0.y + s1.y);
+      o[2] = make_double2(s0.x - s1.x, s0.y - s1.y);
+      o[1] = make_double2(d0.x + d1.x, d0.y + d1.y);
+      o[3] = make_double2(d0.x - d1.x, d0.y - d1.y);
+    } else {
+#pragma unroll
+      for (int k = 0; k < R; k++) {
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const double2 c = T[((k * r) % R) * nR];
+          re += v[r].x * c.x - v[r].y * c.y;
+          im += v[r].x * c.y + v[r].y * c.x;
+        }
+        o[k] = make_double2(re, im);
+      }
+    }
+    double2* b = B + li * ld + (j / Ns) * Ns * R + jj;
+#pragma unroll
+    for (int r = 0; r < R; r++) b[r * Ns] = o[r];
+  }
+}
+
+// One axis of the DFT for nb columns as a Stockham autosort FFT in LDS (the
+// same transform as k_dft_axis: out_k = sum_t in_t e^{-2 pi i k t / n}).  A
+// workgroup loads TL whole lines (element t of line l at base(l) + t stride),
+// runs the passes between two LDS buffers (pass p: radix R, Ns the product of
+// the earlier radices; butterfly j takes in[j + r n/R] times the twiddle
+// w^{(j mod Ns) r n/(Ns R)}, transforms them, and writes out[(j / Ns) Ns R +
+// j mod Ns + r Ns]) and writes the lines back; LAST as in k_dft_axis.  The
+// twiddles w^q = tw[q] are the host validator's table.
+template <int AXIS, bool LAST>
+__global__ __launch_bounds__(256) void k_fft_axis(const double2* __restrict__ in,
+                                                  double2* __restrict__ out, int n0, int n1,
+                                                  int n2, int nb, const double2* __restrict__ tw,
+                                                  FftPlan plan, int TL, double scale,
+                                                  const double* __restrict__ sv,
+                                                  double* __restrict__ Rt, long long ncol,
+                                                  long long c0) {
+  extern __shared__ double2 sm[];
+  const long long ng = (long long)n0 * n1 * n2;
+  const int n = AXIS == 0 ? n0 : (AXIS == 1 ? n1 : n2);
+  const int ld = n + 1;  // LDS row of a line, padded
+  double2* A = sm;
+  double2* B = sm + (size_t)TL * ld;
+  double2* T = sm + 2 * (size_t)TL * ld;
+  const long long lines_per_col = ng / n;
+  const long long nlines = lines_per_col * nb;
+  const long long l0 = (long long)blockIdx.x * TL;
+  // Line l (x fastest among the other two axes, then the column): its
+  // first element and the stride along the axis.
+  auto line_at = [&](long long l, long long* base, long long* stride, long long* col) {
+    const long long c = l / lines_per_col, r = l % lines_per_col;
+    *col = c;
+    if constexpr (AXIS == 0) {
+      *base = c * ng + r * n0;
+      *stride = 1;
+    } else if constexpr (AXIS == 1) {
+      const long long x = r % n0, z = r / n0;
+      *base = c * ng + z * (long long)n0 * n1 + x;
+      *stride = n0;
+    } else {
+      *base = c * ng + r;  // r = x + n0 y
+      *stride = (long long)n0 * n1;
+    }
+  };
+  // Element e of the tile: along the line fastest for axis 0 (contiguous),
+  // across lines fastest otherwise (adjacent lines are adjacent in memory).
+  auto tile_elem = [&](int e, int* li, int* t) {
+    if constexpr (AXIS == 0) {
+      *li = e / n;
+      *t = e - *li * n;
+    } else {
+      *t = e / TL;
+      *li = e - *t * TL;
+    }
+  };
+  for (int q = threadIdx.x; q < n; q += 256) T[q] = tw[q];
+  for (int e = threadIdx.x; e < TL * n; e += 256) {
+    int li, t;
+    tile_elem(e, &li, &t);
+    const long long l = l0 + li;
+    if (l >= nlines) continue;
+    long long base, stride, col;
+    line_at(l, &base, &stride, &col);
+    A[li * ld + t] = in[base + t * stride];
+  }
+  __syncthreads();
+  int Ns = 1;
+  for (int p = 0; p < plan.count; p++) {
+    const int R = plan.radix[p];
+    switch (R) {
+      case 2: fft_pass<2>(A, B, T, TL, ld, n, Ns); break;
+      case 3: fft_pass<3>(A, B, T, TL, ld, n, Ns); break;
+      case 4: fft_pass<4>(A, B, T, TL, ld, n, Ns); break;
+      case 5: fft_pass<5>(A, B, T, TL, ld, n, Ns); break;
+      case 7: fft_pass<7>(A, B, T, TL, ld, n, Ns); break;
+      case 11: fft_pass<11>(A, B, T, TL, ld, n, Ns); break;
+      default: fft_pass<13>(A, B, T, TL, ld, n, Ns); break;
+    }
+    __syncthreads();
+    double2* s = A;
+    A = B;
+    B = s;
+    Ns *= R;
+  }
+  for (int e = threadIdx.x; e < TL * n; e += 256) {
+    int li, t;
+    tile_elem(e, &li, &t);
+    const long long l = l0 + li;
+    if (l >= nlines) continue;
+    long long base, stride, col;
+    line_at(l, &base, &stride, &col);
+    const long long idx = base + t * stride;
+    const double2 v = A[li * ld + t];
+    if constexpr (LAST) {
+      const long long g = idx - col * ng;
+      const double f = scale * sv[g];
+      Rt[g * ncol + c0 + col] = v.x * f;
+      Rt[(ng + g) * ncol + c0 + col] = v.y * f;
+    } else {
+      out[idx] = v;
+    }
+  }
+}
+
+FftPlan fft_plan(int n) {
+  FftPlan pl{};
+  int m = n;
+  auto add = [&](int r) {
+    if (pl.count >= kMaxPass) return false;
+    pl.radix[pl.count++] = r;
+    return true;
+  };
+  while (m % 4 == 0) {
+    add(4);
+    m /= 4;
+  }
+  while (m % 2 == 0) {
+    add(2);
+    m /= 2;
+  }
+  for (int p = 3; m > 1 && p <= kMaxRadix; p += 2) {
+    while (m % p == 0) {
+      add(p);
+      m /= p;
+    }
+  }
+  if (m > 1) pl.count = -1;
+  return pl;
+}
+
+// Lines per FFT workgroup: the two tiles and the twiddles within ~48 KB of LDS.
+int fft_lines(int n) { return std::max(1, std::min(128, 1536 / (n + 1))); }
 
 // C(mu, a) = rho_a(r_mu), stored mu-major: Cm[mu * npair + a].
 __global__ __launch_bounds__(256) void k_coul_cmat(const double* __restrict__ psi, long long ld,
@@ -369,21 +569,38 @@ int coulomb_unlocked(const isdf_problem* p, const double* d_psi, const double* d
   ISDF_HIP_LAUNCHED("k_coul_sqrtv");
 
   // Spectra of every column into Rt.
+  const FftPlan plan[3] = {fft_plan(n0), fft_plan(n1), fft_plan(n2)};
   const double scale = p->length[0] * p->length[1] * p->length[2] / (double)ng;
   for (long long c0 = 0; c0 < ncol; c0 += nb) {
     const int w = (int)std::min<long long>(nb, ncol - c0);
     const dim3 grid((unsigned)cdivl((long long)w * ng, 256));
     hipLaunchKernelGGL(k_coul_cols, grid, dim3(256), 0, st, d_psi, ld, s1, s2, K2, npair, d_theta,
                        ng, c0, w, X);
-    hipLaunchKernelGGL((k_dft_axis<0, false>), grid, dim3(256), 0, st, (const double2*)X, Y, n0,
-                       n1, n2, w, (const double2*)tw, scale, (const double*)sv, Rt, ncol, c0);
-    hipLaunchKernelGGL((k_dft_axis<1, false>), grid, dim3(256), 0, st, (const double2*)Y, X, n0,
-                       n1, n2, w, (const double2*)(tw + n0), scale, (const double*)sv, Rt, ncol,
-                       c0);
-    hipLaunchKernelGGL((k_dft_axis<2, true>), grid, dim3(256), 0, st, (const double2*)X, Y, n0,
-                       n1, n2, w, (const double2*)(tw + n0 + n1), scale, (const double*)sv, Rt,
-                       ncol, c0);
-    ISDF_HIP_LAUNCHED("k_dft_axis");
+    // Per axis: the FFT when its length factors into radices up to
+    // kMaxRadix and its lines fit the LDS bound, else the direct DFT.
+    auto axis = [&](auto ax, auto last, const double2* src, double2* dst, const double2* t) {
+      constexpr int AX = decltype(ax)::value;
+      constexpr bool LAST = decltype(last)::value;
+      const int n = p->ngrid[AX];
+      const int TL = fft_lines(n);
+      const size_t lds = sizeof(double2) * (2 * (size_t)TL * (n + 1) + n);
+      if (plan[AX].count >= 0 && lds <= kFftMaxLds) {
+        const long long nlines = (long long)w * (ng / n);
+        hipLaunchKernelGGL((k_fft_axis<AX, LAST>), dim3((unsigned)cdivl(nlines, TL)), dim3(256),
+                           lds, st, src, dst, n0, n1, n2, w, t, plan[AX], TL, scale,
+                           (const double*)sv, Rt, ncol, c0);
+      } else {
+        hipLaunchKernelGGL((k_dft_axis<AX, LAST>), grid, dim3(256), 0, st, src, dst, n0, n1, n2, w,
+                           t, scale, (const double*)sv, Rt, ncol, c0);
+      }
+    };
+    axis(std::integral_constant<int, 0>(), std::false_type(), (const double2*)X, Y,
+         (const double2*)tw);
+    axis(std::integral_constant<int, 1>(), std::false_type(), (const double2*)Y, X,
+         (const double2*)(tw + n0));
+    axis(std::integral_constant<int, 2>(), std::true_type(), (const double2*)X, Y,
+         (const double2*)(tw + n0 + n1));
+    ISDF_HIP_LAUNCHED("k_fft_axis");
   }
 
   // M = Theta spectra Gram (rk x rk); W = M C (b-major); A = C^T W.

@@ -58,6 +58,8 @@ def test_si8_device_check_matches_reference_oracle_and_host(si8):
     # dual pair sets, ragged pair and rk tiles
     ((10, 12, 9), (4.1, 5.0, 3.7), 22, 11, 9, 7, 5, 11, 100, 40),
     ((16, 14, 18), (6.0, 5.5, 7.0), 40, 20, 20, 20, 8, 20, 300, 30),
+    # radix 11 and 13 FFT axes and a prime axis (17) on the direct DFT
+    ((11, 13, 17), (4.3, 5.1, 6.2), 22, 11, 9, 7, 5, 11, 100, 40),
 ])
 def test_device_coulomb_ragged_and_planted_error(case):
     grid, L, nstate, nocc, nv1, nc1, nv2, nc2, mu, iters = case
