@@ -36,15 +36,16 @@ FP32_PEAK_TFLOPS = 157.3     # dense FP32 MFMA
 BF16_PEAK_TFLOPS = 2500.0    # dense bf16 MFMA
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per k_potrs launch from two rocprofv3 counter passes (FETCH_SIZE x 2.000,
-# WRITE_SIZE x 1.000, factors from a copy kernel of known size), per workload.
+# HBM bytes per launch of the triangular solve (k_potrs_dpp, the default) from two rocprofv3
+# counter passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, factors from a copy kernel of known
+# size), per workload.
 TRAFFIC = {
-    "cfg3": {"bytes": 2.9442e11,
-             "source": "profiles/k_potrs_k_zct_cfg3_fp64_hbm_bytes.md (265.28 GB read + "
-                       "29.15 GB written per launch; 57.98 GB algorithmic)"},
-    "cfg2": {"bytes": 2.490e10,
-             "source": "profiles/k_potrs_k_zct_cfg2_fp64_hbm_bytes.md (20.56 GB read + "
-                       "4.341 GB written per launch)"},
+    "cfg3": {"bytes": 2.9289e11,
+             "source": "profiles/k_potrs_k_zct_cfg3_fp64_hbm_bytes.md (263.90 GB read + "
+                       "28.99 GB written per launch; 57.98 GB algorithmic)"},
+    "cfg2": {"bytes": 2.422e10,
+             "source": "profiles/k_potrs_k_zct_cfg2_fp64_hbm_bytes.md (19.93 GB read + "
+                       "4.295 GB written per launch; 8.59 GB algorithmic)"},
 }
 
 
@@ -123,9 +124,12 @@ def peak_probe():
     out = {}
     for which, name, peak, iters in ((0, "v_mfma_f64_16x16x4f64", FP64_PEAK_TFLOPS, 4000),
                                      (1, "v_fma_f64", FP64_PEAK_TFLOPS, 20000),
-                                     (2, "v_mfma_f32_16x16x4f32", FP32_PEAK_TFLOPS, 8000)):
+                                     (2, "v_mfma_f32_16x16x4f32", FP32_PEAK_TFLOPS, 8000),
+                                     (5, "v_fmac_f64_dpp row_newbcast (64 accumulators)",
+                                      FP64_PEAK_TFLOPS, 5000)):
         res = []
-        for blocks in (256 * 4, 256 * 8):
+        # the DPP loop holds 138 VGPRs (3 waves per SIMD at most): also 1, 2, 3 waves
+        for blocks in ((256, 512, 768, 256 * 4) if which == 5 else (256 * 4, 256 * 8)):
             tf, ms = ctypes.c_double(), ctypes.c_double()
             rc = lib.isdf_probe_peak(which, blocks, iters, ctypes.byref(tf), ctypes.byref(ms))
             if rc != 0:
@@ -152,6 +156,42 @@ def peak_probe():
         raise RuntimeError("add-chain probe failed: hipError %d" % rc)
     out["v_add_f64_dependent_chain"] = {"ns_per_add": 1e3 / (tf.value * 1e6), "ms": ms.value}
     print(json.dumps({"peak_probe": out}))
+
+
+def compare_solve_tiles(a):
+    """FP64 solves of one workload with each listed triangular-solve tile, interleaved
+    in one process (so that every tile sees the same clock and thermal history)."""
+    import torch
+    import libisdf_amd as isdf
+    from libisdf_amd import synth
+    w = synth.WORKLOADS[a.workload]
+    psi = synth.wavefunctions(w)
+    rk = isdf.isdf_rank(w.nv1, w.nc1, w.nv2, w.nc2, w.mu_points)
+    d_psi = torch.from_numpy(psi).cuda()
+    d_theta = torch.empty((rk, w.ng), dtype=torch.float64, device="cuda")
+    first = None
+    saved = os.environ.get("ISDF_POTRS_TILE")
+    try:
+        for t in [int(x) for x in a.compare_solve_tiles.split(",")]:
+            os.environ["ISDF_POTRS_TILE"] = str(t)
+            piv = np.zeros(w.ng, np.int32)
+            rep = isdf.solve(d_psi, isdf.Domain(w.length, w.ngrid), w.nocc, w.nstate, w.nv1,
+                             w.nc1, w.nv2, w.nc2, w.mu_points, d_theta, piv, w.kmeans_iters)
+            torch.cuda.synchronize()
+            if first is None:
+                first = d_theta.clone()
+                same = True
+            else:
+                same = bool(torch.equal(d_theta, first))
+            print(json.dumps({"workload": w.name, "tile": t, "potrs_ms": rep["ms_potrs"],
+                              "potrs_tflops": 2.0 * w.ng * rk * rk / rep["ms_potrs"] / 1e9,
+                              "total_ms": rep["ms_total"], "theta_bitwise_equal_first": same}),
+                  flush=True)
+    finally:
+        if saved is None:
+            os.environ.pop("ISDF_POTRS_TILE", None)
+        else:
+            os.environ["ISDF_POTRS_TILE"] = saved
 
 
 def meets_target(m, fit64):
@@ -184,6 +224,11 @@ def main():
                     help="interpolation-point selection for --profile-solve")
     ap.add_argument("--peak-probe", action="store_true",
                     help="run only register-only throughput loops of the FP64 / FP32 units")
+    ap.add_argument("--compare-solve-tiles", default=None, metavar="LIST",
+                    help="instead of the benchmark: FP64 solves of the workload, one per "
+                         "triangular-solve tile in the comma-separated list (ISDF_POTRS_TILE), "
+                         "in that order in one process; prints each solve's time and whether "
+                         "Theta equals the first solve's bit for bit")
     a = ap.parse_args()
     if a.peak_probe:
         return peak_probe()
@@ -212,6 +257,8 @@ def main():
         return bench_batch(a)
     if a.check:
         return bench_check(a)
+    if a.compare_solve_tiles:
+        return compare_solve_tiles(a)
     if a.profile_solve:
         w = synth.WORKLOADS[a.workload]
         psi = synth.wavefunctions(w)
@@ -307,7 +354,9 @@ def main():
                               "branch (ISDF_ZCT_RESERVE_CUS=0); zct_tflops is the in-solve rate"},
         "stage_ms": {k[3:]: round(v, 4) for k, v in avg.items()},
         "roofline": {
-            "kernel": "k_potrs (blocked in-place triangular solves, FP64 MFMA)",
+            "kernel": "k_potrs_dpp (blocked in-place triangular solves on the FP64 vector FMA "
+                      "pipe, B operand broadcast by DPP; compute-bound: the published FP64 "
+                      "vector and matrix peaks are both 78.6 TFLOP/s)",
             "bound": "mfma",
             "achieved": potrs_tf,
             "peak": FP64_PEAK_TFLOPS,

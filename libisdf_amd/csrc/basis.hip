@@ -767,6 +767,122 @@ __global__ __launch_bounds__(XxtTile::NT, 1) void k_xxt(const double* __restrict
   });
 }
 
+// The in-place solve of k_potrs on the vector FMA pipe, with operands staged
+// by LDS DMA and the B operand broadcast by DPP (dpp_fma16).  A workgroup owns
+// BM = 64 TM rows and walks the block columns as k_potrs does; its NW waves
+// split the 128 columns of a block into groups of 16 TG, each wave taking
+// every row of the strip: lane l holds rows 64 i + l (i < TM) and the columns
+// of its groups.  Per k step a lane reads TM A values and TG B values from LDS
+// (B at column 16 g + l % 16, so each row of 16 lanes holds a group) and runs
+// 16 TM TG FMAs.
+//
+// Staging: global_load_lds_dwordx4 writes 64 lanes x 16 bytes = 128 doubles
+// to LDS from a wave-uniform base, so a chunk of KB k values is KB BM / 128
+// copies of A column pieces and KB copies of B rows, with no VGPRs or LDS
+// stores spent on it; the copies of chunk c + 1 run during the FMAs of chunk
+// c (double buffer, one barrier per chunk).
+// Requires: ng a multiple of BM (no row tail: every strip is whole), every K
+// a multiple of KB (n a multiple of KB), 16-byte aligned rows (ldt even, T
+// and B 16-byte aligned).  Columns of a B row past the block width are read
+// (the F/G allocations carry one block width of padding) and only feed
+// output columns that are not stored.
+template <int TM, int TG, int KB, int WPE>
+__global__ __launch_bounds__(64 * (kSB / (16 * TG)), WPE) void k_potrs_dpp(
+    double* __restrict__ T, long long ldt, int n, const double* __restrict__ Ft,
+    const double* __restrict__ Gt) {
+  constexpr int BM = 64 * TM, BN = kSB, NW = BN / (16 * TG);
+  constexpr int ACOPY = KB * BM / 128, NCOPY = ACOPY + KB;  // 1 KB copies per chunk
+  constexpr int CHUNK = KB * (BM + BN);                       // doubles per buffer
+  static_assert(BM % 128 == 0 && NCOPY % NW == 0, "copies must split over the waves");
+  __shared__ double lds[2 * CHUNK];
+  typedef __attribute__((address_space(3))) void* LdsPtr;
+  typedef __attribute__((address_space(1))) void* GlobalPtr;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double* Tm = T + (long long)blockIdx.x * BM;
+  const int nJ = (n + kSB - 1) / kSB;
+  for (int pass = 0; pass < 2; pass++) {
+    for (int s = 0; s < nJ; s++) {
+      const int J = pass == 0 ? s : nJ - 1 - s;
+      const int J0 = J * kSB;
+      const int jb = min(kSB, n - J0);
+      const int kbeg = pass == 0 ? 0 : J0;
+      const int K = (pass == 0 ? J0 + jb : n) - kbeg;
+      const double* A = Tm + (long long)kbeg * ldt;
+      const double* B = (pass == 0 ? Ft : Gt) + (long long)kbeg * n + J0;
+      const int nch = K / KB;
+      // Copy q of this wave for chunk c into buffer `buf`.
+      auto issue = [&](int c, int buf) {
+        double* As = lds + buf * CHUNK;
+        double* Bs = As + KB * BM;
+#pragma unroll
+        for (int q = 0; q < NCOPY / NW; q++) {
+          const int idx = wave + q * NW;
+          const double* src;
+          double* dst;
+          if (idx < ACOPY) {
+            const int col = idx / (BM / 128), part = idx % (BM / 128);
+            src = A + (long long)(c * KB + col) * ldt + part * 128;
+            dst = As + col * BM + part * 128;
+          } else {
+            const int kk = idx - ACOPY;
+            src = B + (long long)(c * KB + kk) * n;
+            dst = Bs + kk * BN;
+          }
+          __builtin_amdgcn_global_load_lds((GlobalPtr)(src + 2 * lane), (LdsPtr)dst, 16, 0, 0);
+        }
+      };
+      double acc[TM * TG * 16];
+#pragma unroll
+      for (int e = 0; e < TM * TG * 16; e++) acc[e] = 0.0;
+      issue(0, 0);
+      for (int c = 0; c < nch; c++) {
+        // Chunk c has landed for every wave, and every wave is done with the
+        // buffer the next copies overwrite (it held chunk c - 1).
+        __syncthreads();
+        if (c + 1 < nch) issue(c + 1, (c + 1) & 1);
+        const double* Aw = lds + (c & 1) * CHUNK + lane;
+        const double* Bw = lds + (c & 1) * CHUNK + KB * BM + wave * (16 * TG) + (lane & 15);
+        double a[2][TM], b[2][TG];
+#pragma unroll
+        for (int i = 0; i < TM; i++) a[0][i] = Aw[64 * i];
+#pragma unroll
+        for (int g = 0; g < TG; g++) b[0][g] = Bw[16 * g];
+#pragma unroll
+        for (int kk = 0; kk < KB; kk++) {
+          const int x = kk & 1;
+          if (kk + 1 < KB) {
+#pragma unroll
+            for (int i = 0; i < TM; i++) a[x ^ 1][i] = Aw[(kk + 1) * BM + 64 * i];
+#pragma unroll
+            for (int g = 0; g < TG; g++) b[x ^ 1][g] = Bw[(kk + 1) * BN + 16 * g];
+          }
+#pragma unroll
+          for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int g = 0; g < TG; g++) dpp_fma16(acc + (i * TG + g) * 16, b[x][g], a[x][i]);
+        }
+      }
+      // Every global read of this block column (block J's own columns among
+      // them) was a copy that had landed by the last chunk's barrier, so the
+      // columns can be overwritten while other waves still run FMAs from LDS.
+#pragma unroll
+      for (int g = 0; g < TG; g++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const int col = wave * (16 * TG) + 16 * g + j;
+          if (col < jb) {
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+              Tm[(long long)(J0 + col) * ldt + 64 * i + lane] = acc[(i * TG + g) * 16 + j];
+          }
+        }
+      // The next block column reads what this one wrote.
+      __syncthreads();
+    }
+  }
+}
+
 // Tile variants of the solve (chosen per shape, see launch_potrs).
 typedef MfmaTile<256, kSB, 16, 4, 2> PotrsTile0;  // 8 waves of 64 x 64
 typedef MfmaTile<256, kSB, 16, 4, 1> PotrsTile1;  // 4 waves of 64 x 128
@@ -997,13 +1113,45 @@ int launch_linv_transposed(const double* L, int n, const double* Dinv128, double
   return 0;
 }
 
-int potrs_variants() { return 9; }
+int potrs_variants() { return 14; }
+
+namespace {
+// Whether k_potrs_dpp with BM-row strips and KB-deep chunks can take the solve.
+bool dpp_solve_fits(long long ng, long long ldt, int rk, const double* T, const double* Ft,
+                    const double* Gt, int bm, int kb) {
+  return ng % bm == 0 && rk % kb == 0 && ldt % 2 == 0 &&
+         ((uintptr_t)T | (uintptr_t)Ft | (uintptr_t)Gt) % 16 == 0;
+}
+
+// k_potrs_dpp shapes: variant -> (TM, TG, KB, waves per SIMD).
+template <int TM, int TG, int KB, int WPE>
+int launch_potrs_dpp(double* T, long long ng, long long ldt, int rk, const double* Ft,
+                     const double* Gt, hipStream_t st) {
+  constexpr int BM = 64 * TM, NT = 64 * (kSB / (16 * TG));
+  if (!dpp_solve_fits(ng, ldt, rk, T, Ft, Gt, BM, KB))
+    return set_error(1, "the vector-FMA solve needs whole row strips, rk a multiple of its k "
+                        "chunk and 16-byte aligned rows");
+  hipLaunchKernelGGL((k_potrs_dpp<TM, TG, KB, WPE>), dim3((unsigned)(ng / BM)), dim3(NT), 0, st,
+                     T, ldt, rk, Ft, Gt);
+  return 0;
+}
+}  // namespace
 
 int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* Ft, const double* Gt,
                  int variant, hipStream_t st, bool unit_lower) {
   if (unit_lower && ng != rk) return set_error(1, "unit_lower needs a square identity operand");
-  // 6 (LDS-staged write-back) measured 1.5-2.5% faster than 0 on the 64^3 and 96^3 workloads.
-  if (variant < 0) variant = (ng >= 256LL * 512) ? 6 : 2;
+  // Default: the vector-FMA solve where its shape conditions hold (9 on large grids, 10 on
+  // small ones: 7% faster than the MFMA tile at 96^3, 5% on the dual 96^3 workload, 2% at
+  // 64^3, 17% at 32^3), else the MFMA tiles (6, the LDS-staged write-back, measured
+  // 1.5-2.5% faster than 0 on the 64^3 and 96^3 workloads).
+  if (variant < 0) {
+    const bool big = ng >= 256LL * 512;
+    if (!unit_lower && dpp_solve_fits(ng, ldt, rk, T, Ft, Gt, 128, big ? 8 : 16))
+      variant = big ? 9 : 10;
+    else
+      variant = big ? 6 : 2;
+  }
+  if (variant >= 9 && unit_lower) return set_error(1, "the vector-FMA solve runs both passes");
   switch (variant) {
 #define ISDF_POTRS_CASE(V)                                                                      \
   case V:                                                                                       \
@@ -1025,6 +1173,21 @@ int launch_potrs(double* T, long long ng, long long ldt, int rk, const double* F
     ISDF_POTRS_CASE(7)
     ISDF_POTRS_CASE(8)
 #undef ISDF_POTRS_CASE
+    case 9:
+      if (int rc = launch_potrs_dpp<2, 2, 8, 3>(T, ng, ldt, rk, Ft, Gt, st)) return rc;
+      break;
+    case 10:
+      if (int rc = launch_potrs_dpp<2, 1, 16, 4>(T, ng, ldt, rk, Ft, Gt, st)) return rc;
+      break;
+    case 11:
+      if (int rc = launch_potrs_dpp<2, 1, 8, 4>(T, ng, ldt, rk, Ft, Gt, st)) return rc;
+      break;
+    case 12:
+      if (int rc = launch_potrs_dpp<2, 2, 16, 2>(T, ng, ldt, rk, Ft, Gt, st)) return rc;
+      break;
+    case 13:
+      if (int rc = launch_potrs_dpp<4, 1, 8, 2>(T, ng, ldt, rk, Ft, Gt, st)) return rc;
+      break;
     default:
       return set_error(1, "unknown solve tile variant");
   }

@@ -300,6 +300,27 @@ struct ValuTile {
   }
 };
 
+// acc[j] += b(lane j of this lane's row of 16) * a, for j = 0..15: sixteen
+// v_fmac_f64 whose first operand comes through DPP row_newbcast:j, so the
+// sixteen values of a row of lanes act as sixteen wave-uniform operands.  The
+// leading s_nop 1 gives the two wait states a DPP read needs after a VALU
+// write of its register (b is normally an LDS load, but the compiler may
+// move it with a VALU copy).  The memory clobber keeps loads from moving
+// across the block, so callers place their operand reads where they want them.
+__device__ __forceinline__ void dpp_fma16(double* acc, double b, double a) {
+#define ISDF_DF(J) "v_fmac_f64_dpp %" #J ", %16, %17 row_newbcast:" #J " row_mask:0xf bank_mask:0xf\n"
+  asm volatile("s_nop 1\n" ISDF_DF(0) ISDF_DF(1) ISDF_DF(2) ISDF_DF(3) ISDF_DF(4) ISDF_DF(5)
+               ISDF_DF(6) ISDF_DF(7) ISDF_DF(8) ISDF_DF(9) ISDF_DF(10) ISDF_DF(11) ISDF_DF(12)
+               ISDF_DF(13) ISDF_DF(14) ISDF_DF(15)
+               : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]),
+                 "+v"(acc[5]), "+v"(acc[6]), "+v"(acc[7]), "+v"(acc[8]), "+v"(acc[9]),
+                 "+v"(acc[10]), "+v"(acc[11]), "+v"(acc[12]), "+v"(acc[13]), "+v"(acc[14]),
+                 "+v"(acc[15])
+               : "v"(b), "v"(a)
+               : "memory");
+#undef ISDF_DF
+}
+
 // Vector-FMA tile with the B operand in scalar registers and no LDS.  NW
 // waves; lane l of wave w owns rows 64 i + l (i < TM = BM / 64) and the
 // columns w TN + j (j < TN = BN / NW), the same for every lane of the wave:

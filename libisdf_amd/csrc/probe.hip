@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "mfma_gemm.h"
+
 namespace {
 
 constexpr int kChains = 8;
@@ -54,6 +56,25 @@ __global__ __launch_bounds__(256) void k_fma_f64(double a0, double b0, int iters
   }
   double s = 0.0;
   for (int c = 0; c < 16; c++) s += acc[c];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// The solve tile's inner step with no memory traffic: 64 accumulators per
+// lane, two A values and two B registers, four groups of 16 v_fmac_f64 with B
+// broadcast by DPP row_newbcast (isdf_hip::dpp_fma16).
+__global__ __launch_bounds__(256) void k_dpp_f64(double a0, double b0, int iters, double* out) {
+  double acc[64];
+  for (int c = 0; c < 64; c++) acc[c] = c * 1e-3;
+  const double a[2] = {a0 + threadIdx.x * 1e-12, a0 - threadIdx.x * 1e-12};
+  const double b[2] = {b0 + (threadIdx.x & 63) * 1e-9, b0 - (threadIdx.x & 63) * 1e-9};
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int g = 0; g < 2; g++) isdf_hip::dpp_fma16(acc + (i * 2 + g) * 16, b[g], a[i]);
+  }
+  double s = 0.0;
+  for (int c = 0; c < 64; c++) s += acc[c];
   out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
@@ -102,7 +123,8 @@ extern "C" {
 
 // which: 0 v_mfma_f64_16x16x4f64, 1 v_fma_f64, 2 v_mfma_f32_16x16x4f32,
 // 3 dependent v_add_f64 chain (rate of one lane, in 1e12 adds/s), 4 half the
-// waves on v_mfma_f64_16x16x4f64 and half on v_fma_f64 (combined TFLOP/s).
+// waves on v_mfma_f64_16x16x4f64 and half on v_fma_f64 (combined TFLOP/s),
+// 5 v_fmac_f64 with B by DPP row_newbcast in the solve tile's register shape.
 // Launches `blocks` workgroups of 256 threads (4 waves), each running `iters`
 // rounds, twice (the first run warms the clocks up), and returns the second
 // run's rate in TFLOP/s through *tflops and its time through *ms.
@@ -132,6 +154,9 @@ int isdf_probe_peak(int which, int blocks, int iters, double* tflops, double* ms
       // FMA loop runs 4x as many rounds (about the same time on each pipe).
       k_mix_f64<<<blocks, 256>>>(1.0, 0.5, iters, 4 * iters, out);
       flops_per_thread_iter = 0.5 * (32.0 * kChains) + 0.5 * (2.0 * 16 * 4);
+    } else if (which == 5) {
+      k_dpp_f64<<<blocks, 256>>>(0.999, 1e-3, iters, out);
+      flops_per_thread_iter = 2.0 * 64;
     } else {
       // One wave per workgroup, `blocks` of them; *tflops then reports the
       // dependent adds per second of one lane (1e12 units), *ms the time.

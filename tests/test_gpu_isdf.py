@@ -226,3 +226,24 @@ def test_interpolation_property_cfg2():
     assert rel(g["theta"][:, rows], T) < 1e-10
     head = g["theta"][:, km["piv"][:rk]]  # Theta(piv[mu], nu) as (nu, mu)
     assert np.abs(head - np.eye(rk)).max() < 1e-6
+
+
+@pytest.mark.parametrize("tiles", [(2, 10, 11, 13), (6, 9, 12)])
+def test_solve_tiles_bitwise_equal(monkeypatch, tiles):
+    # Every triangular-solve tile (MFMA 16x16x4 f64 tiles and the vector-FMA
+    # tiles with DPP-broadcast operands) accumulates each Theta element as the
+    # same k-ordered chain of fused multiply-adds, so they agree bit for bit;
+    # the first of each group is the MFMA tile the solve used before.
+    w = synth.WORKLOADS["cfg1"]
+    psi = synth.wavefunctions(w)
+    args = (w.length, w.ngrid, w.nocc, w.nstate, w.nv1, w.nc1, w.nv2, w.nc2, w.mu_points, 10)
+    ref = None
+    for t in tiles:
+        monkeypatch.setenv("ISDF_POTRS_TILE", str(t))
+        g = gpu_solve(psi, *args)
+        if ref is None:
+            ref = g
+            continue
+        np.testing.assert_array_equal(g["piv"], ref["piv"])
+        np.testing.assert_array_equal(g["theta"].view(np.uint64), ref["theta"].view(np.uint64),
+                                      err_msg="tile %d" % t)
