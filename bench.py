@@ -159,21 +159,29 @@ def peak_probe():
 
 
 def compare_solve_tiles(a):
-    """FP64 solves of one workload with each listed triangular-solve tile, interleaved
-    in one process (so that every tile sees the same clock and thermal history)."""
+    """FP64 solves of one workload with each listed triangular-solve tile (and Z C^T
+    tile), interleaved in one process so that every tile sees the same clock and
+    thermal history."""
     import torch
     import libisdf_amd as isdf
     from libisdf_amd import synth
     w = synth.WORKLOADS[a.workload]
     psi = synth.wavefunctions(w)
     rk = isdf.isdf_rank(w.nv1, w.nc1, w.nv2, w.nc2, w.mu_points)
+    fl = flops(w, rk)
     d_psi = torch.from_numpy(psi).cuda()
     d_theta = torch.empty((rk, w.ng), dtype=torch.float64, device="cuda")
     first = None
-    saved = os.environ.get("ISDF_POTRS_TILE")
+    names = ("ISDF_POTRS_TILE", "ISDF_ZCT_TILE")
+    saved = {k: os.environ.get(k) for k in names}
     try:
-        for t in [int(x) for x in a.compare_solve_tiles.split(",")]:
-            os.environ["ISDF_POTRS_TILE"] = str(t)
+        for entry in a.compare_solve_tiles.split(","):
+            parts = entry.split("/")
+            for k, v in zip(names, parts + [None]):
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
             piv = np.zeros(w.ng, np.int32)
             rep = isdf.solve(d_psi, isdf.Domain(w.length, w.ngrid), w.nocc, w.nstate, w.nv1,
                              w.nc1, w.nv2, w.nc2, w.mu_points, d_theta, piv, w.kmeans_iters)
@@ -183,15 +191,19 @@ def compare_solve_tiles(a):
                 same = True
             else:
                 same = bool(torch.equal(d_theta, first))
-            print(json.dumps({"workload": w.name, "tile": t, "potrs_ms": rep["ms_potrs"],
-                              "potrs_tflops": 2.0 * w.ng * rk * rk / rep["ms_potrs"] / 1e9,
-                              "total_ms": rep["ms_total"], "theta_bitwise_equal_first": same}),
+            print(json.dumps({"workload": w.name, "tiles": entry, "potrs_ms": rep["ms_potrs"],
+                              "potrs_tflops": fl["potrs"] / rep["ms_potrs"] / 1e9,
+                              "zct_ms": rep["ms_zct"], "zct_tflops": fl["zct"] / rep["ms_zct"] / 1e9,
+                              "total_ms": rep["ms_total"], "theta_bitwise_equal_first": same,
+                              "stage_ms": {k[3:]: round(v, 3) for k, v in rep.items()
+                                           if k.startswith("ms_")}}),
                   flush=True)
     finally:
-        if saved is None:
-            os.environ.pop("ISDF_POTRS_TILE", None)
-        else:
-            os.environ["ISDF_POTRS_TILE"] = saved
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
 
 
 def meets_target(m, fit64):
@@ -225,9 +237,10 @@ def main():
     ap.add_argument("--peak-probe", action="store_true",
                     help="run only register-only throughput loops of the FP64 / FP32 units")
     ap.add_argument("--compare-solve-tiles", default=None, metavar="LIST",
-                    help="instead of the benchmark: FP64 solves of the workload, one per "
-                         "triangular-solve tile in the comma-separated list (ISDF_POTRS_TILE), "
-                         "in that order in one process; prints each solve's time and whether "
+                    help="instead of the benchmark: FP64 solves of the workload, one per entry "
+                         "of the comma-separated list, in that order in one process; an entry is "
+                         "a triangular-solve tile (ISDF_POTRS_TILE) or solve/Z C^T tiles "
+                         "('9/5': ISDF_ZCT_TILE too); prints each solve's times and whether "
                          "Theta equals the first solve's bit for bit")
     a = ap.parse_args()
     if a.peak_probe:

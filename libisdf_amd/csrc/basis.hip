@@ -767,39 +767,21 @@ __global__ __launch_bounds__(XxtTile::NT, 1) void k_xxt(const double* __restrict
   });
 }
 
-// The in-place solve of k_potrs on the vector FMA pipe, with operands staged
-// by LDS DMA and the B operand broadcast by DPP (dpp_fma16).  A workgroup owns
-// BM = 64 TM rows and walks the block columns as k_potrs does; its NW waves
-// split the 128 columns of a block into groups of 16 TG, each wave taking
-// every row of the strip: lane l holds rows 64 i + l (i < TM) and the columns
-// of its groups.  Per k step a lane reads TM A values and TG B values from LDS
-// (B at column 16 g + l % 16, so each row of 16 lanes holds a group) and runs
-// 16 TM TG FMAs.
-//
-// Staging: global_load_lds_dwordx4 writes 64 lanes x 16 bytes = 128 doubles
-// to LDS from a wave-uniform base, so a chunk of KB k values is KB BM / 128
-// copies of A column pieces and KB copies of B rows, with no VGPRs or LDS
-// stores spent on it; the copies of chunk c + 1 run during the FMAs of chunk
-// c (double buffer, one barrier per chunk).
-// Requires: ng a multiple of BM (no row tail: every strip is whole), every K
-// a multiple of KB (n a multiple of KB), 16-byte aligned rows (ldt even, T
-// and B 16-byte aligned).  Columns of a B row past the block width are read
-// (the F/G allocations carry one block width of padding) and only feed
-// output columns that are not stored.
+// The in-place solve of k_potrs on the FP64 vector FMA pipe (DppGemm: B
+// broadcast by DPP, operands staged by LDS DMA).  A workgroup owns BM = 64 TM
+// rows and walks the block columns as k_potrs does.  Requires ng a multiple
+// of BM (every strip whole), n a multiple of KB, 16-byte aligned rows (ldt
+// even, T, Ft, Gt 16-byte aligned); columns of a B row past the block width
+// are read (the F / G allocations carry one block width of padding) and feed
+// only output columns that are not stored.
 template <int TM, int TG, int KB, int WPE>
-__global__ __launch_bounds__(64 * (kSB / (16 * TG)), WPE) void k_potrs_dpp(
+__global__ __launch_bounds__((DppGemm<TM, TG, KB>::NT), WPE) void k_potrs_dpp(
     double* __restrict__ T, long long ldt, int n, const double* __restrict__ Ft,
     const double* __restrict__ Gt) {
-  constexpr int BM = 64 * TM, BN = kSB, NW = BN / (16 * TG);
-  constexpr int ACOPY = KB * BM / 128, NCOPY = ACOPY + KB;  // 1 KB copies per chunk
-  constexpr int CHUNK = KB * (BM + BN);                       // doubles per buffer
-  static_assert(BM % 128 == 0 && NCOPY % NW == 0, "copies must split over the waves");
-  __shared__ double lds[2 * CHUNK];
-  typedef __attribute__((address_space(3))) void* LdsPtr;
-  typedef __attribute__((address_space(1))) void* GlobalPtr;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  double* Tm = T + (long long)blockIdx.x * BM;
+  typedef DppGemm<TM, TG, KB> G;
+  static_assert(G::BN == kSB, "block column width");
+  __shared__ double lds[G::LDS_DOUBLES];
+  double* Tm = T + (long long)blockIdx.x * G::BM;
   const int nJ = (n + kSB - 1) / kSB;
   for (int pass = 0; pass < 2; pass++) {
     for (int s = 0; s < nJ; s++) {
@@ -808,61 +790,10 @@ __global__ __launch_bounds__(64 * (kSB / (16 * TG)), WPE) void k_potrs_dpp(
       const int jb = min(kSB, n - J0);
       const int kbeg = pass == 0 ? 0 : J0;
       const int K = (pass == 0 ? J0 + jb : n) - kbeg;
-      const double* A = Tm + (long long)kbeg * ldt;
-      const double* B = (pass == 0 ? Ft : Gt) + (long long)kbeg * n + J0;
-      const int nch = K / KB;
-      // Copy q of this wave for chunk c into buffer `buf`.
-      auto issue = [&](int c, int buf) {
-        double* As = lds + buf * CHUNK;
-        double* Bs = As + KB * BM;
-#pragma unroll
-        for (int q = 0; q < NCOPY / NW; q++) {
-          const int idx = wave + q * NW;
-          const double* src;
-          double* dst;
-          if (idx < ACOPY) {
-            const int col = idx / (BM / 128), part = idx % (BM / 128);
-            src = A + (long long)(c * KB + col) * ldt + part * 128;
-            dst = As + col * BM + part * 128;
-          } else {
-            const int kk = idx - ACOPY;
-            src = B + (long long)(c * KB + kk) * n;
-            dst = Bs + kk * BN;
-          }
-          __builtin_amdgcn_global_load_lds((GlobalPtr)(src + 2 * lane), (LdsPtr)dst, 16, 0, 0);
-        }
-      };
-      double acc[TM * TG * 16];
-#pragma unroll
-      for (int e = 0; e < TM * TG * 16; e++) acc[e] = 0.0;
-      issue(0, 0);
-      for (int c = 0; c < nch; c++) {
-        // Chunk c has landed for every wave, and every wave is done with the
-        // buffer the next copies overwrite (it held chunk c - 1).
-        __syncthreads();
-        if (c + 1 < nch) issue(c + 1, (c + 1) & 1);
-        const double* Aw = lds + (c & 1) * CHUNK + lane;
-        const double* Bw = lds + (c & 1) * CHUNK + KB * BM + wave * (16 * TG) + (lane & 15);
-        double a[2][TM], b[2][TG];
-#pragma unroll
-        for (int i = 0; i < TM; i++) a[0][i] = Aw[64 * i];
-#pragma unroll
-        for (int g = 0; g < TG; g++) b[0][g] = Bw[16 * g];
-#pragma unroll
-        for (int kk = 0; kk < KB; kk++) {
-          const int x = kk & 1;
-          if (kk + 1 < KB) {
-#pragma unroll
-            for (int i = 0; i < TM; i++) a[x ^ 1][i] = Aw[(kk + 1) * BM + 64 * i];
-#pragma unroll
-            for (int g = 0; g < TG; g++) b[x ^ 1][g] = Bw[(kk + 1) * BN + 16 * g];
-          }
-#pragma unroll
-          for (int i = 0; i < TM; i++)
-#pragma unroll
-            for (int g = 0; g < TG; g++) dpp_fma16(acc + (i * TG + g) * 16, b[x][g], a[x][i]);
-        }
-      }
+      typename G::Acc acc;
+      G::zero(acc);
+      G::run(Tm + (long long)kbeg * ldt, ldt, (pass == 0 ? Ft : Gt) + (long long)kbeg * n + J0, n,
+             K, lds, acc);
       // Every global read of this block column (block J's own columns among
       // them) was a copy that had landed by the last chunk's barrier, so the
       // columns can be overwritten while other waves still run FMAs from LDS.
@@ -870,16 +801,57 @@ __global__ __launch_bounds__(64 * (kSB / (16 * TG)), WPE) void k_potrs_dpp(
       for (int g = 0; g < TG; g++)
 #pragma unroll
         for (int j = 0; j < 16; j++) {
-          const int col = wave * (16 * TG) + 16 * g + j;
+          const int col = G::col(g, j);
           if (col < jb) {
 #pragma unroll
             for (int i = 0; i < TM; i++)
-              Tm[(long long)(J0 + col) * ldt + 64 * i + lane] = acc[(i * TG + g) * 16 + j];
+              Tm[(long long)(J0 + col) * ldt + G::row(i)] = acc[(i * TG + g) * 16 + j];
           }
         }
       // The next block column reads what this one wrote.
       __syncthreads();
     }
+  }
+}
+
+// Z C^T of k_zct on the FP64 vector FMA pipe (DppGemm): C = P1 o P1 or P1 o P2
+// with P1 = A1 B1 (K1 deep), P2 = A2 B2 (K2 deep), one BM x 128 tile per
+// workgroup step (a grid smaller than the tile count walks tiles tb,
+// tb + gridDim.x, ..., a multiple of 8 apart so that a tile stays on its XCD).
+// The factors live only in accumulators; the epilogue writes their product.
+// Requires ng a multiple of BM, rk a multiple of 128, K1 and K2 multiples of
+// KB, even lda and ldb, 16-byte aligned operands.
+template <int TM, int TG, int KB, int WPE, bool DUAL>
+__global__ __launch_bounds__((DppGemm<TM, TG, KB>::NT), WPE) void k_zct_dpp(
+    int ng, int rk, const double* __restrict__ A1, const double* __restrict__ A2,
+    long long lda, int K1, int K2, const double* __restrict__ B1,
+    const double* __restrict__ B2, int ldb, double* __restrict__ C, long long ldc) {
+  typedef DppGemm<TM, TG, KB> G;
+  __shared__ double lds[G::LDS_DOUBLES];
+  const int nrows = ng / G::BM;
+  const int ncolt = rk / G::BN;
+  for (int tb = blockIdx.x; tb < nrows * ncolt; tb += gridDim.x) {
+    int rs, ct;
+    xcd_tile(tb, ncolt, nrows, &rs, &ct);
+    const long long m0 = (long long)rs * G::BM;
+    const int n0 = ct * G::BN;
+    typename G::Acc acc, acc2;
+    G::zero(acc);
+    G::run(A1 + m0, lda, B1 + n0, ldb, K1, lds, acc);
+    if constexpr (DUAL) {
+      G::zero(acc2);
+      G::run(A2 + m0, lda, B2 + n0, ldb, K2, lds, acc2);
+    }
+#pragma unroll
+    for (int g = 0; g < TG; g++)
+#pragma unroll
+      for (int j = 0; j < 16; j++)
+#pragma unroll
+        for (int i = 0; i < TM; i++) {
+          const int e = (i * TG + g) * 16 + j;
+          const double v = acc[e];
+          C[(long long)(n0 + G::col(g, j)) * ldc + m0 + G::row(i)] = DUAL ? v * acc2[e] : v * v;
+        }
   }
 }
 
@@ -903,7 +875,7 @@ int launch_gather_psimu(const double* psi, long long ld, int nstate, const int* 
   return 0;
 }
 
-int zct_variants() { return 5; }
+int zct_variants() { return 7; }
 
 int device_cus() {
   static int cus[64] = {0};
@@ -918,15 +890,52 @@ int device_cus() {
   return cus[dev];
 }
 
+// The MFMA tile for a grid of ng points: 4 (LDS-staged epilogue) measured 6-9% faster
+// than 0 on the 64^3 and 96^3 workloads.
+int zct_mfma_variant(long long ng) { return ng >= 256LL * 256 ? 4 : 2; }
+
 int zct_pick_variant(long long ng, int requested) {
-  // 4 (LDS-staged epilogue) measured 6-9% faster than 0 on the 64^3 and 96^3 workloads.
-  return requested >= 0 ? requested : (ng >= 256LL * 256 ? 4 : 2);
+  // Default: the vector-FMA product (5), which launch_zct replaces by the MFMA tile
+  // where its shape conditions fail.  Per launch at 96^3 18.0 against 19.4 ms, on the
+  // dual 96^3 workload 66 against 74 ms, at 64^3 1.6 against 2.1 ms, bitwise equal.
+  return requested >= 0 ? requested : 5;
 }
 
 int grid_leaving_cus(long long tiles, int reserve) {
   if (reserve <= 0) return (int)tiles;
   return (int)std::min<long long>(tiles, std::max(8, (device_cus() - reserve) / 8 * 8));
 }
+
+namespace {
+// Whether k_zct_dpp with BM-row strips and KB-deep chunks can take the product.
+bool dpp_zct_fits(int ng, int rk, long long lda, int ldb, int K1, int K2, bool dual,
+                  const double* A1, const double* A2, const double* B1, const double* B2, int bm,
+                  int kb) {
+  const uintptr_t al = (uintptr_t)A1 | (uintptr_t)B1 | (dual ? (uintptr_t)A2 | (uintptr_t)B2 : 0);
+  return ng % bm == 0 && rk % 128 == 0 && K1 % kb == 0 && (!dual || K2 % kb == 0) &&
+         lda % 2 == 0 && ldb % 2 == 0 && al % 16 == 0;
+}
+
+// k_zct_dpp shapes: (TM, TG, KB, waves per SIMD); WGS workgroups per CU fit
+// at that occupancy.  With CUs reserved for the concurrent Cholesky branch
+// the grid is WGS per remaining CU (a multiple of 8) and walks the tiles.
+template <int TM, int TG, int KB, int WPE, int WGS, bool DUAL>
+int launch_zct_dpp(int ng, int rk, const double* A1, const double* A2, long long lda, int K1,
+                   int K2, const double* B1, const double* B2, int ldb, double* C, long long ldc,
+                   hipStream_t st, int reserve_cus) {
+  typedef DppGemm<TM, TG, KB> G;
+  if (!dpp_zct_fits(ng, rk, lda, ldb, K1, K2, DUAL, A1, A2, B1, B2, G::BM, KB))
+    return set_error(1, "the vector-FMA Z C^T needs whole row strips, rk a multiple of 128, K "
+                        "a multiple of its k chunk and 16-byte aligned operands");
+  const long long tiles = (long long)(ng / G::BM) * (rk / G::BN);
+  long long grid = tiles;
+  if (reserve_cus > 0)
+    grid = std::min<long long>(tiles, (long long)WGS * std::max(8, (device_cus() - reserve_cus) / 8 * 8));
+  hipLaunchKernelGGL((k_zct_dpp<TM, TG, KB, WPE, DUAL>), dim3((unsigned)grid), dim3(G::NT), 0, st,
+                     ng, rk, A1, A2, lda, K1, K2, B1, B2, ldb, C, ldc);
+  return 0;
+}
+}  // namespace
 
 int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, int rk, int s1,
                int K1, int s2, int K2, bool dual, double* C, long long ldc, int variant,
@@ -936,6 +945,26 @@ int launch_zct(const double* psi, long long ld, int ng, const double* psimuT, in
   const double* B1 = psimuT + (long long)s1 * rk;
   const double* B2 = psimuT + (long long)s2 * rk;
   variant = zct_pick_variant(ng, variant);
+  if ((variant == 5 || variant == 6) &&
+      !dpp_zct_fits(ng, rk, ld, rk, K1, K2, dual, A1, A2, B1, B2, variant == 5 && dual ? 64 : 128,
+                    variant == 6 && !dual ? 16 : 8))
+    variant = zct_mfma_variant(ng);
+  if (variant == 5 || variant == 6) {
+    int rc;
+    if (!dual)
+      rc = variant == 5 ? launch_zct_dpp<2, 2, 8, 3, 3, false>(ng, rk, A1, A2, ld, K1, K2, B1, B2,
+                                                               rk, C, ldc, st, reserve_cus)
+                        : launch_zct_dpp<2, 1, 16, 4, 2, false>(ng, rk, A1, A2, ld, K1, K2, B1,
+                                                                B2, rk, C, ldc, st, reserve_cus);
+    else
+      rc = variant == 5 ? launch_zct_dpp<1, 2, 8, 3, 3, true>(ng, rk, A1, A2, ld, K1, K2, B1, B2,
+                                                              rk, C, ldc, st, reserve_cus)
+                        : launch_zct_dpp<2, 1, 8, 2, 1, true>(ng, rk, A1, A2, ld, K1, K2, B1, B2,
+                                                              rk, C, ldc, st, reserve_cus);
+    if (rc) return rc;
+    ISDF_HIP_LAUNCHED("k_zct_dpp");
+    return 0;
+  }
 #define ISDF_ZCT_LAUNCH(TL, DUALV)                                                              \
   hipLaunchKernelGGL((k_zct<TL, DUALV>), dim3(cdiv(ng, TL::BM) * cdiv(rk, TL::BN)), dim3(TL::NT), \
                      0, st, ng, rk, A1, A2, ld, K1, K2, B1, B2, rk, C, ldc)

@@ -321,6 +321,123 @@ __device__ __forceinline__ void dpp_fma16(double* acc, double b, double a) {
 #undef ISDF_DF
 }
 
+// One workgroup's acc += A (BM x K) * B (K x 128) on the FP64 vector FMA pipe,
+// with the B operand broadcast by DPP (dpp_fma16) and both operands staged
+// into LDS by LDS DMA.  BM = 64 TM rows; the NW = 128 / (16 TG) waves split
+// the 128 columns into groups of 16 TG, each wave taking every row: lane l
+// holds rows 64 i + l (i < TM) and columns 16 TG w + 16 g + j (g < TG, j < 16)
+// in acc[(i TG + g) 16 + j].  Per k step a lane reads TM A values and TG B
+// values from LDS (B at column 16 g + l % 16, so each row of 16 lanes holds a
+// group) for 16 TM TG FMAs: an LDS broadcast of B would return every column
+// to every lane, a scalar load of it would make each use wait for all.
+//
+// Staging: global_load_lds_dwordx4 writes 64 lanes x 16 bytes = 128 doubles
+// of LDS from a wave-uniform base, each lane reading its own 16 bytes from
+// anywhere; a chunk of KB k values is KB BM / 128 copies of A (column
+// pieces, or whole columns in pairs when BM = 64) and KB copies of B rows, no
+// VGPRs or LDS stores spent on it.  The copies of chunk c + 1 run during the
+// FMAs of chunk c (double buffer, one barrier per chunk).
+//
+// Layouts: A(m, k) at A[m + k lda], B(k, n) at B[k ldb + n].  Requires K a
+// multiple of KB, all BM rows of A readable (no row tail), 16-byte aligned
+// column / row starts (lda, ldb even, A and B 16-byte aligned), and every B
+// row readable for 128 columns: columns past the caller's width feed only
+// output columns it does not store.
+template <int TM, int TG, int KB>
+struct DppGemm {
+  static constexpr int BM = 64 * TM, BN = 128, NW = BN / (16 * TG), NT = 64 * NW;
+  static constexpr int ACOPY = KB * BM / 128, NCOPY = ACOPY + KB;  // 1 KB copies per chunk
+  static constexpr int CHUNK = KB * (BM + BN);                       // doubles per buffer
+  static constexpr int LDS_DOUBLES = 2 * CHUNK;
+  static constexpr int NACC = TM * TG * 16;
+  static_assert(TG >= 1 && BN % (16 * TG) == 0, "column groups");
+  static_assert(KB * BM % 128 == 0 && NCOPY % NW == 0, "copies must split over the waves");
+  typedef double Acc[NACC];
+
+  __device__ static void zero(Acc& acc) {
+#pragma unroll
+    for (int e = 0; e < NACC; e++) acc[e] = 0.0;
+  }
+
+  // This wave's copies of chunk c (k values c KB ..) into buffer `buf`.
+  __device__ static void issue(const double* __restrict__ A, long long lda,
+                               const double* __restrict__ B, long long ldb, int c, double* buf,
+                               int wave, int lane) {
+    typedef __attribute__((address_space(3))) void* LdsPtr;
+    typedef __attribute__((address_space(1))) void* GlobalPtr;
+    double* As = buf;
+    double* Bs = buf + KB * BM;
+#pragma unroll
+    for (int q = 0; q < NCOPY / NW; q++) {
+      const int idx = wave + q * NW;
+      const double* src;
+      double* dst;
+      if (idx < ACOPY) {
+        // LDS doubles idx 128 .. idx 128 + 127 of As = [KB][BM]; lane l's pair
+        // 2 l, 2 l + 1 lies in one column (BM even).
+        const int e = idx * 128 + 2 * lane;
+        src = A + (long long)(c * KB + e / BM) * lda + e % BM;
+        dst = As + idx * 128;
+      } else {
+        const int kk = idx - ACOPY;
+        src = B + (long long)(c * KB + kk) * ldb + 2 * lane;
+        dst = Bs + kk * BN;
+      }
+      __builtin_amdgcn_global_load_lds((GlobalPtr)src, (LdsPtr)dst, 16, 0, 0);
+    }
+  }
+
+  // The FMAs of one chunk in buffer `buf`; the operands of step kk + 1 are read
+  // before the FMAs of step kk (dpp_fma16 clobbers memory, so they stay there).
+  __device__ static void compute(const double* buf, int wave, int lane, Acc& acc) {
+    const double* Aw = buf + lane;
+    const double* Bw = buf + KB * BM + wave * (16 * TG) + (lane & 15);
+    double a[2][TM], b[2][TG];
+#pragma unroll
+    for (int i = 0; i < TM; i++) a[0][i] = Aw[64 * i];
+#pragma unroll
+    for (int g = 0; g < TG; g++) b[0][g] = Bw[16 * g];
+#pragma unroll
+    for (int kk = 0; kk < KB; kk++) {
+      const int x = kk & 1;
+      if (kk + 1 < KB) {
+#pragma unroll
+        for (int i = 0; i < TM; i++) a[x ^ 1][i] = Aw[(kk + 1) * BM + 64 * i];
+#pragma unroll
+        for (int g = 0; g < TG; g++) b[x ^ 1][g] = Bw[(kk + 1) * BN + 16 * g];
+      }
+#pragma unroll
+      for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int g = 0; g < TG; g++) dpp_fma16(acc + (i * TG + g) * 16, b[x][g], a[x][i]);
+    }
+  }
+
+  // acc += A * B over K (a multiple of KB).  Starts with a barrier (the LDS
+  // buffers may still be read by a previous call's last chunk); ends after the
+  // last chunk's FMAs, every copy of A and B landed (so A may be overwritten
+  // afterwards), but other waves possibly still reading LDS.
+  __device__ static void run(const double* __restrict__ A, long long lda,
+                             const double* __restrict__ B, long long ldb, int K, double* lds,
+                             Acc& acc) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nch = K / KB;
+    __syncthreads();
+    if (nch > 0) issue(A, lda, B, ldb, 0, lds, wave, lane);
+    for (int c = 0; c < nch; c++) {
+      // Chunk c has landed for every wave, and every wave is done with the
+      // buffer the next copies overwrite (it held chunk c - 1).
+      __syncthreads();
+      if (c + 1 < nch) issue(A, lda, B, ldb, c + 1, lds + ((c + 1) & 1) * CHUNK, wave, lane);
+      compute(lds + (c & 1) * CHUNK, wave, lane, acc);
+    }
+  }
+
+  __device__ static int row(int i) { return 64 * i + (threadIdx.x & 63); }
+  __device__ static int col(int g, int j) { return (threadIdx.x >> 6) * (16 * TG) + 16 * g + j; }
+};
+
 // Vector-FMA tile with the B operand in scalar registers and no LDS.  NW
 // waves; lane l of wave w owns rows 64 i + l (i < TM = BM / 64) and the
 // columns w TN + j (j < TN = BN / NW), the same for every lane of the wave:

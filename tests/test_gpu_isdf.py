@@ -228,22 +228,53 @@ def test_interpolation_property_cfg2():
     assert np.abs(head - np.eye(rk)).max() < 1e-6
 
 
-@pytest.mark.parametrize("tiles", [(2, 10, 11, 13), (6, 9, 12)])
+@pytest.mark.parametrize("tiles", [("2", "10", "11", "13"), ("6", "9", "12"),
+                                   ("2/2", "2/5", "2/6", "10/5")])
 def test_solve_tiles_bitwise_equal(monkeypatch, tiles):
-    # Every triangular-solve tile (MFMA 16x16x4 f64 tiles and the vector-FMA
-    # tiles with DPP-broadcast operands) accumulates each Theta element as the
-    # same k-ordered chain of fused multiply-adds, so they agree bit for bit;
-    # the first of each group is the MFMA tile the solve used before.
+    # Every triangular-solve tile and every FP64 Z C^T tile (MFMA 16x16x4 f64
+    # tiles and the vector-FMA tiles with DPP-broadcast operands) accumulates
+    # each element as the same k-ordered chain of fused multiply-adds, so they
+    # agree bit for bit; the first of each group is an MFMA tile.  An entry is
+    # "solve tile" or "solve tile/Z C^T tile".
     w = synth.WORKLOADS["cfg1"]
     psi = synth.wavefunctions(w)
     args = (w.length, w.ngrid, w.nocc, w.nstate, w.nv1, w.nc1, w.nv2, w.nc2, w.mu_points, 10)
     ref = None
-    for t in tiles:
-        monkeypatch.setenv("ISDF_POTRS_TILE", str(t))
+    for entry in tiles:
+        parts = entry.split("/")
+        monkeypatch.setenv("ISDF_POTRS_TILE", parts[0])
+        if len(parts) > 1:
+            monkeypatch.setenv("ISDF_ZCT_TILE", parts[1])
+        else:
+            monkeypatch.delenv("ISDF_ZCT_TILE", raising=False)
         g = gpu_solve(psi, *args)
         if ref is None:
             ref = g
             continue
         np.testing.assert_array_equal(g["piv"], ref["piv"])
+        np.testing.assert_array_equal(g["cct"].view(np.uint64), ref["cct"].view(np.uint64),
+                                      err_msg="tiles %s" % entry)
         np.testing.assert_array_equal(g["theta"].view(np.uint64), ref["theta"].view(np.uint64),
-                                      err_msg="tile %d" % t)
+                                      err_msg="tiles %s" % entry)
+
+
+def test_dual_zct_tiles_bitwise_equal(monkeypatch):
+    # The two-product Z C^T (differing pair sets) on the MFMA tile and on both
+    # vector-FMA tiles.
+    nstate, nocc, nv1, nc1, nv2, nc2, mu = 24, 12, 12, 12, 8, 8, 256
+    grid, L = (32, 32, 32), (10.26,) * 3
+    ng = grid[0] * grid[1] * grid[2]
+    psi = synth.cholesky_qr2(np.random.default_rng(5).standard_normal((nstate, ng)),
+                             L[0] * L[1] * L[2] / ng)
+    args = (L, grid, nocc, nstate, nv1, nc1, nv2, nc2, mu, 10)
+    ref = None
+    for z in ("4", "5", "6"):
+        monkeypatch.setenv("ISDF_ZCT_TILE", z)
+        g = gpu_solve(psi, *args)
+        if ref is None:
+            ref = g
+            continue
+        np.testing.assert_array_equal(g["cct"].view(np.uint64), ref["cct"].view(np.uint64),
+                                      err_msg="Z C^T tile %s" % z)
+        np.testing.assert_array_equal(g["theta"].view(np.uint64), ref["theta"].view(np.uint64),
+                                      err_msg="Z C^T tile %s" % z)
