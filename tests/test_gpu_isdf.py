@@ -278,3 +278,27 @@ def test_dual_zct_tiles_bitwise_equal(monkeypatch):
                                       err_msg="Z C^T tile %s" % z)
         np.testing.assert_array_equal(g["theta"].view(np.uint64), ref["theta"].view(np.uint64),
                                       err_msg="Z C^T tile %s" % z)
+
+
+@pytest.mark.parametrize("pad", [33, 128])
+def test_strided_psi_solve_bitwise_equal(pad):
+    # psi in a wider buffer (ld_psi = Ng + pad, NaN beyond Ng): an odd leading
+    # dimension takes the FP64 Z C^T off the vector-FMA tile (its copies need
+    # 16-byte aligned columns) onto the MFMA tile, an even one keeps it; both
+    # give the bits of the compact buffer.
+    w = synth.WORKLOADS["cfg1"]
+    psi = synth.wavefunctions(w)
+    args = (w.length, w.ngrid, w.nocc, w.nstate, w.nv1, w.nc1, w.nv2, w.nc2, w.mu_points, 10)
+    ref = gpu_solve(psi, *args)
+    ng, rk = w.ng, w.mu_points
+    wide = np.full((w.nstate, ng + pad), np.nan)
+    wide[:, :ng] = psi
+    d_wide = torch.from_numpy(wide).cuda()
+    d_theta = torch.full((rk, ng), float("nan"), dtype=torch.float64, device="cuda")
+    piv = np.zeros(ng, np.int32)
+    isdf.solve(d_wide, isdf.Domain(w.length, w.ngrid), w.nocc, w.nstate, w.nv1, w.nc1, w.nv2,
+               w.nc2, w.mu_points, d_theta, piv, 10, ld_psi=ng + pad)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(piv, ref["piv"])
+    np.testing.assert_array_equal(d_theta.cpu().numpy().view(np.uint64),
+                                  ref["theta"].view(np.uint64))
