@@ -220,24 +220,33 @@ __global__ __launch_bounds__(Tile::NT, 1) void k_zct_persistent(
 }
 
 // trace summed in ascending order, eps = 1e-10 * trace / rk added to the
-// diagonal (getbasis_scalapack.cpp:398-428).  One block.
+// diagonal (getbasis_scalapack.cpp:398-428).  One block; the diagonal passes
+// through LDS in chunks of 1024 (8 KB), so the block fits on a CU beside a
+// contraction workgroup that holds most of the LDS.
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_trace_eps(double* __restrict__ P, int rk,
                                                    double* __restrict__ eps_out,
                                                    const CholJob* jobs) {
+  constexpr int kChunk = 1024;
   __shared__ double eps_s;
-  __shared__ double diag[8192];
+  __shared__ double diag[kChunk];
   if constexpr (BATCH) {
     const CholJob& J = jobs[blockIdx.z];
     P = J.P;
     rk = J.n;
     eps_out = J.eps;
   }
-  for (int nu = threadIdx.x; nu < rk; nu += blockDim.x) diag[nu] = P[(long long)nu * rk + nu];
-  __syncthreads();
+  double tr = 0.0;  // thread 0's running sum
+  for (int c0 = 0; c0 < rk; c0 += kChunk) {
+    const int m = min(kChunk, rk - c0);
+    for (int i = threadIdx.x; i < m; i += blockDim.x)
+      diag[i] = P[(long long)(c0 + i) * rk + c0 + i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < m; i++) tr += diag[i];
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
-    double tr = 0.0;
-    for (int nu = 0; nu < rk; nu++) tr += diag[nu];
     double eps = 1e-10 * tr / (double)rk;
     eps_s = eps;
     *eps_out = eps;

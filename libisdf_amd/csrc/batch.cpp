@@ -102,12 +102,14 @@ struct Plan {
   StatusSrc* d_src = nullptr;
   InstStatus* d_status = nullptr;
   KmJob* d_km = nullptr;
+  FrontJob* d_front = nullptr;       // the batched operand packing
+  long long front_pack = 0, front_gather = 0;  // its two grids (blocks)
   std::vector<KmJob> h_km;
   std::vector<LpGemmJob> pjobs, zjobs;
   std::vector<LpThetaJob> tjobs;
   long long ptotal = 0, ztotal = 0, ttotal = 0;
   std::vector<hipStream_t> hs;
-  std::vector<hipEvent_t> ev;  // fork, helper joins..., zct fork, zct join
+  std::vector<hipEvent_t> ev;  // kEvCount events (the enum below)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int device = -1;
@@ -131,7 +133,9 @@ struct Plan {
 };
 
 Plan g_plan;
-constexpr int kHelpers = 4;  // concurrent front branches (hardware queues per process)
+constexpr int kHelpers = 2;  // helper streams: Z C^T beside the Cholesky chain; the piv copies
+// Events of the schedule.
+enum { kEvFork, kEvPivJoin, kEvPackFork, kEvPackDone, kEvZFork, kEvZJoin, kEvCount };
 
 bool same_batch(const Plan& pl, int nb, const isdf_problem* probs, const double* const* psi,
                 double* const* theta, int dev) {
@@ -191,7 +195,8 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
   const size_t off_jobs = off;
   off += al256(sizeof(CholJob) * nb) + 2 * al256(sizeof(LpGemmJob) * nb) +
          al256(sizeof(LpThetaJob) * nb) + al256(sizeof(StatusSrc) * nb) +
-         al256(sizeof(InstStatus) * nb) + al256(sizeof(KmJob) * nb);
+         al256(sizeof(InstStatus) * nb) + al256(sizeof(KmJob) * nb) +
+         al256(sizeof(FrontJob) * nb);
   hipError_t e = hipMalloc(&pl.dev, off);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -268,6 +273,22 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
   pl.d_status = (InstStatus*)(d + o);
   o += al256(sizeof(InstStatus) * nb);
   pl.d_km = (KmJob*)(d + o);
+  o += al256(sizeof(KmJob) * nb);
+  pl.d_front = (FrontJob*)(d + o);
+  std::vector<FrontJob> front(nb);
+  pl.front_pack = pl.front_gather = 0;
+  for (int i = 0; i < nb; i++) {
+    const Inst& I = pl.inst[i];
+    int pb = 0, gb = 0;
+    front_blocks(I.ng, I.rk, I.kp, &pb, &gb);
+    front[i] = FrontJob{I.psi, I.ld, I.piv_dev, I.A, I.B, I.ng, I.rk, I.s1, I.K1, I.kp,
+                        (int)pl.front_pack, (int)pl.front_gather};
+    pl.front_pack += pb;
+    pl.front_gather += gb;
+    if (pl.front_pack > 0x7fffffffLL || pl.front_gather > 0x7fffffffLL)
+      return set_error(ISDF_EINVAL, "batch operand packing exceeds the int range");
+  }
+  ISDF_HIP_TRY(hipMemcpy(pl.d_front, front.data(), sizeof(FrontJob) * nb, hipMemcpyHostToDevice));
   ISDF_HIP_TRY(hipMemcpy(pl.d_km, pl.h_km.data(), sizeof(KmJob) * nb, hipMemcpyHostToDevice));
   ISDF_HIP_TRY(hipMemcpy(pl.d_chol, chol.data(), sizeof(CholJob) * nb, hipMemcpyHostToDevice));
   ISDF_HIP_TRY(hipMemcpy(pl.d_src, src.data(), sizeof(StatusSrc) * nb, hipMemcpyHostToDevice));
@@ -277,7 +298,7 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
   const int nstreams = std::min(kHelpers, nb);
   pl.hs.resize(nstreams);
   for (auto& s : pl.hs) ISDF_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  pl.ev.resize(nstreams + 3);
+  pl.ev.resize(kEvCount);
   for (auto& ev : pl.ev) ISDF_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   return 0;
 }
@@ -316,15 +337,6 @@ int set_jobs(Plan& pl, int zv, int tv) {
   return 0;
 }
 
-// Per-instance operand rows, after the batched K-means has written piv.
-int front(Plan& pl, Inst& I, hipStream_t h) {
-  ISDF_HIP_TRY(hipMemcpyAsync(pl.piv_host + I.piv_host_off, I.piv_dev, sizeof(int) * (size_t)I.ng,
-                              hipMemcpyDeviceToHost, h));
-  if (int rc = launch_gather_rows(pl.mode, I.psi, I.ld, I.piv_dev, I.rk, I.s1, I.K1, I.kp, I.B, h))
-    return rc;
-  if (int rc = launch_pack_psi(pl.mode, I.psi, I.ld, I.ng, I.s1, I.K1, I.kp, I.A, h)) return rc;
-  return 0;
-}
 
 struct Timer {
   hipEvent_t a = nullptr, b = nullptr;
@@ -357,17 +369,33 @@ struct Timer {
 // lists must already be set for (zv, tv); this form is the one captured.
 int enqueue(Plan& pl, hipStream_t m, bool tune) {
   const int nb = pl.nb, nh = (int)pl.hs.size();
-  hipEvent_t fork = pl.ev[0], zfork = pl.ev[nh + 1], zjoin = pl.ev[nh + 2];
+  hipEvent_t fork = pl.ev[kEvFork], zfork = pl.ev[kEvZFork], zjoin = pl.ev[kEvZJoin];
+  hipEvent_t pack_done = pl.ev[kEvPackDone];
+  // The grid rows of every instance need psi only: with a second helper stream they
+  // are packed there beside the K-means (bandwidth-bound beside latency-bound),
+  // and the piv copies to the host follow on that stream once K-means is done.
+  hipStream_t ps = nh > 1 ? pl.hs[nh - 1] : m;
+  if (nh > 1) {
+    ISDF_HIP_TRY(hipEventRecord(pl.ev[kEvPackFork], m));
+    ISDF_HIP_TRY(hipStreamWaitEvent(ps, pl.ev[kEvPackFork], 0));
+  }
+  if (int rc = launch_pack_batched(pl.mode, pl.d_front, nb, pl.front_pack, ps)) return rc;
+  ISDF_HIP_TRY(hipEventRecord(pack_done, ps));
   // K-means of every instance, each stage one launch over the batch.
   if (int rc = kmeans_batched(pl.d_km, pl.h_km.data(), nb, m)) return rc;
-  ISDF_HIP_TRY(hipEventRecord(fork, m));
-  for (int h = 0; h < nh; h++) ISDF_HIP_TRY(hipStreamWaitEvent(pl.hs[h], fork, 0));
-  for (int i = 0; i < nb; i++)
-    if (int rc = front(pl, pl.inst[i], pl.hs[i % nh])) return rc;
-  for (int h = 0; h < nh; h++) {
-    ISDF_HIP_TRY(hipEventRecord(pl.ev[1 + h], pl.hs[h]));
-    ISDF_HIP_TRY(hipStreamWaitEvent(m, pl.ev[1 + h], 0));
+  if (nh > 1) {
+    ISDF_HIP_TRY(hipEventRecord(fork, m));
+    ISDF_HIP_TRY(hipStreamWaitEvent(ps, fork, 0));
+    for (int i = 0; i < nb; i++) {
+      const Inst& I = pl.inst[i];
+      ISDF_HIP_TRY(hipMemcpyAsync(pl.piv_host + I.piv_host_off, I.piv_dev,
+                                  sizeof(int) * (size_t)I.ng, hipMemcpyDeviceToHost, ps));
+    }
+    ISDF_HIP_TRY(hipEventRecord(pl.ev[kEvPivJoin], ps));
   }
+  // The pivot rows of every instance, one launch.
+  if (int rc = launch_gather_batched(pl.mode, pl.d_front, nb, pl.front_gather, m)) return rc;
+  if (tune) ISDF_HIP_TRY(hipStreamWaitEvent(m, pack_done, 0));  // Z C^T variants run on m
   const int nvar = lp_variants(pl.mode);
   if (tune) {
     Timer tm;
@@ -384,18 +412,20 @@ int enqueue(Plan& pl, hipStream_t m, bool tune) {
   }
   if (tune)
     if (int rc = set_jobs(pl, pl.zv, 0)) return rc;
-  // Helper stream 0: the grouped Z C^T beside the Cholesky chain.
+  // The pivot-row C C^T first, on the whole GPU (a few tens of microseconds); then
+  // helper stream 0 runs the grouped Z C^T beside the rest of the Cholesky chain.
+  if (int rc = launch_zct_lp_grouped(pl.mode, -1, pl.d_pjobs, nb, pl.ptotal, pl.grid, m))
+    return rc;
   ISDF_HIP_TRY(hipEventRecord(zfork, m));
   if (!tune) {
     ISDF_HIP_TRY(hipStreamWaitEvent(pl.hs[0], zfork, 0));
+    ISDF_HIP_TRY(hipStreamWaitEvent(pl.hs[0], pack_done, 0));
     if (int rc = launch_zct_lp_grouped(pl.mode, pl.zv, pl.d_zjobs, nb, pl.ztotal, pl.grid,
                                        pl.hs[0]))
       return rc;
     ISDF_HIP_TRY(hipEventRecord(zjoin, pl.hs[0]));
   }
   // The Cholesky chain of every instance.
-  if (int rc = launch_zct_lp_grouped(pl.mode, -1, pl.d_pjobs, nb, pl.ptotal, pl.grid, m))
-    return rc;
   if (int rc = launch_cct_lp_batched(pl.d_chol, nb, pl.max_rk, m)) return rc;
   if (int rc = launch_clear_info_batched(pl.d_chol, nb, m)) return rc;
   if (int rc = launch_potrf_batched(pl.d_chol, nb, pl.max_rk, m)) return rc;
@@ -419,6 +449,15 @@ int enqueue(Plan& pl, hipStream_t m, bool tune) {
   } else {
     if (int rc = launch_theta_lp_grouped(pl.mode, pl.tv, pl.d_tjobs, nb, pl.ttotal, pl.grid, m))
       return rc;
+  }
+  if (nh > 1) {
+    ISDF_HIP_TRY(hipStreamWaitEvent(m, pl.ev[kEvPivJoin], 0));
+  } else {
+    for (int i = 0; i < nb; i++) {
+      const Inst& I = pl.inst[i];
+      ISDF_HIP_TRY(hipMemcpyAsync(pl.piv_host + I.piv_host_off, I.piv_dev,
+                                  sizeof(int) * (size_t)I.ng, hipMemcpyDeviceToHost, m));
+    }
   }
   hipLaunchKernelGGL(k_batch_status, dim3((nb + 63) / 64), dim3(64), 0, m, pl.d_src, nb,
                      pl.d_status);

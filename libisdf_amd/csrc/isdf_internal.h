@@ -247,6 +247,27 @@ int lp_operand_bytes(int mode);
 int launch_pack_psi(int mode, const double* psi, long long ld, int ng, int c0, int K, int kp,
                     void* out, hipStream_t st);
 // Operand rows nu of psi(piv[nu], c0 + k) in the mode's format.
+// One instance of the batched operand packing: psi columns c0 .. c0 + K - 1 of
+// its grid rows into A (ng x kp) and of its pivot rows into B (rk x kp), in the
+// format of the mode; pack_b0 / gather_b0 are the job's first blocks in the two
+// batched grids (front_blocks gives each job's block counts).
+constexpr int kMaxFrontJobs = 1024;
+struct FrontJob {
+  const double* psi;
+  long long ld;
+  const int* piv;
+  void* A;
+  void* B;
+  int ng, rk, c0, K, kp;
+  int pack_b0, gather_b0;
+};
+int front_blocks(int ng, int rk, int kp, int* pack_blocks, int* gather_blocks);
+// The grid rows of every instance (A; needs psi only) and its pivot rows (B;
+// needs piv), one launch each.
+int launch_pack_batched(int mode, const FrontJob* jobs, int nb, long long blocks,
+                        hipStream_t st);
+int launch_gather_batched(int mode, const FrontJob* jobs, int nb, long long blocks,
+                          hipStream_t st);
 int launch_gather_rows(int mode, const double* psi, long long ld, const int* piv, int rk, int c0,
                        int K, int kp, void* out, hipStream_t st);
 // Zt[g*ldz + nu] = P1(g,nu) P2(g,nu) (FP32), zero for rk <= nu < ldz.

@@ -509,14 +509,15 @@ __device__ __forceinline__ void put_operand(void* out, int kp, long long r, int 
 
 // Packs columns [c0, c0+K) of the FP64 column-major psi (ld) into operand rows
 // g (FP32 or pre-split), zero for K <= k < kp.  64 grid points x 32 states per
-// block, transposed through LDS so that both sides are coalesced.
+// block (block bx, by of the grid cdiv(ng, 64) x cdiv(kp, 32)), transposed
+// through LDS so that both sides are coalesced.
 template <bool SPLIT>
-__global__ __launch_bounds__(256) void k_pack_rows(const double* __restrict__ psi, long long ld,
-                                                   int ng, int c0, int K, int kp,
-                                                   void* __restrict__ out) {
+__device__ __forceinline__ void pack_rows_block(const double* __restrict__ psi, long long ld,
+                                                int ng, int c0, int K, int kp,
+                                                void* __restrict__ out, int bx, int by) {
   __shared__ float tile[32][65];
-  const long long g0 = (long long)blockIdx.x * 64;
-  const int k0 = blockIdx.y * 32;
+  const long long g0 = (long long)bx * 64;
+  const int k0 = by * 32;
   const int t = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < 8; q++) {
@@ -558,16 +559,70 @@ __global__ __launch_bounds__(256) void k_pack_rows(const double* __restrict__ ps
   }
 }
 
-// Operand row nu = psi(piv[nu], c0 + k) (FP32 or pre-split), zero for k >= K.
 template <bool SPLIT>
-__global__ __launch_bounds__(256) void k_gather_rows(const double* __restrict__ psi, long long ld,
-                                                     const int* __restrict__ piv, int rk, int c0,
-                                                     int K, int kp, void* __restrict__ out) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_pack_rows(const double* __restrict__ psi, long long ld,
+                                                   int ng, int c0, int K, int kp,
+                                                   void* __restrict__ out) {
+  pack_rows_block<SPLIT>(psi, ld, ng, c0, K, kp, out, blockIdx.x, blockIdx.y);
+}
+
+// Operand row nu = psi(piv[nu], c0 + k) (FP32 or pre-split), zero for k >= K;
+// element t = nu kp + k.
+template <bool SPLIT>
+__device__ __forceinline__ void gather_rows_elem(const double* __restrict__ psi, long long ld,
+                                                 const int* __restrict__ piv, int rk, int c0,
+                                                 int K, int kp, void* __restrict__ out,
+                                                 long long t) {
   if (t >= (long long)rk * kp) return;
   const int k = (int)(t % kp), nu = (int)(t / kp);
   put_operand<SPLIT>(out, kp, nu, k,
                      k < K ? (float)psi[(long long)(c0 + k) * ld + piv[nu]] : 0.f);
+}
+
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_gather_rows(const double* __restrict__ psi, long long ld,
+                                                     const int* __restrict__ piv, int rk, int c0,
+                                                     int K, int kp, void* __restrict__ out) {
+  gather_rows_elem<SPLIT>(psi, ld, piv, rk, c0, K, kp, out,
+                          (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The job of block b in a batched launch whose jobs start at block b0 (first
+// field read through `start`): the last job with start(j) <= b.  The starts
+// pass through LDS once per block.
+template <class Job, class Start>
+__device__ __forceinline__ int batched_job(const Job* __restrict__ jobs, int nb, int b,
+                                           Start start) {
+  __shared__ int s0[kMaxFrontJobs];
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) s0[j] = start(jobs[j]);
+  __syncthreads();
+  int j = 0;
+  while (j + 1 < nb && s0[j + 1] <= b) j++;
+  return j;
+}
+
+// Every instance's operand rows in two launches (the per-instance
+// launch_pack_psi / launch_gather_rows): the blocks of job j are
+// pack_b0 .. pack_b0 + cdiv(ng, 64) cdiv(kp, 32) - 1 of the pack grid and
+// gather_b0 .. gather_b0 + cdiv(rk kp, 256) - 1 of the gather grid.
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_pack_rows_batched(const FrontJob* __restrict__ jobs,
+                                                           int nb) {
+  const int b = blockIdx.x;
+  const int j = batched_job(jobs, nb, b, [](const FrontJob& f) { return f.pack_b0; });
+  const FrontJob& f = jobs[j];
+  const int lb = b - f.pack_b0, nbx = (f.ng + 63) / 64;
+  pack_rows_block<SPLIT>(f.psi, f.ld, f.ng, f.c0, f.K, f.kp, f.A, lb % nbx, lb / nbx);
+}
+
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_gather_rows_batched(const FrontJob* __restrict__ jobs,
+                                                             int nb) {
+  const int b = blockIdx.x;
+  const int j = batched_job(jobs, nb, b, [](const FrontJob& f) { return f.gather_b0; });
+  const FrontJob& f = jobs[j];
+  gather_rows_elem<SPLIT>(f.psi, f.ld, f.piv, f.rk, f.c0, f.K, f.kp, f.B,
+                          (long long)(b - f.gather_b0) * 256 + threadIdx.x);
 }
 
 // Zt[g][nu] = P1(g, nu) * P2(g, nu) (P2 = P1 when the pair sets agree), for
@@ -923,6 +978,40 @@ int launch_pack_psi(int mode, const double* psi, long long ld, int ng, int c0, i
   else
     hipLaunchKernelGGL(k_pack_rows<false>, grid, dim3(256), 0, st, psi, ld, ng, c0, K, kp, out);
   ISDF_HIP_LAUNCHED("k_pack_rows");
+  return 0;
+}
+
+int front_blocks(int ng, int rk, int kp, int* pack_blocks, int* gather_blocks) {
+  *pack_blocks = cdiv(ng, 64) * cdiv(kp, 32);
+  *gather_blocks = cdiv((long long)rk * kp, 256);
+  return 0;
+}
+
+int launch_pack_batched(int mode, const FrontJob* jobs, int nb, long long blocks,
+                        hipStream_t st) {
+  if (nb > kMaxFrontJobs) return set_error(1, "batched operand packing: too many instances");
+  if (blocks > 0x7fffffffLL) return set_error(1, "batched operand packing: grid too large");
+  if (mode == 2)
+    hipLaunchKernelGGL(k_pack_rows_batched<true>, dim3((unsigned)blocks), dim3(256), 0, st, jobs,
+                       nb);
+  else
+    hipLaunchKernelGGL(k_pack_rows_batched<false>, dim3((unsigned)blocks), dim3(256), 0, st, jobs,
+                       nb);
+  ISDF_HIP_LAUNCHED("k_pack_rows_batched");
+  return 0;
+}
+
+int launch_gather_batched(int mode, const FrontJob* jobs, int nb, long long blocks,
+                          hipStream_t st) {
+  if (nb > kMaxFrontJobs) return set_error(1, "batched operand packing: too many instances");
+  if (blocks > 0x7fffffffLL) return set_error(1, "batched operand packing: grid too large");
+  if (mode == 2)
+    hipLaunchKernelGGL(k_gather_rows_batched<true>, dim3((unsigned)blocks), dim3(256), 0, st,
+                       jobs, nb);
+  else
+    hipLaunchKernelGGL(k_gather_rows_batched<false>, dim3((unsigned)blocks), dim3(256), 0, st,
+                       jobs, nb);
+  ISDF_HIP_LAUNCHED("k_gather_rows_batched");
   return 0;
 }
 
