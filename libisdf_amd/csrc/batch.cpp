@@ -110,17 +110,27 @@ struct Plan {
   long long ptotal = 0, ztotal = 0, ttotal = 0;
   std::vector<hipStream_t> hs;
   std::vector<hipEvent_t> ev;  // kEvCount events (the enum below)
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // Recorded once every piv is in piv_host: the host copies piv out to the caller
+  // while the GPU runs the rest.
+  hipEvent_t piv_ready = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};     // the front and back parts
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipStream_t copy_stream = nullptr;             // piv copies beside the back part
   int device = -1;
 
   void release() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
+    for (int p = 0; p < 2; p++) {
+      if (exec[p]) (void)hipGraphExecDestroy(exec[p]);
+      if (graph[p]) (void)hipGraphDestroy(graph[p]);
+      exec[p] = nullptr;
+      graph[p] = nullptr;
+    }
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+    copy_stream = nullptr;
     for (auto s : hs) (void)hipStreamDestroy(s);
     for (auto e : ev) (void)hipEventDestroy(e);
+    if (piv_ready) (void)hipEventDestroy(piv_ready);
+    piv_ready = nullptr;
     hs.clear();
     ev.clear();
     if (dev) (void)hipFree(dev);
@@ -135,7 +145,7 @@ struct Plan {
 Plan g_plan;
 constexpr int kHelpers = 2;  // helper streams: Z C^T beside the Cholesky chain; the piv copies
 // Events of the schedule.
-enum { kEvFork, kEvPivJoin, kEvPackFork, kEvPackDone, kEvZFork, kEvZJoin, kEvCount };
+enum { kEvFork, kEvPackFork, kEvPackDone, kEvZFork, kEvZJoin, kEvCount };
 
 bool same_batch(const Plan& pl, int nb, const isdf_problem* probs, const double* const* psi,
                 double* const* theta, int dev) {
@@ -300,6 +310,8 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
   for (auto& s : pl.hs) ISDF_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   pl.ev.resize(kEvCount);
   for (auto& ev : pl.ev) ISDF_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  ISDF_HIP_TRY(hipEventCreateWithFlags(&pl.piv_ready, hipEventDisableTiming));
+  ISDF_HIP_TRY(hipStreamCreateWithFlags(&pl.copy_stream, hipStreamNonBlocking));
   return 0;
 }
 
@@ -367,35 +379,46 @@ struct Timer {
 // results are those of the last variant run, identical for every variant:
 // the k order of each sum does not depend on the tile).  Otherwise the job
 // lists must already be set for (zv, tv); this form is the one captured.
-int enqueue(Plan& pl, hipStream_t m, bool tune) {
+// The batch schedule in two parts, each captured as its own graph:
+//  front: the grid rows of every instance (psi only) packed on a helper stream
+//         beside the K-means of every instance;
+//  back:  pivot rows, the contractions, the Cholesky chains, Theta, status.
+// Between them the piv copies to the host run outside the graphs (issue_piv_copies),
+// beside the back part, so that the host can copy piv on to the caller while the
+// GPU runs the rest (a host-waitable event inside a captured graph is not available).
+int enqueue_front(Plan& pl, hipStream_t m) {
   const int nb = pl.nb, nh = (int)pl.hs.size();
-  hipEvent_t fork = pl.ev[kEvFork], zfork = pl.ev[kEvZFork], zjoin = pl.ev[kEvZJoin];
-  hipEvent_t pack_done = pl.ev[kEvPackDone];
-  // The grid rows of every instance need psi only: with a second helper stream they
-  // are packed there beside the K-means (bandwidth-bound beside latency-bound),
-  // and the piv copies to the host follow on that stream once K-means is done.
   hipStream_t ps = nh > 1 ? pl.hs[nh - 1] : m;
   if (nh > 1) {
     ISDF_HIP_TRY(hipEventRecord(pl.ev[kEvPackFork], m));
     ISDF_HIP_TRY(hipStreamWaitEvent(ps, pl.ev[kEvPackFork], 0));
   }
   if (int rc = launch_pack_batched(pl.mode, pl.d_front, nb, pl.front_pack, ps)) return rc;
-  ISDF_HIP_TRY(hipEventRecord(pack_done, ps));
+  if (nh > 1) ISDF_HIP_TRY(hipEventRecord(pl.ev[kEvPackDone], ps));
   // K-means of every instance, each stage one launch over the batch.
   if (int rc = kmeans_batched(pl.d_km, pl.h_km.data(), nb, m)) return rc;
-  if (nh > 1) {
-    ISDF_HIP_TRY(hipEventRecord(fork, m));
-    ISDF_HIP_TRY(hipStreamWaitEvent(ps, fork, 0));
-    for (int i = 0; i < nb; i++) {
-      const Inst& I = pl.inst[i];
-      ISDF_HIP_TRY(hipMemcpyAsync(pl.piv_host + I.piv_host_off, I.piv_dev,
-                                  sizeof(int) * (size_t)I.ng, hipMemcpyDeviceToHost, ps));
-    }
-    ISDF_HIP_TRY(hipEventRecord(pl.ev[kEvPivJoin], ps));
+  if (nh > 1) ISDF_HIP_TRY(hipStreamWaitEvent(m, pl.ev[kEvPackDone], 0));
+  return 0;
+}
+
+// piv of every instance to the pinned staging, on stream s once `after` is done;
+// records pl.piv_ready.
+int issue_piv_copies(Plan& pl, hipStream_t s, hipEvent_t after) {
+  if (after) ISDF_HIP_TRY(hipStreamWaitEvent(s, after, 0));
+  for (int i = 0; i < pl.nb; i++) {
+    const Inst& I = pl.inst[i];
+    ISDF_HIP_TRY(hipMemcpyAsync(pl.piv_host + I.piv_host_off, I.piv_dev,
+                                sizeof(int) * (size_t)I.ng, hipMemcpyDeviceToHost, s));
   }
+  ISDF_HIP_TRY(hipEventRecord(pl.piv_ready, s));
+  return 0;
+}
+
+int enqueue_back(Plan& pl, hipStream_t m, bool tune) {
+  const int nb = pl.nb;
+  hipEvent_t zfork = pl.ev[kEvZFork], zjoin = pl.ev[kEvZJoin];
   // The pivot rows of every instance, one launch.
   if (int rc = launch_gather_batched(pl.mode, pl.d_front, nb, pl.front_gather, m)) return rc;
-  if (tune) ISDF_HIP_TRY(hipStreamWaitEvent(m, pack_done, 0));  // Z C^T variants run on m
   const int nvar = lp_variants(pl.mode);
   if (tune) {
     Timer tm;
@@ -419,7 +442,6 @@ int enqueue(Plan& pl, hipStream_t m, bool tune) {
   ISDF_HIP_TRY(hipEventRecord(zfork, m));
   if (!tune) {
     ISDF_HIP_TRY(hipStreamWaitEvent(pl.hs[0], zfork, 0));
-    ISDF_HIP_TRY(hipStreamWaitEvent(pl.hs[0], pack_done, 0));
     if (int rc = launch_zct_lp_grouped(pl.mode, pl.zv, pl.d_zjobs, nb, pl.ztotal, pl.grid,
                                        pl.hs[0]))
       return rc;
@@ -450,21 +472,21 @@ int enqueue(Plan& pl, hipStream_t m, bool tune) {
     if (int rc = launch_theta_lp_grouped(pl.mode, pl.tv, pl.d_tjobs, nb, pl.ttotal, pl.grid, m))
       return rc;
   }
-  if (nh > 1) {
-    ISDF_HIP_TRY(hipStreamWaitEvent(m, pl.ev[kEvPivJoin], 0));
-  } else {
-    for (int i = 0; i < nb; i++) {
-      const Inst& I = pl.inst[i];
-      ISDF_HIP_TRY(hipMemcpyAsync(pl.piv_host + I.piv_host_off, I.piv_dev,
-                                  sizeof(int) * (size_t)I.ng, hipMemcpyDeviceToHost, m));
-    }
-  }
   hipLaunchKernelGGL(k_batch_status, dim3((nb + 63) / 64), dim3(64), 0, m, pl.d_src, nb,
                      pl.d_status);
   ISDF_HIP_LAUNCHED("k_batch_status");
   ISDF_HIP_TRY(hipMemcpyAsync(pl.status_host, pl.d_status, sizeof(InstStatus) * nb,
                               hipMemcpyDeviceToHost, m));
   return 0;
+}
+
+// piv of every instance from the pinned staging to the caller's buffers (an
+// instance that falls back to the single solve overwrites its own afterwards).
+void copy_piv_out(const Plan& pl, int* const* h_piv) {
+  for (int i = 0; i < pl.nb; i++) {
+    const Inst& I = pl.inst[i];
+    std::memcpy(h_piv[i], pl.piv_host + I.piv_host_off, sizeof(int) * (size_t)I.ng);
+  }
 }
 
 int solve_batched_locked(int nb, const isdf_problem* probs, const double* const* d_psi,
@@ -513,29 +535,44 @@ int solve_batched_locked(int nb, const isdf_problem* probs, const double* const*
     }
   }
   ISDF_HIP_TRY(hipEventRecord(t0, st));
-  if (!pl.exec) {
-    // First call of this shape: eager run with tuning, then capture.
-    if (int rc = enqueue(pl, st, true)) return rc;
+  hipStream_t cps = pl.copy_stream;
+  if (!pl.exec[0]) {
+    // First call of this shape: eager run with tuning, then capture of both parts.
+    if (int rc = enqueue_front(pl, st)) return rc;
+    ISDF_HIP_TRY(hipEventRecord(pl.ev[kEvFork], st));
+    if (int rc = issue_piv_copies(pl, cps, pl.ev[kEvFork])) return rc;
+    if (int rc = enqueue_back(pl, st, true)) return rc;
     ISDF_HIP_TRY(hipEventRecord(t1, st));
     ISDF_HIP_TRY(hipStreamSynchronize(st));
-    hipStream_t cs = nullptr;
-    ISDF_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-    ISDF_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue(pl, cs, false);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(cs, &g);
-    (void)hipStreamDestroy(cs);
-    if (rc) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
+    ISDF_HIP_TRY(hipStreamSynchronize(cps));
+    for (int part = 0; part < 2; part++) {
+      hipStream_t cs = nullptr;
+      ISDF_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+      ISDF_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      int rc = part == 0 ? enqueue_front(pl, cs) : enqueue_back(pl, cs, false);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(cs, &g);
+      (void)hipStreamDestroy(cs);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return set_error(ISDF_EHIP, std::string("graph capture failed: ") +
+                                                           hipGetErrorString(e));
+      pl.graph[part] = g;
+      ISDF_HIP_TRY(hipGraphInstantiate(&pl.exec[part], g, nullptr, nullptr, 0));
     }
-    if (e != hipSuccess) return set_error(ISDF_EHIP, std::string("graph capture failed: ") +
-                                                         hipGetErrorString(e));
-    pl.graph = g;
-    ISDF_HIP_TRY(hipGraphInstantiate(&pl.exec, g, nullptr, nullptr, 0));
+    copy_piv_out(pl, h_piv);
   } else {
-    ISDF_HIP_TRY(hipGraphLaunch(pl.exec, st));
+    ISDF_HIP_TRY(hipGraphLaunch(pl.exec[0], st));
+    ISDF_HIP_TRY(hipEventRecord(pl.ev[kEvFork], st));
+    ISDF_HIP_TRY(hipGraphLaunch(pl.exec[1], st));
     ISDF_HIP_TRY(hipEventRecord(t1, st));
+    // piv reaches pinned memory right after the K-means; it goes on to the caller's
+    // buffers while the GPU runs the contractions and the Cholesky chains.
+    if (int rc = issue_piv_copies(pl, cps, pl.ev[kEvFork])) return rc;
+    ISDF_HIP_TRY(hipEventSynchronize(pl.piv_ready));
+    copy_piv_out(pl, h_piv);
     ISDF_HIP_TRY(hipStreamSynchronize(st));
     replayed = 1;
   }
@@ -553,7 +590,6 @@ int solve_batched_locked(int nb, const isdf_problem* probs, const double* const*
                                  isdf_hip_last_error());
       continue;
     }
-    std::memcpy(h_piv[i], pl.piv_host + I.piv_host_off, sizeof(int) * (size_t)I.ng);
     if (reps) {
       isdf_report& r = reps[i];
       std::memset(&r, 0, sizeof(r));

@@ -90,7 +90,35 @@ def test_batched_matches_single_solves_and_replays(precision):
         for i, a in enumerate(inst):
             np.testing.assert_array_equal(a["d_theta"].cpu().numpy().view(np.uint64),
                                           first[i].view(np.uint64))
+            np.testing.assert_array_equal(a["piv"], single(a, precision)[1])
             assert reps2[i]["kmeans_iters"] == reps[i]["kmeans_iters"]
+
+
+@pytest.mark.parametrize("precision", [isdf.ISDF_PREC_FP32, isdf.ISDF_PREC_BF16X3])
+def test_batched_replay_on_new_data(precision):
+    # The captured graph replayed on other wavefunctions written into the same
+    # buffers (same shapes): piv and Theta must be those of the new data, so
+    # nothing of the previous call (piv staging on the host included) survives.
+    batch = make_batch(41, 5)
+    inst = to_instances(batch)
+    isdf.solve_batched(inst, precision=precision)
+    torch.cuda.synchronize()
+    for seed in (42, 43):
+        rng = np.random.default_rng(seed)
+        for a, b in zip(inst, batch):
+            psi = synth.cholesky_qr2(rng.standard_normal(b["psi"].shape),
+                                     float(np.prod(b["L"])) / b["psi"].shape[1])
+            a["d_psi"].copy_(torch.from_numpy(psi))
+            a["piv"][:] = -1
+        torch.cuda.synchronize()
+        _, brep = isdf.solve_batched(inst, precision=precision)
+        torch.cuda.synchronize()
+        assert brep["graph_replayed"] == 1
+        for i, a in enumerate(inst):
+            th, piv, _ = single(a, precision)
+            np.testing.assert_array_equal(a["piv"], piv, err_msg="seed %d instance %d" % (seed, i))
+            err = np.linalg.norm(a["d_theta"].cpu().numpy() - th) / np.linalg.norm(th)
+            assert err < 1e-12, (seed, i, err)
 
 
 def test_batched_pivots_match_oracle():
