@@ -76,11 +76,12 @@ def buffer_sizes(ng, nstate, rk, ld_psi=0):
     return ld * (max(nstate, 1) - 1) + ng, ng * rk
 
 
-def _dev_ptr(x, need=None, name="buffer"):
+def _dev_ptr(x, need=None, name="buffer", current=None):
     """Device address of x.  For a tensor: on the current HIP device, float64,
     contiguous and holding at least `need` elements (a short buffer would be
     written or read past its end by the kernels).  An int is taken as a raw
-    device address whose size the caller vouches for."""
+    device address whose size the caller vouches for.  `current`: the current
+    device index when the caller already has it."""
     if isinstance(x, int):
         return x
     if hasattr(x, "data_ptr"):
@@ -91,14 +92,25 @@ def _dev_ptr(x, need=None, name="buffer"):
             raise ValueError("%s: float64 buffer expected" % name)
         if not x.is_contiguous():
             raise ValueError("%s: contiguous buffer expected" % name)
-        if x.device.index != torch.cuda.current_device():
+        cur = torch.cuda.current_device() if current is None else current
+        if x.device.index != cur:
             raise ValueError("%s is on device %d, the current device is %d"
-                             % (name, x.device.index, torch.cuda.current_device()))
+                             % (name, x.device.index, cur))
         if need is not None and x.numel() < need:
             raise ValueError("%s holds %d elements, the problem needs %d"
                              % (name, x.numel(), need))
         return x.data_ptr()
     raise TypeError("device buffer must be a torch tensor on the GPU or an int pointer")
+
+
+def _current_device(instances):
+    """The current HIP device index, read once for a batch (None when no
+    instance holds a tensor)."""
+    if any(hasattr(a.get("d_psi"), "data_ptr") or hasattr(a.get("d_theta"), "data_ptr")
+           for a in instances):
+        import torch
+        return torch.cuda.current_device()
+    return None
 
 
 def _host_piv(piv, n, name="piv"):
@@ -243,6 +255,7 @@ def solve_batched(instances, precision=ISDF_PREC_FP32, stream=None):
     report)."""
     L = _lib.lib()
     n = len(instances)
+    cur = _current_device(instances)
     probs = (_lib.IsdfProblem * n)()
     psis = (ctypes.c_void_p * n)()
     thetas = (ctypes.c_void_p * n)()
@@ -254,8 +267,8 @@ def solve_batched(instances, precision=ISDF_PREC_FP32, stream=None):
         piv = _host_piv(a["piv"], ng, "instance %d piv" % i)
         rk = isdf_rank(a["nv1"], a["nc1"], a["nv2"], a["nc2"], a["mu_points"])
         n_psi, n_theta = buffer_sizes(ng, a["nstate"], rk)
-        psis[i] = _dev_ptr(a["d_psi"], n_psi, "instance %d d_psi" % i)
-        thetas[i] = _dev_ptr(a["d_theta"], n_theta, "instance %d d_theta" % i)
+        psis[i] = _dev_ptr(a["d_psi"], n_psi, "instance %d d_psi" % i, cur)
+        thetas[i] = _dev_ptr(a["d_theta"], n_theta, "instance %d d_theta" % i, cur)
         pivs[i] = piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
     reps = (_lib.IsdfReport * n)()
     brep = _lib.IsdfBatchReport()

@@ -587,15 +587,17 @@ __global__ __launch_bounds__(256) void k_gather_rows(const double* __restrict__ 
                           (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// The job of block b in a batched launch whose jobs start at block b0 (first
-// field read through `start`): the last job with start(j) <= b.  The starts
-// pass through LDS once per block.
+// The first blocks of a batched launch's jobs (field read through `start`) into
+// LDS, once per workgroup.
 template <class Job, class Start>
-__device__ __forceinline__ int batched_job(const Job* __restrict__ jobs, int nb, int b,
-                                           Start start) {
-  __shared__ int s0[kMaxFrontJobs];
+__device__ __forceinline__ void load_job_starts(const Job* __restrict__ jobs, int nb, int* s0,
+                                                Start start) {
   for (int j = threadIdx.x; j < nb; j += blockDim.x) s0[j] = start(jobs[j]);
   __syncthreads();
+}
+
+// The job of block b: the last job j with s0[j] <= b.
+__device__ __forceinline__ int job_of(const int* s0, int nb, int b) {
   int j = 0;
   while (j + 1 < nb && s0[j + 1] <= b) j++;
   return j;
@@ -603,24 +605,30 @@ __device__ __forceinline__ int batched_job(const Job* __restrict__ jobs, int nb,
 
 // Every instance's operand rows in two launches (the per-instance
 // launch_pack_psi / launch_gather_rows): the blocks of job j are
-// pack_b0 .. pack_b0 + cdiv(ng, 64) cdiv(kp, 32) - 1 of the pack grid and
-// gather_b0 .. gather_b0 + cdiv(rk kp, 256) - 1 of the gather grid.
+// pack_b0 .. pack_b0 + cdiv(ng, 64) cdiv(kp, 32) - 1 of the pack block list and
+// gather_b0 .. gather_b0 + cdiv(rk kp, 256) - 1 of the gather grid.  The pack
+// runs beside the K-means, so it walks its block list with a bounded grid and
+// leaves room on every CU for the K-means kernels.
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_pack_rows_batched(const FrontJob* __restrict__ jobs,
-                                                           int nb) {
-  const int b = blockIdx.x;
-  const int j = batched_job(jobs, nb, b, [](const FrontJob& f) { return f.pack_b0; });
-  const FrontJob& f = jobs[j];
-  const int lb = b - f.pack_b0, nbx = (f.ng + 63) / 64;
-  pack_rows_block<SPLIT>(f.psi, f.ld, f.ng, f.c0, f.K, f.kp, f.A, lb % nbx, lb / nbx);
+                                                           int nb, int nblocks) {
+  __shared__ int s0[kMaxFrontJobs];
+  load_job_starts(jobs, nb, s0, [](const FrontJob& f) { return f.pack_b0; });
+  for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const FrontJob& f = jobs[job_of(s0, nb, b)];
+    const int lb = b - f.pack_b0, nbx = (f.ng + 63) / 64;
+    pack_rows_block<SPLIT>(f.psi, f.ld, f.ng, f.c0, f.K, f.kp, f.A, lb % nbx, lb / nbx);
+    __syncthreads();  // the block's LDS tile is rewritten by the next one
+  }
 }
 
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_gather_rows_batched(const FrontJob* __restrict__ jobs,
                                                              int nb) {
+  __shared__ int s0[kMaxFrontJobs];
+  load_job_starts(jobs, nb, s0, [](const FrontJob& f) { return f.gather_b0; });
   const int b = blockIdx.x;
-  const int j = batched_job(jobs, nb, b, [](const FrontJob& f) { return f.gather_b0; });
-  const FrontJob& f = jobs[j];
+  const FrontJob& f = jobs[job_of(s0, nb, b)];
   gather_rows_elem<SPLIT>(f.psi, f.ld, f.piv, f.rk, f.c0, f.K, f.kp, f.B,
                           (long long)(b - f.gather_b0) * 256 + threadIdx.x);
 }
@@ -991,12 +999,16 @@ int launch_pack_batched(int mode, const FrontJob* jobs, int nb, long long blocks
                         hipStream_t st) {
   if (nb > kMaxFrontJobs) return set_error(1, "batched operand packing: too many instances");
   if (blocks > 0x7fffffffLL) return set_error(1, "batched operand packing: grid too large");
+  // Two workgroups per CU (256 CUs): enough loads in flight for the bandwidth the
+  // packing needs, without taking every workgroup slot from the K-means beside it.
+  const unsigned grid = (unsigned)std::min<long long>(blocks, 512);
+  if (grid == 0) return 0;
   if (mode == 2)
-    hipLaunchKernelGGL(k_pack_rows_batched<true>, dim3((unsigned)blocks), dim3(256), 0, st, jobs,
-                       nb);
+    hipLaunchKernelGGL(k_pack_rows_batched<true>, dim3(grid), dim3(256), 0, st, jobs, nb,
+                       (int)blocks);
   else
-    hipLaunchKernelGGL(k_pack_rows_batched<false>, dim3((unsigned)blocks), dim3(256), 0, st, jobs,
-                       nb);
+    hipLaunchKernelGGL(k_pack_rows_batched<false>, dim3(grid), dim3(256), 0, st, jobs, nb,
+                       (int)blocks);
   ISDF_HIP_LAUNCHED("k_pack_rows_batched");
   return 0;
 }
