@@ -1577,13 +1577,37 @@ __global__ __launch_bounds__(1024) void k_control(int s, int max_iter, double ma
 // update of iteration s, in one launch queued after the sums of iteration s.
 // Those sums read only the labels of iteration s-1, which the control does not
 // touch; when the control ends the loop, the sums are simply not used.
+// Batched: one block per instance (blockIdx.y), each with its own iteration
+// limit and buffers.
+template <bool BATCH>
 __global__ __launch_bounds__(1024) void k_control_update(
     int s, int max_iter, double maxF, int nblk, const int* __restrict__ changed_blk, int* trace,
     const double* __restrict__ sw, const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ sz, double* __restrict__ cx, double* __restrict__ cy,
     double* __restrict__ cz, int* __restrict__ map, int* scal, int* __restrict__ bbox,
-    int rkmax, const int* __restrict__ cost, int* __restrict__ order) {
+    int rkmax, const int* __restrict__ cost, int* __restrict__ order, const KmJob* jobs) {
   __shared__ int go;
+  if constexpr (BATCH) {
+    const KmJob& J = jobs[blockIdx.y];
+    max_iter = J.max_iter;
+    maxF = J.maxF;
+    nblk = J.b.nblk_pts;
+    changed_blk = J.b.changed_blk;
+    trace = J.b.trace;
+    sw = J.b.sw;
+    sx = J.b.sx;
+    sy = J.b.sy;
+    sz = J.b.sz;
+    cx = J.b.cx;
+    cy = J.b.cy;
+    cz = J.b.cz;
+    map = J.b.map;
+    scal = J.b.scal;
+    bbox = J.b.bbox;
+    rkmax = J.b.rkmax;
+    cost = nullptr;
+    order = nullptr;
+  }
   if (scal[KS_ACTIVE] == 0) return;
   loop_control(s - 1, max_iter, maxF, nblk, changed_blk, trace, scal);
   if (threadIdx.x == 0) go = scal[KS_ACTIVE];
@@ -1991,10 +2015,10 @@ int kmeans_iterate(KmeansBufs* b, int rk, int max_iter, double kmeans_tol, int* 
                          b->grid_mode ? b->bbox : (int*)nullptr, b->rkmax, cost,
                          cost ? b->order : (int*)nullptr, (const KmJob*)nullptr);
     else
-      hipLaunchKernelGGL(k_control_update, dim3(1), dim3(1024), 0, st, s, max_iter, maxF,
+      hipLaunchKernelGGL(k_control_update<false>, dim3(1), dim3(1024), 0, st, s, max_iter, maxF,
                          b->nblk_pts, b->changed_blk, b->trace, b->sw, b->sx, b->sy, b->sz, b->cx,
                          b->cy, b->cz, b->map, b->scal, b->grid_mode ? b->bbox : (int*)nullptr,
-                         b->rkmax, cost, cost ? b->order : (int*)nullptr);
+                         b->rkmax, cost, cost ? b->order : (int*)nullptr, (const KmJob*)nullptr);
     last = s;
     // Grid mode: the super-brick candidate lists first (the bricks cull from
     // them).  The first iterations keep the wide candidate list; their
@@ -2113,16 +2137,26 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
   hipLaunchKernelGGL(k_km_batch_active, dim3(1, nb), dim3(64), 0, st, J);
   ISDF_HIP_LAUNCHED("k-means setup (batched)");
   // Lloyd iterations (utility.cpp:812-922); converged instances return at once.
+  // As in kmeans_iterate: the sums of iteration s, then the loop control of
+  // iteration s-1 with the update of iteration s in one launch, then the
+  // assignment; the control of the last iteration after the loop.
   for (int s = 0; s < maxit; s++) {
     hipLaunchKernelGGL((k_centroid_sums_box<true, 4, 2>), dim3(maxrk, nb), dim3(320), 0, st, 0, 0,
                        (const int*)nullptr, (const double*)nullptr, (const double*)nullptr,
                        (const double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0,
                        (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr,
                        (const int*)nullptr, (int*)nullptr, (const int*)nullptr, J);
-    hipLaunchKernelGGL(k_update<true>, dim3(1, nb), dim3(1024), 0, st, (const double*)nullptr,
-                       (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
-                       (double*)nullptr, (double*)nullptr, (double*)nullptr, (int*)nullptr,
-                       (int*)nullptr, (int*)nullptr, 0, (const int*)nullptr, (int*)nullptr, J);
+    if (s == 0)
+      hipLaunchKernelGGL(k_update<true>, dim3(1, nb), dim3(1024), 0, st, (const double*)nullptr,
+                         (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
+                         (double*)nullptr, (double*)nullptr, (double*)nullptr, (int*)nullptr,
+                         (int*)nullptr, (int*)nullptr, 0, (const int*)nullptr, (int*)nullptr, J);
+    else
+      hipLaunchKernelGGL(k_control_update<true>, dim3(1, nb), dim3(1024), 0, st, s, 0, 0.0, 0,
+                         (const int*)nullptr, (int*)nullptr, (const double*)nullptr,
+                         (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
+                         (double*)nullptr, (double*)nullptr, (double*)nullptr, (int*)nullptr,
+                         (int*)nullptr, (int*)nullptr, 0, (const int*)nullptr, (int*)nullptr, J);
     if (s == 0)
       hipLaunchKernelGGL((k_assign_culled<true, kCandCapWide, kBrickPZ>), dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
@@ -2135,10 +2169,11 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                          (int*)nullptr, (int*)nullptr, 1, (int*)nullptr, 0, 1, 0, 0, 0, 0.0, 0.0, 0.0, (int*)nullptr,
                          (const int*)nullptr, (const double*)nullptr, (const int*)nullptr, 0, J);
-    hipLaunchKernelGGL(k_control<true>, dim3(1, nb), dim3(1024), 0, st, s, 0, 0.0, 0,
-                       (const int*)nullptr, (int*)nullptr, (int*)nullptr, J);
     ISDF_HIP_LAUNCHED("k-means iteration (batched)");
   }
+  if (maxit > 0)
+    hipLaunchKernelGGL(k_control<true>, dim3(1, nb), dim3(1024), 0, st, maxit - 1, 0, 0.0, 0,
+                       (const int*)nullptr, (int*)nullptr, (int*)nullptr, J);
   // Final nearest points, unique(), piv (utility.cpp:926-983).
   hipLaunchKernelGGL(k_final_window<true>, dim3(cdiv(maxrk, 4), nb), dim3(256), 0, st, 0, 0, 0, 0.0,
                      0.0, 0.0, (const double*)nullptr, (const double*)nullptr,
