@@ -1586,7 +1586,6 @@ __global__ __launch_bounds__(1024) void k_control_update(
     const double* __restrict__ sz, double* __restrict__ cx, double* __restrict__ cy,
     double* __restrict__ cz, int* __restrict__ map, int* scal, int* __restrict__ bbox,
     int rkmax, const int* __restrict__ cost, int* __restrict__ order, const KmJob* jobs) {
-  __shared__ int go;
   if constexpr (BATCH) {
     const KmJob& J = jobs[blockIdx.y];
     max_iter = J.max_iter;
@@ -1608,12 +1607,94 @@ __global__ __launch_bounds__(1024) void k_control_update(
     cost = nullptr;
     order = nullptr;
   }
-  if (scal[KS_ACTIVE] == 0) return;
-  loop_control(s - 1, max_iter, maxF, nblk, changed_blk, trace, scal);
-  if (threadIdx.x == 0) go = scal[KS_ACTIVE];
-  __syncthreads();
+  // Every load that does not depend on another goes out first: the flag and
+  // rk, this thread's changed-label counts, and the sums and costs of its
+  // clusters (c = t + 1024 u; indices below rkmax are in the buffers whatever
+  // rk is), so the kernel waits for memory about once instead of once per step.
+  constexpr int PF = 4;
+  const int t = threadIdx.x;
+  const int active = scal[KS_ACTIVE];
+  const int rk = scal[KS_RK];
+  int v = 0;
+  for (int b = t; b < nblk; b += blockDim.x) v += changed_blk[b];
+  const bool pre = rkmax <= PF * (int)blockDim.x;
+  double pw[PF], px[PF], py[PF], pz[PF];
+  int pc[PF];
+#pragma unroll
+  for (int u = 0; u < PF; u++) {
+    const int c = t + u * (int)blockDim.x;
+    const bool ok = pre && c < rkmax;
+    pw[u] = ok ? sw[c] : 1.0;
+    px[u] = ok ? sx[c] : 0.0;
+    py[u] = ok ? sy[c] : 0.0;
+    pz[u] = ok ? sz[c] : 0.0;
+    pc[u] = (ok && cost) ? cost[c] : 0;
+  }
+  if (active == 0) return;
+  // Loop control of iteration s-1 (utility.cpp:813, :906-921).
+  int f;
+  block_excl_scan(v, &f);
+  const bool go = (double)f > maxF && s < max_iter;
+  if (t == 0) {
+    trace[s - 1] = f;
+    scal[KS_ITERS] = s;
+    if (!go) scal[KS_ACTIVE] = 0;
+  }
   if (!go) return;
-  update_centroids(sw, sx, sy, sz, cx, cy, cz, map, scal, bbox, rkmax, cost, order);
+  if (!pre) {
+    update_centroids(sw, sx, sy, sz, cx, cy, cz, map, scal, bbox, rkmax, cost, order);
+    return;
+  }
+  // A cluster that lost every point takes the compaction path, which reads the
+  // sums again; otherwise the update runs from the registers.
+  int empty = 0;
+#pragma unroll
+  for (int u = 0; u < PF; u++) {
+    const int c = t + u * (int)blockDim.x;
+    if (c < rk && pw[u] == 0.0) empty = 1;
+  }
+  if (__syncthreads_or(empty)) {
+    update_centroids(sw, sx, sy, sz, cx, cy, cz, map, scal, bbox, rkmax, cost, order);
+    return;
+  }
+  if (bbox)  // the next assignment rebuilds the cluster boxes
+    for (int e = t; e < 6 * rkmax; e += blockDim.x) bbox[e] = (e < 3 * rkmax) ? 0x7fffffff : -1;
+#pragma unroll
+  for (int u = 0; u < PF; u++) {
+    const int c = t + u * (int)blockDim.x;
+    if (c < rk) {
+      cx[c] = px[u] / pw[u];
+      cy[c] = py[u] / pw[u];
+      cz[c] = pz[u] / pw[u];
+    }
+  }
+  if (t == 0) scal[KS_CHANGED] = 0;
+  if (order) {  // order_by_cost with the costs in registers
+    __shared__ int hist[32];
+    if (t < 32) hist[t] = 0;
+    __syncthreads();
+    int bk[PF];
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+      bk[u] = pc[u] > 0 ? 31 - __clz(pc[u]) : 0;
+      if (t + u * (int)blockDim.x < rk) atomicAdd(&hist[bk[u]], 1);
+    }
+    __syncthreads();
+    if (t == 0) {  // exclusive offsets, bucket 31 first
+      int o = 0;
+      for (int k = 31; k >= 0; k--) {
+        const int hv = hist[k];
+        hist[k] = o;
+        o += hv;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+      const int c = t + u * (int)blockDim.x;
+      if (c < rk) order[atomicAdd(&hist[bk[u]], 1)] = c;
+    }
+  }
 }
 
 // findMin(Dim=0), utility.cpp:926-945: per centroid, the first point at the
