@@ -269,6 +269,13 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
     kj.maxF = 1e-5 * I.ng;
     kj.piv = I.piv_dev;
   }
+  // The batched K-means kernels take job blockIdx.y; every job carries its own
+  // buffers, so the order is free.  Largest clusters (Ng / rk) first: the
+  // centroid sums of those instances are the longest dependent chains, and
+  // workgroups start in grid order.
+  std::stable_sort(pl.h_km.begin(), pl.h_km.end(), [](const KmJob& a, const KmJob& b) {
+    return (double)a.b.n / a.rk > (double)b.b.n / b.rk;
+  });
   size_t o = off_jobs;
   pl.d_chol = (CholJob*)(d + o);
   o += al256(sizeof(CholJob) * nb);
