@@ -102,6 +102,8 @@ struct Plan {
   StatusSrc* d_src = nullptr;
   InstStatus* d_status = nullptr;
   KmJob* d_km = nullptr;
+  int* d_bricks = nullptr;           // (job, brick) of every K-means assignment block
+  int nbricks = 0;
   FrontJob* d_front = nullptr;       // the batched operand packing
   long long front_pack = 0, front_gather = 0;  // its two grids (blocks)
   std::vector<KmJob> h_km;
@@ -170,6 +172,7 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
   // Layout: per instance [K-means | piv | A | B | Zt | Pz | Wf | P | Dinv | D128 | Ft | Gt | W |
   // scalars], then the job arrays.
   size_t off = 0, piv_total = 0;
+  long long bricks = 0;
   std::vector<size_t> base(nb);
   for (int i = 0; i < nb; i++) {
     Inst& I = pl.inst[i];
@@ -201,12 +204,16 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
     off += 256;
     I.piv_host_off = piv_total;
     piv_total += (size_t)I.ng;
+    bricks += kmeans_grid_bricks(probs[i].ngrid);
   }
+  if (bricks > 0x7fffffffLL)
+    return set_error(ISDF_EINVAL, "batch K-means assignment exceeds the int range of blocks");
+  pl.nbricks = (int)bricks;
   const size_t off_jobs = off;
   off += al256(sizeof(CholJob) * nb) + 2 * al256(sizeof(LpGemmJob) * nb) +
          al256(sizeof(LpThetaJob) * nb) + al256(sizeof(StatusSrc) * nb) +
          al256(sizeof(InstStatus) * nb) + al256(sizeof(KmJob) * nb) +
-         al256(sizeof(FrontJob) * nb);
+         al256(sizeof(FrontJob) * nb) + al256(sizeof(int) * 2 * (size_t)pl.nbricks);
   hipError_t e = hipMalloc(&pl.dev, off);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -292,6 +299,21 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
   pl.d_km = (KmJob*)(d + o);
   o += al256(sizeof(KmJob) * nb);
   pl.d_front = (FrontJob*)(d + o);
+  o += al256(sizeof(FrontJob) * nb);
+  pl.d_bricks = (int*)(d + o);
+  {
+    std::vector<int> bl;
+    bl.reserve(2 * (size_t)pl.nbricks);
+    for (int j = 0; j < nb; j++)
+      for (int x = 0; x < pl.h_km[j].b.nblk_pts; x++) {
+        bl.push_back(j);
+        bl.push_back(x);
+      }
+    if (bl.size() != 2 * (size_t)pl.nbricks)
+      return set_error(ISDF_EINVAL, "batch K-means brick count mismatch");
+    ISDF_HIP_TRY(hipMemcpy(pl.d_bricks, bl.data(), sizeof(int) * bl.size(),
+                           hipMemcpyHostToDevice));
+  }
   std::vector<FrontJob> front(nb);
   pl.front_pack = pl.front_gather = 0;
   for (int i = 0; i < nb; i++) {
@@ -403,7 +425,7 @@ int enqueue_front(Plan& pl, hipStream_t m) {
   if (int rc = launch_pack_batched(pl.mode, pl.d_front, nb, pl.front_pack, ps)) return rc;
   if (nh > 1) ISDF_HIP_TRY(hipEventRecord(pl.ev[kEvPackDone], ps));
   // K-means of every instance, each stage one launch over the batch.
-  if (int rc = kmeans_batched(pl.d_km, pl.h_km.data(), nb, m)) return rc;
+  if (int rc = kmeans_batched(pl.d_km, pl.h_km.data(), nb, pl.d_bricks, pl.nbricks, m)) return rc;
   if (nh > 1) ISDF_HIP_TRY(hipStreamWaitEvent(m, pl.ev[kEvPackDone], 0));
   return 0;
 }

@@ -100,8 +100,13 @@ struct KmJob {
 
 // Weights -> K-means -> piv for every instance, each stage one launch over all
 // instances; no host synchronisation (every iteration is queued; converged
-// instances return at once).  h_jobs mirrors d_jobs on the host.
-int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t st);
+// instances return at once).  h_jobs mirrors d_jobs on the host.  d_bricks
+// lists the assignment blocks of every instance, (job, brick) pairs of ints
+// (nbricks pairs: every brick of job 0, then of job 1, ...).
+int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, const int* d_bricks,
+                   int nbricks, hipStream_t st);
+// Assignment blocks (bricks) of a grid whose every point is a K-means point.
+int kmeans_grid_bricks(const int ngrid[3]);
 
 size_t kmeans_bytes(int n, const int ngrid[3], int rkmax, int max_iter);
 void kmeans_carve(KmeansBufs* b, void* base, int n, const int ngrid[3], int rkmax, int max_iter);

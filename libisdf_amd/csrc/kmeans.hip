@@ -665,9 +665,13 @@ __global__ __launch_bounds__(256) void k_assign_culled(
     const double* __restrict__ slxyz, const int* __restrict__ scount, int nsb,
     const KmJob* jobs) {
   __shared__ int cand[CAP];
+  int b = blockIdx.x;  // the block (brick) of the instance
   if constexpr (BATCH) {
-    const KmJob& J = jobs[blockIdx.y];
-    if ((int)blockIdx.x >= J.b.nblk_pts) return;
+    // porder is the batch's brick list: (job, brick) of block x at
+    // porder[2x], porder[2x + 1]; only bricks of the instances are launched.
+    const int2 jb = *reinterpret_cast<const int2*>(porder + 2 * (long long)blockIdx.x);
+    const KmJob& J = jobs[jb.x];
+    b = jb.y;
     porder = J.b.porder;
     X = J.b.X;
     Y = J.b.Y;
@@ -702,7 +706,6 @@ __global__ __launch_bounds__(256) void k_assign_culled(
   const int rk = scal[KS_RK];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int nbx = (n0 + 7) / 8, nby = (n1 + 7) / 8;
-  const int b = blockIdx.x;
   const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);  // grid mode
   int i[PZ];
   if (grid_mode) {
@@ -2005,6 +2008,12 @@ int kmeans_prepare(KmeansBufs* b, const int ngrid[3], const double length[3], do
   return 0;
 }
 
+int kmeans_grid_bricks(const int ngrid[3]) {
+  const int nbx = (ngrid[0] + 7) / 8, nby = (ngrid[1] + 7) / 8;
+  const int nbz = (ngrid[2] + 4 * kBrickPZ - 1) / (4 * kBrickPZ);
+  return nbx * nby * nbz;
+}
+
 void kmeans_geometry(KmeansBufs* b, const int ngrid[3], const double length[3], int npt) {
   for (int d = 0; d < 3; d++) {
     b->ngrid[d] = ngrid[d];
@@ -2012,9 +2021,7 @@ void kmeans_geometry(KmeansBufs* b, const int ngrid[3], const double length[3], 
   }
   b->grid_mode = (npt == b->n);
   if (b->grid_mode) {
-    const int nbx = (ngrid[0] + 7) / 8, nby = (ngrid[1] + 7) / 8;
-    const int nbz = (ngrid[2] + 4 * kBrickPZ - 1) / (4 * kBrickPZ);
-    b->nblk_pts = nbx * nby * nbz;
+    b->nblk_pts = kmeans_grid_bricks(ngrid);
   } else {
     b->nblk_pts = (int)cdiv(b->n, 256);
   }
@@ -2173,17 +2180,21 @@ int kmeans_finish(KmeansBufs* b, int* piv_dev, hipStream_t st, bool nosync) {
   return compact_flags(b, 0, rku, piv_dev, st);
 }
 
-int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t st) {
-  int maxn = 0, maxrk = 0, maxflag = 0, maxpts = 0, maxit = 0;
+int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, const int* d_bricks,
+                   int nbricks, hipStream_t st) {
+  int maxn = 0, maxrk = 0, maxflag = 0, maxit = 0;
+  long long bricks = 0;
   for (int i = 0; i < nb; i++) {
     const KmeansBufs& b = h_jobs[i].b;
     if (!b.grid_mode) return set_error(1, "batched K-means needs every grid point as a point");
     maxn = std::max(maxn, b.n);
     maxrk = std::max(maxrk, b.rkmax);
     maxflag = std::max(maxflag, b.nblk_flag);
-    maxpts = std::max(maxpts, b.nblk_pts);
+    bricks += b.nblk_pts;
     maxit = std::max(maxit, h_jobs[i].max_iter);
   }
+  if (bricks != nbricks || !d_bricks)
+    return set_error(1, "batched K-means: the brick list does not match the instances");
   const KmJob* J = d_jobs;
   const unsigned gn = cdiv(maxn, 256);
   // Weights (getpoints_Kmeans_MPI.cpp:33-45).
@@ -2209,8 +2220,8 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
                      (double*)nullptr, (int*)nullptr, J);
   hipLaunchKernelGGL(k_reset_boxes<true>, dim3(cdiv(6LL * maxrk, 256), nb), dim3(256), 0, st,
                      (int*)nullptr, 0, J);
-  hipLaunchKernelGGL((k_assign_culled<true, kCandCapWide, kBrickPZ>), dim3(maxpts, nb), dim3(256), 0, st,
-                     (const int*)nullptr,
+  hipLaunchKernelGGL((k_assign_culled<true, kCandCapWide, kBrickPZ>), dim3(nbricks), dim3(256), 0, st,
+                     d_bricks,
                      (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                      (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                      (int*)nullptr, (int*)nullptr, 0, (int*)nullptr, 0, 1, 0, 0, 0, 0.0, 0.0, 0.0, (int*)nullptr,
@@ -2239,13 +2250,13 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, hipStream_t
                          (double*)nullptr, (double*)nullptr, (double*)nullptr, (int*)nullptr,
                          (int*)nullptr, (int*)nullptr, 0, (const int*)nullptr, (int*)nullptr, J);
     if (s == 0)
-      hipLaunchKernelGGL((k_assign_culled<true, kCandCapWide, kBrickPZ>), dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
+      hipLaunchKernelGGL((k_assign_culled<true, kCandCapWide, kBrickPZ>), dim3(nbricks), dim3(256), 0, st, d_bricks,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                          (int*)nullptr, (int*)nullptr, 1, (int*)nullptr, 0, 1, 0, 0, 0, 0.0, 0.0, 0.0, (int*)nullptr,
                          (const int*)nullptr, (const double*)nullptr, (const int*)nullptr, 0, J);
     else
-      hipLaunchKernelGGL((k_assign_culled<true, kCandCap, kBrickPZ>), dim3(maxpts, nb), dim3(256), 0, st, (const int*)nullptr,
+      hipLaunchKernelGGL((k_assign_culled<true, kCandCap, kBrickPZ>), dim3(nbricks), dim3(256), 0, st, d_bricks,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
                          (int*)nullptr, (int*)nullptr, 1, (int*)nullptr, 0, 1, 0, 0, 0, 0.0, 0.0, 0.0, (int*)nullptr,
