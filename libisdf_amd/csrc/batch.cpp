@@ -104,6 +104,8 @@ struct Plan {
   KmJob* d_km = nullptr;
   int* d_bricks = nullptr;           // (job, brick) of every K-means assignment block
   int nbricks = 0;
+  int* d_clusters = nullptr;         // (job, cluster) of every centroid-sum block
+  int nclusters = 0;
   FrontJob* d_front = nullptr;       // the batched operand packing
   long long front_pack = 0, front_gather = 0;  // its two grids (blocks)
   std::vector<KmJob> h_km;
@@ -172,7 +174,7 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
   // Layout: per instance [K-means | piv | A | B | Zt | Pz | Wf | P | Dinv | D128 | Ft | Gt | W |
   // scalars], then the job arrays.
   size_t off = 0, piv_total = 0;
-  long long bricks = 0;
+  long long bricks = 0, clusters = 0;
   std::vector<size_t> base(nb);
   for (int i = 0; i < nb; i++) {
     Inst& I = pl.inst[i];
@@ -205,15 +207,18 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
     I.piv_host_off = piv_total;
     piv_total += (size_t)I.ng;
     bricks += kmeans_grid_bricks(probs[i].ngrid);
+    clusters += I.rk;
   }
-  if (bricks > 0x7fffffffLL)
-    return set_error(ISDF_EINVAL, "batch K-means assignment exceeds the int range of blocks");
+  if (bricks > 0x7fffffffLL || clusters > 0x7fffffffLL)
+    return set_error(ISDF_EINVAL, "batch K-means exceeds the int range of blocks");
   pl.nbricks = (int)bricks;
+  pl.nclusters = (int)clusters;
   const size_t off_jobs = off;
   off += al256(sizeof(CholJob) * nb) + 2 * al256(sizeof(LpGemmJob) * nb) +
          al256(sizeof(LpThetaJob) * nb) + al256(sizeof(StatusSrc) * nb) +
          al256(sizeof(InstStatus) * nb) + al256(sizeof(KmJob) * nb) +
-         al256(sizeof(FrontJob) * nb) + al256(sizeof(int) * 2 * (size_t)pl.nbricks);
+         al256(sizeof(FrontJob) * nb) + al256(sizeof(int) * 2 * (size_t)pl.nbricks) +
+         al256(sizeof(int) * 2 * (size_t)pl.nclusters);
   hipError_t e = hipMalloc(&pl.dev, off);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -301,17 +306,28 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
   pl.d_front = (FrontJob*)(d + o);
   o += al256(sizeof(FrontJob) * nb);
   pl.d_bricks = (int*)(d + o);
+  o += al256(sizeof(int) * 2 * (size_t)pl.nbricks);
+  pl.d_clusters = (int*)(d + o);
   {
-    std::vector<int> bl;
+    // Block lists of the batched K-means, job by job in the order of d_km.
+    std::vector<int> bl, cl;
     bl.reserve(2 * (size_t)pl.nbricks);
-    for (int j = 0; j < nb; j++)
+    cl.reserve(2 * (size_t)pl.nclusters);
+    for (int j = 0; j < nb; j++) {
       for (int x = 0; x < pl.h_km[j].b.nblk_pts; x++) {
         bl.push_back(j);
         bl.push_back(x);
       }
-    if (bl.size() != 2 * (size_t)pl.nbricks)
-      return set_error(ISDF_EINVAL, "batch K-means brick count mismatch");
+      for (int c = 0; c < pl.h_km[j].b.rkmax; c++) {
+        cl.push_back(j);
+        cl.push_back(c);
+      }
+    }
+    if (bl.size() != 2 * (size_t)pl.nbricks || cl.size() != 2 * (size_t)pl.nclusters)
+      return set_error(ISDF_EINVAL, "batch K-means block count mismatch");
     ISDF_HIP_TRY(hipMemcpy(pl.d_bricks, bl.data(), sizeof(int) * bl.size(),
+                           hipMemcpyHostToDevice));
+    ISDF_HIP_TRY(hipMemcpy(pl.d_clusters, cl.data(), sizeof(int) * cl.size(),
                            hipMemcpyHostToDevice));
   }
   std::vector<FrontJob> front(nb);
@@ -425,7 +441,8 @@ int enqueue_front(Plan& pl, hipStream_t m) {
   if (int rc = launch_pack_batched(pl.mode, pl.d_front, nb, pl.front_pack, ps)) return rc;
   if (nh > 1) ISDF_HIP_TRY(hipEventRecord(pl.ev[kEvPackDone], ps));
   // K-means of every instance, each stage one launch over the batch.
-  if (int rc = kmeans_batched(pl.d_km, pl.h_km.data(), nb, pl.d_bricks, pl.nbricks, m)) return rc;
+  if (int rc = kmeans_batched(pl.d_km, pl.h_km.data(), nb, pl.d_bricks, pl.nbricks, pl.d_clusters,
+                                pl.nclusters, m)) return rc;
   if (nh > 1) ISDF_HIP_TRY(hipStreamWaitEvent(m, pl.ev[kEvPackDone], 0));
   return 0;
 }

@@ -102,9 +102,10 @@ struct KmJob {
 // instances; no host synchronisation (every iteration is queued; converged
 // instances return at once).  h_jobs mirrors d_jobs on the host.  d_bricks
 // lists the assignment blocks of every instance, (job, brick) pairs of ints
-// (nbricks pairs: every brick of job 0, then of job 1, ...).
+// (nbricks pairs: every brick of job 0, then of job 1, ...); d_clusters the
+// centroid-sum blocks, (job, cluster) pairs (nclusters pairs, rkmax per job).
 int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, const int* d_bricks,
-                   int nbricks, hipStream_t st);
+                   int nbricks, const int* d_clusters, int nclusters, hipStream_t st);
 // Assignment blocks (bricks) of a grid whose every point is a K-means point.
 int kmeans_grid_bricks(const int ngrid[3]);
 

@@ -1019,8 +1019,13 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_centroid_sums_box(
   // zeros up to the next multiple of 32 (the adder takes whole batches).
   __shared__ __align__(16) double buf[4][CH];
   __shared__ __align__(16) int wsum[NP];
+  int cl = blockIdx.x;  // the workgroup's place in the cluster order
   if constexpr (BATCH) {
-    const KmJob& J = jobs[blockIdx.y];
+    // order is the batch's cluster list: (job, cluster) of block x at
+    // order[2x], order[2x + 1] (every cluster of every instance, in index order).
+    const int2 jc = *reinterpret_cast<const int2*>(order + 2 * (long long)blockIdx.x);
+    const KmJob& J = jobs[jc.x];
+    cl = jc.y;
     n0 = J.b.ngrid[0];
     n1 = J.b.ngrid[1];
     label = J.b.label;
@@ -1040,9 +1045,9 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_centroid_sums_box(
   }
   if (scal[KS_ACTIVE] == 0) return;
   const int rk = scal[KS_RK];
-  if ((int)blockIdx.x >= rk) return;
+  if (cl >= rk) return;
   // Workgroup -> cluster: costliest first when an order is given.
-  const int c = order ? order[blockIdx.x] : blockIdx.x;
+  const int c = order ? order[cl] : cl;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const bool adder = wave == 0;
   const int pw = wave - 1;              // producer wave (adder: -1)
@@ -2181,9 +2186,9 @@ int kmeans_finish(KmeansBufs* b, int* piv_dev, hipStream_t st, bool nosync) {
 }
 
 int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, const int* d_bricks,
-                   int nbricks, hipStream_t st) {
+                   int nbricks, const int* d_clusters, int nclusters, hipStream_t st) {
   int maxn = 0, maxrk = 0, maxflag = 0, maxit = 0;
-  long long bricks = 0;
+  long long bricks = 0, clusters = 0;
   for (int i = 0; i < nb; i++) {
     const KmeansBufs& b = h_jobs[i].b;
     if (!b.grid_mode) return set_error(1, "batched K-means needs every grid point as a point");
@@ -2191,10 +2196,11 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, const int* 
     maxrk = std::max(maxrk, b.rkmax);
     maxflag = std::max(maxflag, b.nblk_flag);
     bricks += b.nblk_pts;
+    clusters += b.rkmax;
     maxit = std::max(maxit, h_jobs[i].max_iter);
   }
-  if (bricks != nbricks || !d_bricks)
-    return set_error(1, "batched K-means: the brick list does not match the instances");
+  if (bricks != nbricks || !d_bricks || clusters != nclusters || !d_clusters)
+    return set_error(1, "batched K-means: the block lists do not match the instances");
   const KmJob* J = d_jobs;
   const unsigned gn = cdiv(maxn, 256);
   // Weights (getpoints_Kmeans_MPI.cpp:33-45).
@@ -2233,11 +2239,11 @@ int kmeans_batched(const KmJob* d_jobs, const KmJob* h_jobs, int nb, const int* 
   // iteration s-1 with the update of iteration s in one launch, then the
   // assignment; the control of the last iteration after the loop.
   for (int s = 0; s < maxit; s++) {
-    hipLaunchKernelGGL((k_centroid_sums_box<true, 4, 2>), dim3(maxrk, nb), dim3(320), 0, st, 0, 0,
+    hipLaunchKernelGGL((k_centroid_sums_box<true, 4, 2>), dim3(nclusters), dim3(320), 0, st, 0, 0,
                        (const int*)nullptr, (const double*)nullptr, (const double*)nullptr,
                        (const double*)nullptr, (const double*)nullptr, (const int*)nullptr, 0,
                        (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr,
-                       (const int*)nullptr, (int*)nullptr, (const int*)nullptr, J);
+                       (const int*)nullptr, (int*)nullptr, d_clusters, J);
     if (s == 0)
       hipLaunchKernelGGL(k_update<true>, dim3(1, nb), dim3(1024), 0, st, (const double*)nullptr,
                          (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
