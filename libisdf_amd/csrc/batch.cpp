@@ -2,20 +2,23 @@
 // the mixed-precision forms.  LibISDF has no batched entry point; each
 // instance computes what isdf_hip_solve computes for it.
 //
-// Schedule of one batch (all of it captured once into a hipGraph and replayed
-// on later calls with the same shapes and buffers):
-//   1. weights -> K-means -> piv of every instance, each stage one launch over
-//      the whole batch (blockIdx.y = instance; every Lloyd iteration queued,
-//      converged instances return at once), then per instance the operand
-//      rows of Psi(mu) and psi and the copy of piv to pinned host memory;
-//   2. one persistent grouped launch of the pivot-row contractions (C C^T of
-//      every instance), then the batched Cholesky chain (every launch covers
-//      all instances: blockIdx.z = instance) up to each instance's W;
-//      beside it, on a helper stream, one persistent grouped launch of every
-//      instance's Z C^T;
+// Schedule of one batch (captured once into two hipGraphs and replayed on
+// later calls with the same shapes and buffers):
+//   1. (first graph) weights -> K-means -> piv of every instance, each stage
+//      one launch over the whole batch (every Lloyd iteration queued,
+//      converged instances return at once; the per-iteration kernels take
+//      (job, brick) / (job, cluster) block lists), with the grid rows of psi
+//      of every instance packed beside it on a helper stream;
+//   -- the copies of piv to pinned host memory run between the two graphs on a
+//      side stream, and the host hands piv on while the second graph runs;
+//   2. (second graph) the pivot rows of every instance in one launch, one
+//      persistent grouped launch of the pivot-row contractions (C C^T of every
+//      instance), then the batched Cholesky chain (every launch covers all
+//      instances: blockIdx.z = instance) up to each instance's W; beside it, on
+//      a helper stream, one persistent grouped launch of every instance's Z C^T;
 //   3. one persistent grouped launch of every instance's Theta = Zt W.
 // The first call for a batch shape runs the schedule eagerly and times every
-// tile variant of the two grouped contractions (the autotuner); the graph is
+// tile variant of the two grouped contractions (the autotuner); the graphs are
 // captured with the fastest pair.  K-means is assumed to see every grid point
 // (npt == Ng); an instance where that fails is solved again by the single
 // solve, which handles any npt.
