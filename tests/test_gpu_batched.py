@@ -132,6 +132,27 @@ def test_batched_pivots_match_oracle():
         np.testing.assert_array_equal(a["piv"], km["piv"])
 
 
+def test_batched_per_instance_iteration_limits():
+    # Each instance stops at its own K-means iteration limit (0, 1, 3, 7 and 20
+    # in one batch): the loop control of iteration s-1 fused with the update of
+    # iteration s, and the last control after the loop, must leave every
+    # instance's piv and iteration count as the oracle and its own single
+    # solve give them.
+    batch = make_batch(29, 5)
+    for b, it in zip(batch, (0, 1, 3, 7, 20)):
+        b["iters"] = it
+    inst = to_instances(batch)
+    reps, _ = isdf.solve_batched(inst, precision=isdf.ISDF_PREC_BF16X3)
+    torch.cuda.synchronize()
+    for i, (b, a, rep) in enumerate(zip(batch, inst, reps)):
+        w = oracle.weights(b["psi"], b["nocc"], b["nocc"], b["nocc"], b["nocc"], b["nocc"])
+        km = oracle.kmeans(w, b["mu"], b["iters"], b["L"], b["grid"])
+        np.testing.assert_array_equal(a["piv"], km["piv"], err_msg="instance %d" % i)
+        _, piv, srep = single(a, isdf.ISDF_PREC_BF16X3)
+        np.testing.assert_array_equal(piv, a["piv"], err_msg="instance %d" % i)
+        assert rep["kmeans_iters"] == srep["kmeans_iters"] == km["iters"], (i, rep, srep)
+
+
 def test_batched_ragged_rk_above_one_block():
     # rk 136, 264 and 384 (nstate 34, 66, 96): several 128-wide block columns
     # per instance, so the off-diagonal F / G blocks, the forward-only identity
