@@ -166,6 +166,27 @@ def test_kmeans_maxiter_zero():
     assert fg == pytest.approx(fo, rel=1e-8)
 
 
+def test_kmeans_rk_above_register_update():
+    # rk 4500 on a 24^3 grid: more clusters than the control + update launch
+    # keeps in registers (4 per thread of 1024, rk <= 4096), so every update
+    # takes the general path.  The K-means outputs must still be the oracle's
+    # bit for bit (the full Theta oracle at this rk is too slow on the host).
+    grid, L = (24, 24, 24), (7.68, 7.68, 7.68)
+    ng = grid[0] * grid[1] * grid[2]
+    nstate, nocc, rk, iters = 96, 48, 4500, 10
+    psi = synth.cholesky_qr2(np.random.default_rng(4500).standard_normal((nstate, ng)),
+                             L[0] * L[1] * L[2] / ng)
+    g = gpu_solve(psi, L, grid, nocc, nstate, nocc, nocc, nocc, nocc, rk, iters)
+    w = oracle.weights(psi, nocc, nocc, nocc, nocc, nocc)
+    km = oracle.kmeans(w, rk, iters, L, grid)
+    np.testing.assert_array_equal(g["weights"].view(np.uint64), w.view(np.uint64))
+    np.testing.assert_array_equal(g["trace"], km["trace"])
+    np.testing.assert_array_equal(g["labels"], km["labels"])
+    np.testing.assert_array_equal(g["piv"], km["piv"])
+    assert g["rep"]["rk_kmeans"] == km["rk_kmeans"]
+    assert np.isfinite(g["theta"]).all()
+
+
 def test_sparse_density_degenerate_branch_and_not_spd():
     # psi supported on 20 grid points only: npt = 20 < rk, the reference's
     # npt<rk branch (utility.cpp:667-681) fills piv from the threshold list;
