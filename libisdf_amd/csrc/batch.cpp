@@ -284,7 +284,8 @@ int build_plan(Plan& pl, int nb, const isdf_problem* probs, const double* const*
     kj.maxF = 1e-5 * I.ng;
     kj.piv = I.piv_dev;
   }
-  // The batched K-means kernels take job blockIdx.y; every job carries its own
+  // The batched K-means kernels take their job from blockIdx.y (set-up) or
+  // from the block lists below (Lloyd iterations); every job carries its own
   // buffers, so the order is free.  Largest clusters (Ng / rk) first: the
   // centroid sums of those instances are the longest dependent chains, and
   // workgroups start in grid order.
