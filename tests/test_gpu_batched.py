@@ -153,6 +153,34 @@ def test_batched_per_instance_iteration_limits():
         assert rep["kmeans_iters"] == srep["kmeans_iters"] == km["iters"], (i, rep, srep)
 
 
+@pytest.mark.parametrize("shape", ["one", "tiny"])
+def test_batched_one_instance_and_tiny_instance(shape):
+    # Edge cases of the batch's block lists: a batch of one instance, and an
+    # 8 x 8 x 8 instance (two assignment bricks, rk 32) beside a 20^3 one.
+    rng = np.random.default_rng(808)
+    cases = [((16, 18, 16), 16)] if shape == "one" else [((8, 8, 8), 8), ((20, 20, 20), 16)]
+    batch = []
+    for grid, nstate in cases:
+        L = tuple(0.32 * g for g in grid)
+        ng = grid[0] * grid[1] * grid[2]
+        psi = synth.cholesky_qr2(rng.standard_normal((nstate, ng)), L[0] * L[1] * L[2] / ng)
+        batch.append(dict(psi=psi, grid=grid, L=L, nstate=nstate, nocc=nstate // 2,
+                          mu=8 * (nstate // 2), iters=12))
+    inst = to_instances(batch)
+    isdf.solve_batched(inst, precision=isdf.ISDF_PREC_BF16X3)
+    torch.cuda.synchronize()
+    for i, (b, a) in enumerate(zip(batch, inst)):
+        w = oracle.weights(b["psi"], b["nocc"], b["nocc"], b["nocc"], b["nocc"], b["nocc"])
+        km = oracle.kmeans(w, b["mu"], b["iters"], b["L"], b["grid"])
+        np.testing.assert_array_equal(a["piv"], km["piv"], err_msg="instance %d" % i)
+        th, piv, _ = single(a, isdf.ISDF_PREC_BF16X3)
+        np.testing.assert_array_equal(piv, a["piv"], err_msg="instance %d" % i)
+        th0 = a["d_theta"].cpu().numpy()
+        assert np.isfinite(th0).all()
+        err = np.linalg.norm(th0 - th) / np.linalg.norm(th)
+        assert err < 1e-12, (i, err)
+
+
 def test_batched_ragged_rk_above_one_block():
     # rk 136, 264 and 384 (nstate 34, 66, 96): several 128-wide block columns
     # per instance, so the off-diagonal F / G blocks, the forward-only identity
