@@ -724,9 +724,19 @@ __global__ __launch_bounds__(256) void k_assign_culled(
 #pragma unroll
   for (int p = 0; p < PZ; p++) {
     valid[p] = i[p] >= 0;
-    px[p] = valid[p] ? X[i[p]] : 0.0;
-    py[p] = valid[p] ? Y[i[p]] : 0.0;
-    pz[p] = valid[p] ? Z[i[p]] : 0.0;
+    px[p] = py[p] = pz[p] = 0.0;
+    if (valid[p]) {
+      if (grid_mode) {
+        // Point i is grid point i: its coordinates from the same formula and
+        // operations as k_setup_points (bit for bit what X, Y, Z hold), so no
+        // load waits here; the old label is needed only at the end.
+        grid_coords(i[p], n0, n1, dx0, dx1, dx2, &px[p], &py[p], &pz[p]);
+      } else {
+        px[p] = X[i[p]];
+        py[p] = Y[i[p]];
+        pz[p] = Z[i[p]];
+      }
+    }
     old[p] = (count_changes && valid[p]) ? label[i[p]] : -1;
   }
 
