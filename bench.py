@@ -234,6 +234,9 @@ def main():
                          "and nothing else (a fixed program for rocprofv3 counter passes)")
     ap.add_argument("--select", choices=("kmeans", "qrcp"), default="kmeans",
                     help="interpolation-point selection for --profile-solve")
+    ap.add_argument("--repeat", type=int, default=1, metavar="N",
+                    help="with --profile-solve: run the solve N times back to back (a sustained "
+                         "load for sampling board power and clocks)")
     ap.add_argument("--peak-probe", action="store_true",
                     help="run only register-only throughput loops of the FP64 / FP32 units")
     ap.add_argument("--compare-solve-tiles", default=None, metavar="LIST",
@@ -282,12 +285,17 @@ def main():
         d_theta = torch.empty((rk, w.ng), dtype=torch.float64, device="cuda")
         piv = np.zeros(w.ng, np.int32)
         sel = isdf.ISDF_SELECT_QRCP if a.select == "qrcp" else isdf.ISDF_SELECT_KMEANS
-        rep = isdf.solve(d_psi, isdf.Domain(w.length, w.ngrid), w.nocc, w.nstate, w.nv1, w.nc1,
-                         w.nv2, w.nc2, w.mu_points, d_theta, piv, w.kmeans_iters, precision=prec,
-                         select=sel)
+        t0 = time.perf_counter()
+        for r in range(max(1, a.repeat)):
+            rep = isdf.solve(d_psi, isdf.Domain(w.length, w.ngrid), w.nocc, w.nstate, w.nv1,
+                             w.nc1, w.nv2, w.nc2, w.mu_points, d_theta, piv, w.kmeans_iters,
+                             precision=prec, select=sel)
+            if a.repeat > 1 and (r + 1) % 20 == 0:
+                print("solve %d of %d" % (r + 1, a.repeat), file=sys.stderr, flush=True)
         torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / max(1, a.repeat)
         print(json.dumps({"profile_solve": a.profile_solve, "select": a.select,
-                          "workload": w.name,
+                          "workload": w.name, "repeat": max(1, a.repeat), "ms_per_solve": dt * 1e3,
                           "stage_ms": {k[3:]: v for k, v in rep.items() if k.startswith("ms_")}}))
         return
     w = synth.WORKLOADS[a.workload]
